@@ -1,0 +1,94 @@
+"""ctypes binding of libeigenfns_hip.so (the C ABI in include/eigenfns_hip.h).
+
+The library is the only compute path: if it is missing or cannot be loaded,
+every entry point raises — there is no CPU or framework fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+_LIB_PATH = Path(__file__).resolve().parent / "libeigenfns_hip.so"
+_lock = threading.Lock()
+_lib = None
+
+EFN_OK = 0
+EFN_EINVAL = -1
+EFN_EUNSUPPORTED = -2
+EFN_EHIP = -3
+EFN_ENOMEM = -4
+
+_vp = C.c_void_p
+_i = C.c_int
+_i64 = C.c_int64
+_f = C.c_float
+_d = C.c_double
+_sz = C.c_size_t
+
+_SIGNATURES = {
+    "efn_version": (_i, []),
+    "efn_last_error": (C.c_char_p, []),
+    "efn_grid_supported": (_i, [_i]),
+    "efn_ctx_create": (_i, [_i, _d, _d, _d, _d, _vp, _i, C.POINTER(_vp)]),
+    "efn_ctx_destroy": (_i, [_vp]),
+    "efn_ctx_query": (_i, [_vp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i), C.POINTER(_sz)]),
+    "efn_ctx_basis": (_i, [_vp, _vp, _vp, _vp]),
+    "efn_theta": (_i, [_vp, _vp, _vp, _i, _f, _f, _vp]),
+    "efn_precondition": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "efn_precondition_simple": (_i, [_vp, _vp, _vp, _i, _f, _vp]),
+    "efn_cheb_step": (_i, [_vp, _vp, _vp, _vp, _f, _f, _i, _vp]),
+    "efn_kpm_step": (_i, [_vp, _vp, _vp, _vp, _i, _i, _f, _i, _vp]),
+    "efn_kpm_count_step": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _f, _i, _vp]),
+    "efn_gram_work_bytes": (_sz, [_i, _i, _i64]),
+    "efn_gram": (_i, [_vp, _i, _vp, _i, _i64, _vp, _vp, _sz, _vp]),
+    "efn_combine": (_i, [_vp, _i, _i, _vp, _i64, _vp, _f, _vp, _vp]),
+    "efn_rowreduce_work_bytes": (_sz, [_i, _i64]),
+    "efn_rowreduce": (_i, [_i, _vp, _vp, _i, _i64, _vp, _vp, _sz, _vp]),
+    "efn_row_axpby": (_i, [_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp]),
+    "efn_axpby": (_i, [_vp, _f, _vp, _f, _i64, _vp, _vp]),
+    "efn_random_block": (_i, [_vp, _i, C.c_uint64, _i64, _i, _vp, _vp]),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def lib_path() -> Path:
+    return _LIB_PATH
+
+
+def load():
+    """Load (once) and return the ctypes library handle; raise loudly if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not _LIB_PATH.exists():
+            raise LibraryMissing(
+                f"{_LIB_PATH} is not built; run `python -m eigenfns_amd.build` "
+                "(hipcc, gfx950). eigenfns_amd has no fallback compute path.")
+        lib = C.CDLL(str(_LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc == EFN_OK:
+        return
+    msg = load().efn_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc in (EFN_EINVAL, EFN_EUNSUPPORTED):
+        raise ValueError(text)
+    if rc == EFN_ENOMEM:
+        raise MemoryError(text)
+    raise RuntimeError(text)

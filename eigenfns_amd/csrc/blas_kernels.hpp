@@ -1,0 +1,263 @@
+// Dense block kernels of the eigensolvers: Gram AᴴB, combine Cᵀ·S, row
+// reductions, row-wise linear combinations and the counter-based random blocks.
+//
+// Reference: eigenfns/solver.py:35-57 (gram, combine), :95-126 (deflate,
+// ortho_block), :196-205 and :228-238 (random rows, Rayleigh quotients and
+// residual norms); interior.py:75-82.
+// Gram products of FP32 inputs are exact in FP64 and summed in FP64, so the
+// Gram carries no FP32 accumulation error (the compensated Gram the SVQB drop
+// threshold and the normalised Rayleigh quotient rely on).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace efn {
+
+// ---------------------------------------------------------------------------
+// Gram partials: part[s][i][j] = Σ_{k in split s} conj(A[i][k])·B[j][k]
+constexpr int kGT = 64;   // output tile edge
+constexpr int kGK = 32;   // k-chunk staged in LDS
+constexpr int kGPad = kGT + 1;
+
+__global__ __launch_bounds__(256) void gram_partial_kernel(const float2* __restrict__ A, int ma,
+                                                           const float2* __restrict__ B, int mb,
+                                                           long long n, long long kper,
+                                                           double2* __restrict__ part) {
+  __shared__ float2 As[kGK * kGPad];
+  __shared__ float2 Bs[kGK * kGPad];
+  const int tid = threadIdx.x;
+  const int ti = tid >> 4, tj = tid & 15;
+  const int i0 = blockIdx.y * kGT, j0 = blockIdx.z * kGT;
+  const long long kb = (long long)blockIdx.x * kper;
+  const long long ke = kb + kper < n ? kb + kper : n;
+  double ar[4][4], ai[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ar[q][r] = ai[q][r] = 0.0;
+  for (long long k0 = kb; k0 < ke; k0 += kGK) {
+    // stage A[i0.., k0..] and B[j0.., k0..] transposed into LDS: As[k][i]
+#pragma unroll
+    for (int s = 0; s < (kGT * kGK) / 256; ++s) {
+      const int e = tid + 256 * s;
+      const int row = e / kGK, col = e - row * kGK;
+      const long long k = k0 + col;
+      float2 va = make_float2(0.f, 0.f), vb = make_float2(0.f, 0.f);
+      if (k < ke) {
+        if (i0 + row < ma) va = A[(long long)(i0 + row) * n + k];
+        if (j0 + row < mb) vb = B[(long long)(j0 + row) * n + k];
+      }
+      As[col * kGPad + row] = va;
+      Bs[col * kGPad + row] = vb;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < kGK; ++k) {
+      double axr[4], axi[4], bxr[4], bxi[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float2 a = As[k * kGPad + ti + 16 * q];
+        const float2 b = Bs[k * kGPad + tj + 16 * q];
+        axr[q] = a.x; axi[q] = a.y; bxr[q] = b.x; bxi[q] = b.y;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          // conj(a)·b = (ar·br + ai·bi) + i(ar·bi − ai·br)
+          ar[q][r] = fma(axr[q], bxr[r], fma(axi[q], bxi[r], ar[q][r]));
+          ai[q][r] = fma(axr[q], bxi[r], fma(-axi[q], bxr[r], ai[q][r]));
+        }
+    }
+    __syncthreads();
+  }
+  double2* out = part + (long long)blockIdx.x * ma * mb;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + ti + 16 * q, j = j0 + tj + 16 * r;
+      if (i < ma && j < mb) out[(long long)i * mb + j] = make_double2(ar[q][r], ai[q][r]);
+    }
+}
+
+// out[e] = Σ_s part[s][e] in fixed order
+__global__ void sum_partials_kernel(const double2* __restrict__ part, int nsplit, long long count,
+                                    double2* __restrict__ out) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  double re = 0.0, im = 0.0;
+  for (int s = 0; s < nsplit; ++s) {
+    const double2 p = part[(long long)s * count + e];
+    re += p.x;
+    im += p.y;
+  }
+  out[e] = make_double2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// combine: out[i][k] = alpha·X[i][k] + Σ_j C[j][i]·S[j][k]   (C is m_in × m_out, c64)
+template <int TI>
+__global__ __launch_bounds__(256) void combine_kernel(const float2* __restrict__ C, int m_in, int m_out,
+                                                      const float2* __restrict__ S, long long n,
+                                                      const float2* __restrict__ X, float alpha,
+                                                      float2* __restrict__ out) {
+  const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int i0 = blockIdx.y * TI;
+  if (k >= n) return;
+  float2 acc[TI];
+#pragma unroll
+  for (int i = 0; i < TI; ++i) acc[i] = make_float2(0.f, 0.f);
+  for (int j = 0; j < m_in; ++j) {
+    const float2 s = S[(long long)j * n + k];
+    const float2* crow = C + (long long)j * m_out + i0;
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+      if (i0 + i < m_out) {
+        const float2 c = crow[i];
+        acc[i].x = fmaf(c.x, s.x, fmaf(-c.y, s.y, acc[i].x));
+        acc[i].y = fmaf(c.x, s.y, fmaf(c.y, s.x, acc[i].y));
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TI; ++i) {
+    if (i0 + i < m_out) {
+      float2 r = acc[i];
+      if (X != nullptr) {
+        const float2 x = X[(long long)(i0 + i) * n + k];
+        r.x = fmaf(alpha, x.x, r.x);
+        r.y = fmaf(alpha, x.y, r.y);
+      }
+      out[(long long)(i0 + i) * n + k] = r;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row reductions in FP64 over splits: part[i][s] = Σ_k f(A[i][k], B[i][k])
+// op 0: |A|²  (re only)   op 1: conj(A)·B (re, im)
+__device__ __forceinline__ double2 wave_sum2(double2 v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_xor(v.x, off, 64);
+    v.y += __shfl_xor(v.y, off, 64);
+  }
+  return v;
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void rowreduce_partial_kernel(const float2* __restrict__ A,
+                                                                const float2* __restrict__ B,
+                                                                long long n, long long kper,
+                                                                double2* __restrict__ part) {
+  __shared__ double2 red[4];
+  const int i = blockIdx.y, s = blockIdx.x, nsplit = gridDim.x;
+  const long long kb = (long long)s * kper;
+  const long long ke = kb + kper < n ? kb + kper : n;
+  const float2* a = A + (long long)i * n;
+  const float2* b = B + (long long)i * n;
+  double re = 0.0, im = 0.0;
+  for (long long k = kb + threadIdx.x; k < ke; k += 256) {
+    const float2 x = a[k];
+    if (OP == 0) {
+      re += (double)x.x * x.x + (double)x.y * x.y;
+    } else {
+      const float2 y = b[k];
+      re += (double)x.x * y.x + (double)x.y * y.y;
+      im += (double)x.x * y.y - (double)x.y * y.x;
+    }
+  }
+  double2 v = wave_sum2(make_double2(re, im));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 t = red[0];
+    for (int w = 1; w < 4; ++w) { t.x += red[w].x; t.y += red[w].y; }
+    part[(long long)i * nsplit + s] = t;
+  }
+}
+
+__global__ void rowreduce_final_kernel(const double2* __restrict__ part, int nsplit, int m,
+                                       double2* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double re = 0.0, im = 0.0;
+  for (int s = 0; s < nsplit; ++s) {
+    re += part[(long long)i * nsplit + s].x;
+    im += part[(long long)i * nsplit + s].y;
+  }
+  out[i] = make_double2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// out[i][k] = a[i]·X[i][k] + b[i]·Y[i][k] (a, b real per row, FP64 on device,
+// applied in FP32).  Y may be null.
+__global__ void row_axpby_kernel(const float2* __restrict__ X, const double* __restrict__ a,
+                                 const float2* __restrict__ Y, const double* __restrict__ b,
+                                 long long n, float2* __restrict__ out) {
+  const int i = blockIdx.y;
+  const float ai = (float)a[i];
+  const float bi = Y ? (float)b[i] : 0.f;
+  const long long base = (long long)i * n;
+  for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
+    const float2 x = X[base + k];
+    float2 r = make_float2(ai * x.x, ai * x.y);
+    if (Y) {
+      const float2 y = Y[base + k];
+      r.x = fmaf(bi, y.x, r.x);
+      r.y = fmaf(bi, y.y, r.y);
+    }
+    out[base + k] = r;
+  }
+}
+
+// out = a·X + b·Y with scalar coefficients over a flat range
+__global__ void axpby_kernel(const float2* __restrict__ X, float a, const float2* __restrict__ Y,
+                             float b, long long count, float2* __restrict__ out) {
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < count; e += (long long)gridDim.x * 256) {
+    const float2 x = X[e];
+    float2 r = make_float2(a * x.x, a * x.y);
+    if (Y) {
+      const float2 y = Y[e];
+      r.x = fmaf(b, y.x, r.x);
+      r.y = fmaf(b, y.y, r.y);
+    }
+    out[e] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based random blocks (splitmix64 stream + Box–Muller / Rademacher),
+// multiplied by the Γ mask (zero at the singular voxel).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void random_block_kernel(int kind, uint64_t key, long long row0, long long G3,
+                                    long long count, long long sing_voxel,
+                                    float2* __restrict__ out) {
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < count; e += (long long)gridDim.x * 256) {
+    const uint64_t ge = (uint64_t)row0 * (uint64_t)(2 * G3) + (uint64_t)e;
+    const uint64_t h = splitmix64(key + ge * 0x9E3779B97F4A7C15ull);
+    float2 z;
+    if (kind == 0) {
+      const double u1 = ((double)(h >> 40) + 0.5) * 0x1p-24;
+      const double u2 = ((double)((h >> 16) & 0xFFFFFFull) + 0.5) * 0x1p-24;
+      const double r = sqrt(-2.0 * log(u1));
+      z = make_float2((float)(r * cos(2.0 * 3.14159265358979323846 * u2)),
+                      (float)(r * sin(2.0 * 3.14159265358979323846 * u2)));
+    } else {
+      z = make_float2((h >> 63) ? -1.f : 1.f, 0.f);
+    }
+    const long long vox = e % G3;
+    if (vox == sing_voxel) z = make_float2(0.f, 0.f);
+    out[e] = z;
+  }
+}
+
+}  // namespace efn

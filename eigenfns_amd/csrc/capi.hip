@@ -1,0 +1,662 @@
+// C ABI of libeigenfns_hip.so: operator contexts and kernel dispatch.
+// Declarations and reference citations: include/eigenfns_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/eigenfns_hip.h"
+#include "blas_kernels.hpp"
+#include "operator_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define EFN_HIP_CHECK(expr)                                                              \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      return fail(EFN_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));          \
+  } while (0)
+
+#define EFN_LAUNCH_CHECK()                                                               \
+  do {                                                                                   \
+    hipError_t _e = hipGetLastError();                                                   \
+    if (_e != hipSuccess)                                                                \
+      return fail(EFN_EHIP, std::string("kernel launch: ") + hipGetErrorString(_e));     \
+  } while (0)
+
+using efn::EpiArgs;
+using efn::FrameTables;
+
+struct OpRequest {
+  int kind;              // efn::kTheta / kPrecond
+  int epi;               // efn::Epi
+  const float2* in;      // T_k (operator input)
+  int m;
+  EpiArgs ea;            // pointers for row 0; advanced per chunk
+  int reduce_mode;       // 0 none, 1 KPM moments, 2 KPM count
+  double* mom;           // KPM: moments base
+  int ld;
+  int k;
+  double* est;           // KPM count
+  double ck64;
+};
+
+}  // namespace
+
+struct efn_ctx {
+  int G = 0;
+  double L = 0.0;
+  double k[3] = {0, 0, 0};
+  int chunk = 0;
+  long long sing_voxel = -1;
+  float kn_floor = 0.f;
+  double eps_mean_inv = 0.0;
+  float* d_axes = nullptr;       // kx | ky | kz  (3G floats)
+  uint32_t* d_swap = nullptr;    // G³ bits
+  float* d_inv_eps = nullptr;    // ε⁻¹/G³
+  float* d_eps = nullptr;        // ε/G³
+  float* d_kn2 = nullptr;        // |k+G|² (for the diagonal preconditioner)
+  float2* d_twid = nullptr;      // exp(−2πi e/G)
+  float2* d_ws = nullptr;        // chunk·3·G³ complex64
+  double* d_partial = nullptr;   // chunk·(G²/R)·2
+  double2* d_red = nullptr;      // chunk·2 doubles of reduced sums
+  size_t ws_bytes = 0;
+  int (*run)(efn_ctx*, const OpRequest&, hipStream_t) = nullptr;
+
+  FrameTables tables() const {
+    FrameTables ft;
+    ft.kx = d_axes;
+    ft.ky = d_axes + G;
+    ft.kz = d_axes + 2 * G;
+    ft.swap = d_swap;
+    ft.sing_voxel = sing_voxel;
+    ft.kn_floor = kn_floor;
+    return ft;
+  }
+};
+
+namespace {
+
+// reduce per-block partials of one chunk and apply the KPM formulas
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nb, int nvec,
+                                       int mode, double* __restrict__ mom, int ld, int k,
+                                       double* __restrict__ est, double ck) {
+  __shared__ double red[8];
+  const int v = blockIdx.x;
+  double s0 = 0.0, s1 = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    s0 += partial[2 * ((long long)v * nb + b)];
+    s1 += partial[2 * ((long long)v * nb + b) + 1];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = s0;
+    red[4 + (threadIdx.x >> 6)] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = red[0] + red[1] + red[2] + red[3];
+    double t1 = red[4] + red[5] + red[6] + red[7];
+    if (mode == 1) {
+      double* row = mom + (long long)v * ld;
+      row[2 * k] = 2.0 * t0 - row[0];
+      row[2 * k + 1] = 2.0 * t1 - row[1];
+    } else if (mode == 2) {
+      est[v] += ck * t0;
+    }
+  }
+}
+
+template <int G>
+int run_op(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
+  using namespace efn;
+  constexpr int R = zrows_for(G);
+  constexpr long long G3 = (long long)G * G * G;
+  constexpr int nzt = (G + ztile_for(G) - 1) / ztile_for(G);
+  constexpr int nb = G * G / R;  // zin/zout blocks per vector
+  const FrameTables ft = c->tables();
+  const float* rmul = rq.kind == kTheta ? c->d_inv_eps : c->d_eps;
+  for (int v0 = 0; v0 < rq.m; v0 += c->chunk) {
+    const int nv = rq.m - v0 < c->chunk ? rq.m - v0 : c->chunk;
+    const long long off = (long long)v0 * 2 * G3;
+    const float2* x = rq.in + off;
+    const dim3 gz((unsigned)(nv * nb)), gs((unsigned)(nv * 3 * G * nzt)), blk(kThreads);
+    if (rq.kind == kTheta)
+      hipLaunchKernelGGL((zin_kernel<G, kTheta>), gz, blk, 0, s, x, c->d_ws, nv, ft, c->d_twid);
+    else
+      hipLaunchKernelGGL((zin_kernel<G, kPrecond>), gz, blk, 0, s, x, c->d_ws, nv, ft, c->d_twid);
+    EFN_LAUNCH_CHECK();
+    hipLaunchKernelGGL((strided_kernel<G, 1, +1, false>), gs, blk, 0, s, c->d_ws, rmul, c->d_twid);
+    hipLaunchKernelGGL((strided_kernel<G, 0, +1, true>), gs, blk, 0, s, c->d_ws, rmul, c->d_twid);
+    hipLaunchKernelGGL((strided_kernel<G, 1, -1, false>), gs, blk, 0, s, c->d_ws, rmul, c->d_twid);
+    EFN_LAUNCH_CHECK();
+    EpiArgs ea = rq.ea;
+    ea.x = rq.in + off;
+    ea.out = rq.ea.out + off;
+    if (rq.ea.y) ea.y = rq.ea.y + off;
+    if (rq.ea.z) ea.z = rq.ea.z + off;
+    ea.partial = c->d_partial;
+    if (rq.kind == kPrecond) {
+      hipLaunchKernelGGL((zout_kernel<G, kPrecond, kStore>), gz, blk, 0, s, c->d_ws, nv, ft, c->d_twid, ea);
+    } else {
+      switch (rq.epi) {
+        case kStore:
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kStore>), gz, blk, 0, s, c->d_ws, nv, ft, c->d_twid, ea);
+          break;
+        case kCheb:
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kCheb>), gz, blk, 0, s, c->d_ws, nv, ft, c->d_twid, ea);
+          break;
+        case kKpm:
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kKpm>), gz, blk, 0, s, c->d_ws, nv, ft, c->d_twid, ea);
+          break;
+        default:
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kDot>), gz, blk, 0, s, c->d_ws, nv, ft, c->d_twid, ea);
+          break;
+      }
+    }
+    EFN_LAUNCH_CHECK();
+    if (rq.reduce_mode) {
+      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nv), dim3(256), 0, s, c->d_partial, nb, nv,
+                         rq.reduce_mode, rq.mom ? rq.mom + (long long)v0 * rq.ld : nullptr, rq.ld,
+                         rq.k, rq.est ? rq.est + v0 : nullptr, rq.ck64);
+      EFN_LAUNCH_CHECK();
+    }
+  }
+  return EFN_OK;
+}
+
+struct GridEntry {
+  int G;
+  int (*run)(efn_ctx*, const OpRequest&, hipStream_t);
+};
+
+#define EFN_GRID(g) {g, &run_op<g>}
+const GridEntry kGrids[] = {EFN_GRID(4),   EFN_GRID(6),   EFN_GRID(8),   EFN_GRID(12),
+                            EFN_GRID(16),  EFN_GRID(24),  EFN_GRID(32),  EFN_GRID(48),
+                            EFN_GRID(64),  EFN_GRID(96),  EFN_GRID(128), EFN_GRID(192),
+                            EFN_GRID(256)};
+#undef EFN_GRID
+
+const GridEntry* find_grid(int G) {
+  for (const auto& e : kGrids)
+    if (e.G == G) return &e;
+  return nullptr;
+}
+
+int zrows_runtime(int G) { return efn::zrows_for(G); }
+
+template <class T>
+void free_dev(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void destroy(efn_ctx* c) {
+  free_dev(c->d_axes);
+  free_dev(c->d_swap);
+  free_dev(c->d_inv_eps);
+  free_dev(c->d_eps);
+  free_dev(c->d_kn2);
+  free_dev(c->d_twid);
+  free_dev(c->d_ws);
+  free_dev(c->d_partial);
+  free_dev(c->d_red);
+  delete c;
+}
+
+int gram_nsplit(int ma, int mb, long long n) {
+  const long long tiles = (long long)((ma + efn::kGT - 1) / efn::kGT) * ((mb + efn::kGT - 1) / efn::kGT);
+  long long want = 1024 / (tiles > 0 ? tiles : 1);
+  long long maxsplit = (n + 4095) / 4096;  // at least 4096 entries per split
+  long long ns = want < maxsplit ? want : maxsplit;
+  return (int)(ns < 1 ? 1 : ns);
+}
+
+int rowreduce_nsplit(long long n) {
+  long long ns = (n + 65535) / 65536;
+  return (int)(ns < 1 ? 1 : (ns > 256 ? 256 : ns));
+}
+
+}  // namespace
+
+extern "C" {
+
+int efn_version(void) { return 100; }
+
+const char* efn_last_error(void) { return g_last_error.c_str(); }
+
+int efn_grid_supported(int grid_size) { return find_grid(grid_size) != nullptr; }
+
+int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float* eps_host,
+                   int max_chunk, efn_ctx** out) {
+  if (!out) return fail(EFN_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!eps_host) return fail(EFN_EINVAL, "eps is NULL");
+  if (!(L > 0.0)) return fail(EFN_EINVAL, "box_size must be positive");
+  const GridEntry* ge = find_grid(G);
+  if (!ge) return fail(EFN_EUNSUPPORTED, "grid size " + std::to_string(G) + " has no compiled kernels");
+  efn_ctx* c = new efn_ctx();
+  c->G = G;
+  c->L = L;
+  c->k[0] = kx; c->k[1] = ky; c->k[2] = kz;
+  c->run = ge->run;
+  const long long G3 = (long long)G * G * G;
+  // --- FP64 axis values exactly as operator.py:47-49 (2π·fftfreq(G, L/G) + 2π·k/L)
+  std::vector<double> ax[3];
+  const double val = 1.0 / (G * (L / G));
+  for (int a = 0; a < 3; ++a) {
+    ax[a].resize(G);
+    const double k0 = 2 * M_PI * c->k[a] / L;
+    for (int i = 0; i < G; ++i) {
+      const int f = i < (G + 1) / 2 ? i : i - G;  // fftfreq order
+      ax[a][i] = 2 * M_PI * (f * val) + k0;
+    }
+  }
+  std::vector<float> axes(3 * G);
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < G; ++i) axes[a * G + i] = (float)ax[a][i];
+  // --- swap bits (operator.py:57, FP64) and singular voxel; FP32 |k+G| for kn_floor
+  std::vector<uint32_t> swap((G3 + 31) / 32, 0u);
+  std::vector<float> kn2f(G3);
+  float knmin = INFINITY;
+  long long sing = -1;
+  for (int x = 0; x < G; ++x)
+    for (int y = 0; y < G; ++y)
+      for (int z = 0; z < G; ++z) {
+        const long long v = ((long long)x * G + y) * G + z;
+        const double KX = ax[0][x], KY = ax[1][y], KZ = ax[2][z];
+        const double kn = std::sqrt((KX * KX + KY * KY) + KZ * KZ);
+        const double knc = kn > 1e-300 ? kn : 1e-300;
+        if (std::fabs(KZ) > 0.9 * knc) swap[v >> 5] |= 1u << (v & 31);
+        if (KX == 0.0 && KY == 0.0 && KZ == 0.0) sing = v;
+        const float knf = (float)kn;
+        kn2f[v] = knf * knf;
+        if (knf > 0.f && knf < knmin) knmin = knf;
+      }
+  c->sing_voxel = sing;
+  c->kn_floor = 0.5f * knmin;
+  // --- ε tables
+  std::vector<float> inv_eps(G3), epsv(G3);
+  double sum_inv = 0.0;
+  const double inv_g3 = 1.0 / (double)G3;
+  for (long long v = 0; v < G3; ++v) {
+    const double e = (double)eps_host[v];
+    if (!(e > 0.0)) {
+      destroy(c);
+      return fail(EFN_EINVAL, "eps must be positive everywhere");
+    }
+    const float inv32 = 1.0f / eps_host[v];   // FP32 division as operator.py:146
+    sum_inv += 1.0 / e;
+    inv_eps[v] = (float)((double)inv32 * inv_g3);
+    epsv[v] = (float)(e * inv_g3);
+  }
+  c->eps_mean_inv = sum_inv / (double)G3;
+  std::vector<float2> twid(G);
+  for (int e = 0; e < G; ++e)
+    twid[e] = make_float2((float)std::cos(2 * M_PI * e / G), (float)-std::sin(2 * M_PI * e / G));
+  // --- chunk: default keeps the workspace near 1 GiB (at least one vector)
+  const size_t per_vec = (size_t)3 * G3 * sizeof(float2);
+  int chunk = max_chunk;
+  if (chunk <= 0) {
+    chunk = (int)((size_t(1) << 30) / per_vec);
+    if (chunk < 1) chunk = 1;
+    if (chunk > 64) chunk = 64;
+  }
+  c->chunk = chunk;
+  c->ws_bytes = per_vec * chunk;
+  const int nb = G * G / zrows_runtime(G);
+#define EFN_ALLOC(ptr, bytes)                                                         \
+  do {                                                                                \
+    if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess) {                           \
+      (void)hipGetLastError();                                                        \
+      destroy(c);                                                                     \
+      return fail(EFN_ENOMEM, "hipMalloc of " + std::to_string((size_t)(bytes)) + " bytes failed"); \
+    }                                                                                 \
+  } while (0)
+  EFN_ALLOC(c->d_axes, axes.size() * sizeof(float));
+  EFN_ALLOC(c->d_swap, swap.size() * sizeof(uint32_t));
+  EFN_ALLOC(c->d_inv_eps, G3 * sizeof(float));
+  EFN_ALLOC(c->d_eps, G3 * sizeof(float));
+  EFN_ALLOC(c->d_kn2, G3 * sizeof(float));
+  EFN_ALLOC(c->d_twid, G * sizeof(float2));
+  EFN_ALLOC(c->d_ws, c->ws_bytes);
+  EFN_ALLOC(c->d_partial, (size_t)chunk * nb * 2 * sizeof(double));
+  EFN_ALLOC(c->d_red, (size_t)chunk * sizeof(double2));
+#undef EFN_ALLOC
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMemcpy(c->d_axes, axes.data(), axes.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_swap, swap.data(), swap.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_inv_eps, inv_eps.data(), G3 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_eps, epsv.data(), G3 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_kn2, kn2f.data(), G3 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_twid, twid.data(), G * sizeof(float2), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    destroy(c);
+    return fail(EFN_EHIP, std::string("upload of operator tables: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return EFN_OK;
+}
+
+int efn_ctx_destroy(efn_ctx* ctx) {
+  if (ctx) destroy(ctx);
+  return EFN_OK;
+}
+
+int efn_ctx_query(const efn_ctx* c, double* kn_floor, double* eps_mean_inv, int* chunk,
+                  size_t* workspace_bytes) {
+  if (!c) return fail(EFN_EINVAL, "ctx is NULL");
+  if (kn_floor) *kn_floor = c->kn_floor;
+  if (eps_mean_inv) *eps_mean_inv = c->eps_mean_inv;
+  if (chunk) *chunk = c->chunk;
+  if (workspace_bytes) *workspace_bytes = c->ws_bytes;
+  return EFN_OK;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void basis_kernel(efn::FrameTables ft, int G, float* __restrict__ kn, float* __restrict__ t) {
+  const long long G3 = (long long)G * G * G;
+  for (long long v = (long long)blockIdx.x * 256 + threadIdx.x; v < G3; v += (long long)gridDim.x * 256) {
+    const int x = (int)(v / ((long long)G * G));
+    const int y = (int)((v / G) % G);
+    const int z = (int)(v % G);
+    const efn::Frame f = efn::make_frame(ft, x, y, z, v);
+    if (kn) kn[v] = f.kn;
+    if (t) {
+      t[0 * G3 + v] = f.t1x; t[1 * G3 + v] = f.t1y; t[2 * G3 + v] = f.t1z;
+      t[3 * G3 + v] = f.t2x; t[4 * G3 + v] = f.t2y; t[5 * G3 + v] = f.t2z;
+    }
+  }
+}
+
+__global__ void precond_simple_kernel(const float2* __restrict__ in, const float* __restrict__ kn2,
+                                      long long G3, long long count, float scale,
+                                      float2* __restrict__ out) {
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < count; e += (long long)gridDim.x * 256) {
+    const float d = 1.0f / (kn2[e % G3] + scale);
+    const float2 v = in[e];
+    out[e] = make_float2(v.x * d, v.y * d);
+  }
+}
+
+unsigned grid_for(long long count) {
+  long long b = (count + 255) / 256;
+  if (b > 65536) b = 65536;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+}  // namespace
+
+extern "C" {
+
+int efn_ctx_basis(const efn_ctx* c, float* kn_dev, float* t_dev, void* stream) {
+  if (!c) return fail(EFN_EINVAL, "ctx is NULL");
+  const long long G3 = (long long)c->G * c->G * c->G;
+  hipLaunchKernelGGL(basis_kernel, dim3(grid_for(G3)), dim3(256), 0, (hipStream_t)stream,
+                     c->tables(), c->G, kn_dev, t_dev);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+static int check_block(const efn_ctx* c, const void* p, int m, const char* what) {
+  if (!c) return fail(EFN_EINVAL, "ctx is NULL");
+  if (m < 0) return fail(EFN_EINVAL, "negative block size");
+  if (m > 0 && !p) return fail(EFN_EINVAL, std::string(what) + " is NULL");
+  return EFN_OK;
+}
+
+int efn_theta(efn_ctx* c, const void* in, void* out, int m, float alpha, float beta, void* stream) {
+  int rc;
+  if ((rc = check_block(c, in, m, "in")) || (rc = check_block(c, out, m, "out"))) return rc;
+  if (m == 0) return EFN_OK;
+  if (in == out) return fail(EFN_EINVAL, "in and out must not alias");
+  OpRequest rq{};
+  rq.kind = efn::kTheta;
+  rq.epi = efn::kStore;
+  rq.in = (const float2*)in;
+  rq.m = m;
+  rq.ea.alpha = alpha;
+  rq.ea.beta = beta;
+  rq.ea.out = (float2*)out;
+  return c->run(c, rq, (hipStream_t)stream);
+}
+
+int efn_precondition(efn_ctx* c, const void* in, void* out, int m, void* stream) {
+  int rc;
+  if ((rc = check_block(c, in, m, "in")) || (rc = check_block(c, out, m, "out"))) return rc;
+  if (m == 0) return EFN_OK;
+  if (in == out) return fail(EFN_EINVAL, "in and out must not alias");
+  OpRequest rq{};
+  rq.kind = efn::kPrecond;
+  rq.epi = efn::kStore;
+  rq.in = (const float2*)in;
+  rq.m = m;
+  rq.ea.alpha = 1.f;
+  rq.ea.beta = 0.f;
+  rq.ea.out = (float2*)out;
+  return c->run(c, rq, (hipStream_t)stream);
+}
+
+int efn_precondition_simple(efn_ctx* c, const void* in, void* out, int m, float scale, void* stream) {
+  int rc;
+  if ((rc = check_block(c, in, m, "in")) || (rc = check_block(c, out, m, "out"))) return rc;
+  if (m == 0) return EFN_OK;
+  const long long G3 = (long long)c->G * c->G * c->G;
+  const long long count = (long long)m * 2 * G3;
+  hipLaunchKernelGGL(precond_simple_kernel, dim3(grid_for(count)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)in, c->d_kn2, G3, count, scale, (float2*)out);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+int efn_cheb_step(efn_ctx* c, const void* T1, void* T0, void* Y, float alpha, float ck, int m,
+                  void* stream) {
+  int rc;
+  if ((rc = check_block(c, T1, m, "T1")) || (rc = check_block(c, T0, m, "T0")) ||
+      (rc = check_block(c, Y, m, "Y")))
+    return rc;
+  if (m == 0) return EFN_OK;
+  if (T1 == T0 || T1 == Y || T0 == Y) return fail(EFN_EINVAL, "T1, T0 and Y must be distinct");
+  OpRequest rq{};
+  rq.kind = efn::kTheta;
+  rq.epi = efn::kCheb;
+  rq.in = (const float2*)T1;
+  rq.m = m;
+  rq.ea.alpha = alpha;
+  rq.ea.ck = ck;
+  rq.ea.out = (float2*)T0;
+  rq.ea.y = (float2*)Y;
+  return c->run(c, rq, (hipStream_t)stream);
+}
+
+int efn_kpm_step(efn_ctx* c, const void* T1, void* T0, double* mom, int ld, int k, float alpha,
+                 int m, void* stream) {
+  int rc;
+  if ((rc = check_block(c, T1, m, "T1")) || (rc = check_block(c, T0, m, "T0"))) return rc;
+  if (m == 0) return EFN_OK;
+  if (!mom || k < 1 || 2 * k + 1 >= ld) return fail(EFN_EINVAL, "bad moment buffer / index");
+  if (T1 == T0) return fail(EFN_EINVAL, "T1 and T0 must be distinct");
+  OpRequest rq{};
+  rq.kind = efn::kTheta;
+  rq.epi = efn::kKpm;
+  rq.in = (const float2*)T1;
+  rq.m = m;
+  rq.ea.alpha = alpha;
+  rq.ea.out = (float2*)T0;
+  rq.reduce_mode = 1;
+  rq.mom = mom;
+  rq.ld = ld;
+  rq.k = k;
+  return c->run(c, rq, (hipStream_t)stream);
+}
+
+int efn_kpm_count_step(efn_ctx* c, const void* T1, void* T0, const void* Z, double* est, double ck,
+                       float alpha, int m, void* stream) {
+  int rc;
+  if ((rc = check_block(c, T1, m, "T1")) || (rc = check_block(c, T0, m, "T0")) ||
+      (rc = check_block(c, Z, m, "Z")))
+    return rc;
+  if (m == 0) return EFN_OK;
+  if (!est) return fail(EFN_EINVAL, "est is NULL");
+  if (T1 == T0) return fail(EFN_EINVAL, "T1 and T0 must be distinct");
+  OpRequest rq{};
+  rq.kind = efn::kTheta;
+  rq.epi = efn::kDot;
+  rq.in = (const float2*)T1;
+  rq.m = m;
+  rq.ea.alpha = alpha;
+  rq.ea.out = (float2*)T0;
+  rq.ea.z = (const float2*)Z;
+  rq.reduce_mode = 2;
+  rq.est = est;
+  rq.ck64 = ck;
+  return c->run(c, rq, (hipStream_t)stream);
+}
+
+size_t efn_gram_work_bytes(int ma, int mb, int64_t n) {
+  if (ma <= 0 || mb <= 0 || n <= 0) return 0;
+  return (size_t)gram_nsplit(ma, mb, n) * ma * mb * sizeof(double2);
+}
+
+int efn_gram(const void* A, int ma, const void* B, int mb, int64_t n, void* out, void* work,
+             size_t work_bytes, void* stream) {
+  if (ma < 0 || mb < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
+  if (ma == 0 || mb == 0) return EFN_OK;
+  if (!A || !B || !out) return fail(EFN_EINVAL, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (n == 0) {
+    EFN_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)ma * mb * sizeof(double2), s));
+    return EFN_OK;
+  }
+  const size_t need = efn_gram_work_bytes(ma, mb, n);
+  if (!work || work_bytes < need) return fail(EFN_EINVAL, "gram work buffer too small");
+  const int ns = gram_nsplit(ma, mb, n);
+  long long kper = (n + ns - 1) / ns;
+  kper = (kper + efn::kGK - 1) / efn::kGK * efn::kGK;
+  const dim3 g((unsigned)ns, (unsigned)((ma + efn::kGT - 1) / efn::kGT), (unsigned)((mb + efn::kGT - 1) / efn::kGT));
+  hipLaunchKernelGGL(efn::gram_partial_kernel, g, dim3(256), 0, s, (const float2*)A, ma,
+                     (const float2*)B, mb, (long long)n, kper, (double2*)work);
+  EFN_LAUNCH_CHECK();
+  const long long count = (long long)ma * mb;
+  hipLaunchKernelGGL(efn::sum_partials_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+                     (const double2*)work, ns, count, (double2*)out);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+int efn_combine(const void* C, int m_in, int m_out, const void* S, int64_t n, const void* X,
+                float alpha, void* out, void* stream) {
+  if (m_in < 0 || m_out < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
+  if (m_out == 0 || n == 0) return EFN_OK;
+  if (!out || (m_in > 0 && (!C || !S))) return fail(EFN_EINVAL, "NULL pointer");
+  if (out == S) return fail(EFN_EINVAL, "out must not alias S");
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned gx = (unsigned)((n + 255) / 256);
+  if (m_out <= 8) {
+    hipLaunchKernelGGL((efn::combine_kernel<8>), dim3(gx, (m_out + 7) / 8), dim3(256), 0, s,
+                       (const float2*)C, m_in, m_out, (const float2*)S, (long long)n,
+                       (const float2*)X, alpha, (float2*)out);
+  } else {
+    hipLaunchKernelGGL((efn::combine_kernel<16>), dim3(gx, (m_out + 15) / 16), dim3(256), 0, s,
+                       (const float2*)C, m_in, m_out, (const float2*)S, (long long)n,
+                       (const float2*)X, alpha, (float2*)out);
+  }
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+size_t efn_rowreduce_work_bytes(int m, int64_t n) {
+  if (m <= 0 || n <= 0) return 0;
+  return (size_t)rowreduce_nsplit(n) * m * sizeof(double2);
+}
+
+int efn_rowreduce(int op, const void* A, const void* B, int m, int64_t n, void* out, void* work,
+                  size_t work_bytes, void* stream) {
+  if (op != 0 && op != 1) return fail(EFN_EINVAL, "op must be 0 or 1");
+  if (m < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
+  if (m == 0) return EFN_OK;
+  if (!A || !out || (op == 1 && !B)) return fail(EFN_EINVAL, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  if (n == 0) {
+    EFN_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)m * sizeof(double2), s));
+    return EFN_OK;
+  }
+  const size_t need = efn_rowreduce_work_bytes(m, n);
+  if (!work || work_bytes < need) return fail(EFN_EINVAL, "rowreduce work buffer too small");
+  const int ns = rowreduce_nsplit(n);
+  const long long kper = (n + ns - 1) / ns;
+  if (op == 0)
+    hipLaunchKernelGGL((efn::rowreduce_partial_kernel<0>), dim3(ns, m), dim3(256), 0, s,
+                       (const float2*)A, (const float2*)A, (long long)n, kper, (double2*)work);
+  else
+    hipLaunchKernelGGL((efn::rowreduce_partial_kernel<1>), dim3(ns, m), dim3(256), 0, s,
+                       (const float2*)A, (const float2*)B, (long long)n, kper, (double2*)work);
+  EFN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(efn::rowreduce_final_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
+                     (const double2*)work, ns, m, (double2*)out);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+int efn_row_axpby(const void* X, const double* a, const void* Y, const double* b, int m, int64_t n,
+                  void* out, void* stream) {
+  if (m < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
+  if (m == 0 || n == 0) return EFN_OK;
+  if (!X || !a || !out || (Y && !b)) return fail(EFN_EINVAL, "NULL pointer");
+  long long gx = (n + 255) / 256;
+  if (gx > 4096) gx = 4096;
+  hipLaunchKernelGGL(efn::row_axpby_kernel, dim3((unsigned)gx, (unsigned)m), dim3(256), 0,
+                     (hipStream_t)stream, (const float2*)X, a, (const float2*)Y, b, (long long)n,
+                     (float2*)out);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+int efn_axpby(const void* X, float a, const void* Y, float b, int64_t count, void* out, void* stream) {
+  if (count < 0) return fail(EFN_EINVAL, "negative size");
+  if (count == 0) return EFN_OK;
+  if (!X || !out) return fail(EFN_EINVAL, "NULL pointer");
+  hipLaunchKernelGGL(efn::axpby_kernel, dim3(grid_for(count)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)X, a, (const float2*)Y, b, (long long)count, (float2*)out);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+int efn_random_block(const efn_ctx* c, int kind, uint64_t seed, int64_t row0, int m, void* out,
+                     void* stream) {
+  int rc;
+  if ((rc = check_block(c, out, m, "out"))) return rc;
+  if (kind != 0 && kind != 1) return fail(EFN_EINVAL, "kind must be 0 (normal) or 1 (rademacher)");
+  if (row0 < 0) return fail(EFN_EINVAL, "negative row offset");
+  if (m == 0) return EFN_OK;
+  const long long G3 = (long long)c->G * c->G * c->G;
+  const long long count = (long long)m * 2 * G3;
+  const uint64_t key = [](uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }(seed);
+  hipLaunchKernelGGL(efn::random_block_kernel, dim3(grid_for(count)), dim3(256), 0, (hipStream_t)stream,
+                     kind, key, (long long)row0, G3, count, c->sing_voxel, (float2*)out);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+}  // extern "C"

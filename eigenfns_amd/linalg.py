@@ -1,0 +1,210 @@
+"""Block linear algebra on the device: Gram, combine, deflation, SVQB, row reductions.
+
+Kernels: libeigenfns_hip.so (efn_gram / efn_combine / efn_rowreduce /
+efn_row_axpby).  Small dense FP64 problems (Gram eigendecompositions,
+Rayleigh–Ritz) run in FP64 on the GPU through torch.linalg (rocSOLVER), or on
+the host when EFN_SMALL_DENSE=host.
+
+Reference: eigenfns/solver.py:27-126 and the row reductions of
+solver.py:228-238 / interior.py:75-82.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._device import SCRATCH, as_block, device, stream_ptr
+
+SVQB_DROP = 1e-5        # solver.py:27 — relative Gram-eigenvalue drop threshold
+RR_DEAD_PENALTY = 1e4   # solver.py:28
+DEFLATE_CHUNK = 128     # solver.py:60
+
+
+def _flat_n(X: torch.Tensor) -> int:
+    return int(X[0].numel()) if X.shape[0] else int(np.prod(X.shape[1:]))
+
+
+def _small_dense_on_host() -> bool:
+    return os.environ.get("EFN_SMALL_DENSE", "gpu").lower() == "host"
+
+
+def gram(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
+    """⟨A_i, B_j⟩ as an (m_a, m_b) complex128 device tensor (solver.py:35-43).
+
+    Products of the FP32 inputs are exact in FP64 and summed in FP64."""
+    A = as_block(A)
+    B = as_block(B)
+    ma, mb = int(A.shape[0]), int(B.shape[0])
+    n = _flat_n(A)
+    if mb and _flat_n(B) != n:
+        raise ValueError("gram: row lengths differ")
+    out = torch.empty((ma, mb), dtype=torch.complex128, device=A.device)
+    if ma == 0 or mb == 0:
+        return out
+    lib = _lib.load()
+    need = lib.efn_gram_work_bytes(ma, mb, n)
+    work = SCRATCH.get(need, "gram")
+    _lib.check(lib.efn_gram(A.data_ptr(), ma, B.data_ptr(), mb, n, out.data_ptr(),
+                            work.data_ptr(), work.numel(), stream_ptr()), "gram")
+    return out
+
+
+def combine_into(C: torch.Tensor, S: torch.Tensor, out: torch.Tensor, X=None,
+                 alpha: float = 0.0) -> torch.Tensor:
+    """out_i = alpha·X_i + Σ_j C[j, i]·S_j with S one contiguous block (m_in rows)."""
+    C = C.to(device=S.device, dtype=torch.complex64).contiguous()
+    m_in, m_out = int(C.shape[0]), int(C.shape[1])
+    if int(S.shape[0]) != m_in:
+        raise ValueError(f"combine: C has {m_in} rows but the block has {int(S.shape[0])}")
+    n = _flat_n(S) if m_in else _flat_n(out)
+    _lib.check(_lib.load().efn_combine(C.data_ptr(), m_in, m_out, S.data_ptr(), n,
+                                       None if X is None else X.data_ptr(), float(alpha),
+                                       out.data_ptr(), stream_ptr()), "combine")
+    return out
+
+
+def combine(C, *blocks) -> torch.Tensor:
+    """Linear combination out_i = Σ_j C[j, i]·S_j over the concatenated blocks (solver.py:49-57)."""
+    blocks = [as_block(b) for b in blocks]
+    C = torch.as_tensor(C).to(device=blocks[0].device)
+    m_out = int(C.shape[1])
+    shape = (m_out,) + tuple(blocks[0].shape[1:])
+    out = torch.empty(shape, dtype=torch.complex64, device=blocks[0].device)
+    off = 0
+    for i, Bk in enumerate(blocks):
+        r = int(Bk.shape[0])
+        combine_into(C[off:off + r], Bk, out, X=out if i else None, alpha=1.0 if i else 0.0)
+        off += r
+    return out
+
+
+def rownorm2(X: torch.Tensor) -> torch.Tensor:
+    """Σ|x|² per row in FP64 (device float64 tensor)."""
+    return _rowreduce(0, X, None).real
+
+
+def rowdot(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
+    """⟨a_i, b_i⟩ per row in FP64 (device complex128 tensor)."""
+    return _rowreduce(1, A, B)
+
+
+def _rowreduce(op, A, B):
+    A = as_block(A)
+    m, n = int(A.shape[0]), _flat_n(A)
+    out = torch.empty(m, dtype=torch.complex128, device=A.device)
+    if m == 0:
+        return out
+    lib = _lib.load()
+    need = lib.efn_rowreduce_work_bytes(m, n)
+    work = SCRATCH.get(need, "rowred")
+    _lib.check(lib.efn_rowreduce(op, A.data_ptr(), None if B is None else as_block(B).data_ptr(),
+                                 m, n, out.data_ptr(), work.data_ptr(), work.numel(),
+                                 stream_ptr()), "rowreduce")
+    return out
+
+
+def rownorm(X) -> torch.Tensor:
+    return torch.sqrt(rownorm2(X))
+
+
+def row_axpby(X, a, Y=None, b=None, out=None) -> torch.Tensor:
+    """out_i = a_i·X_i + b_i·Y_i with per-row coefficients (FP64 device tensors)."""
+    X = as_block(X)
+    m, n = int(X.shape[0]), _flat_n(X)
+    a = torch.as_tensor(a, dtype=torch.float64, device=X.device).reshape(-1).contiguous()
+    if Y is not None:
+        b = torch.as_tensor(b, dtype=torch.float64, device=X.device).reshape(-1).contiguous()
+    if out is None:
+        out = torch.empty_like(X)
+    _lib.check(_lib.load().efn_row_axpby(X.data_ptr(), a.data_ptr(),
+                                         None if Y is None else Y.data_ptr(),
+                                         None if Y is None else b.data_ptr(), m, n,
+                                         out.data_ptr(), stream_ptr()), "row_axpby")
+    return out
+
+
+def axpby(X, a: float, Y=None, b: float = 0.0, out=None) -> torch.Tensor:
+    """out = a·X + b·Y elementwise (complex64)."""
+    if out is None:
+        out = torch.empty_like(X)
+    _lib.check(_lib.load().efn_axpby(X.data_ptr(), float(a), None if Y is None else Y.data_ptr(),
+                                     float(b), X.numel(), out.data_ptr(), stream_ptr()), "axpby")
+    return out
+
+
+def normalize_rows(X, floor: float = 1e-30, out=None) -> torch.Tensor:
+    """X_i / max(‖X_i‖, floor) (solver.py:204-205, :306-307)."""
+    s = 1.0 / torch.clamp(rownorm(X), min=floor)
+    return row_axpby(X, s, out=out)
+
+
+# ---- small dense FP64 ------------------------------------------------------
+def eigh(A: torch.Tensor):
+    """Hermitian eigendecomposition in FP64 (ascending), on the GPU by default."""
+    A = (A + A.conj().T) / 2
+    if _small_dense_on_host():
+        w, V = np.linalg.eigh(A.detach().cpu().numpy())
+        return (torch.as_tensor(w, device=A.device), torch.as_tensor(V, device=A.device))
+    return torch.linalg.eigh(A)
+
+
+def svqb_coeffs(Gm: torch.Tensor, drop: float = SVQB_DROP) -> torch.Tensor:
+    """V·w^{-1/2} with directions below drop·max(w) zeroed (solver.py:119-124)."""
+    w, V = eigh(Gm)
+    wmax = torch.clamp(w.max(), min=1e-300)
+    keep = w > drop * wmax
+    scale = torch.where(keep, 1.0 / torch.sqrt(torch.where(keep, w, torch.ones_like(w))),
+                        torch.zeros_like(w))
+    return V * scale[None, :]
+
+
+def deflate_chunk_rows(grid_size: int, target_bytes: float = 0.75e9) -> int:
+    """Rows per deflation chunk (solver.py:85-92).  The device-resident locked
+    set makes chunking unnecessary; kept for API compatibility."""
+    return max(16, int(target_bytes // (2 * grid_size ** 3 * 8)))
+
+
+def deflate(X, locked, chunk_rows: int = DEFLATE_CHUNK, n_locked: int | None = None):
+    """X ← X − L(LᴴX) against the locked rows (solver.py:95-110), one pass.
+
+    The locked set is device-resident (no host streaming); `chunk_rows` only
+    bounds the Gram size per call."""
+    if locked is None:
+        return X
+    L = locked if n_locked is None else locked[:n_locked]
+    if isinstance(L, np.ndarray):
+        L = as_block(L)
+    if L.shape[0] == 0:
+        return X
+    X = as_block(X)
+    out = X
+    for s in range(0, int(L.shape[0]), chunk_rows):
+        Lc = L[s:s + chunk_rows]
+        Cg = gram(Lc, out)
+        res = torch.empty_like(X) if out is X else out
+        combine_into(-Cg, Lc, res, X=out, alpha=1.0)
+        out = res
+    return out
+
+
+def ortho_block(X, passes: int = 2, drop: float = SVQB_DROP):
+    """SVQB orthonormalisation, FP64 Gram algebra, `passes` sweeps (solver.py:113-126).
+
+    Rank-dropped directions come back as zero rows."""
+    X = as_block(X)
+    for _ in range(passes):
+        Cw = svqb_coeffs(gram(X, X), drop)
+        Y = torch.empty_like(X)
+        combine_into(Cw, X, Y)
+        X = Y
+    return X
+
+
+def apply_chunked(fn, X, rows: int):
+    """Apply a per-vector operator in sub-chunks of `rows` vectors (solver.py:74-82)."""
+    if X.shape[0] <= rows:
+        return fn(X)
+    return torch.cat([fn(X[s:s + rows]) for s in range(0, X.shape[0], rows)], dim=0)

@@ -1,0 +1,272 @@
+"""Deflated block LOBPCG for the transverse Maxwell operator, device-resident.
+
+Drop-in for eigenfns.solver (reference eigenfns/solver.py).  The algorithm is
+the reference's (solver.py:136-382): guard warm start, re-deflation every
+iteration, SVQB with the 1e-5 drop threshold, fixed-shape P whitening,
+fresh Θ(W), Θ(P) for the Rayleigh–Ritz matrix, dead-row penalty, lock the
+lowest m−guard pairs, monotone final sort.  What changes is where things
+live: [X; W; P] and [ΘX; ΘW; ΘP] are two contiguous device blocks, so the
+Rayleigh–Ritz matrix is one Gram kernel and each update one combine kernel;
+the locked set stays in HBM (no host streaming); the small FP64 algebra runs
+on the device.  Rayleigh quotients are normalised by ‖x‖² (the reference's
+interior.py:69-74 fix) unless ``normalized_rq=False``.
+"""
+from __future__ import annotations
+
+import gc
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .linalg import (DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_DROP, apply_chunked, combine,
+                     combine_into, deflate, deflate_chunk_rows, eigh, gram, normalize_rows,
+                     ortho_block, row_axpby, rowdot, rownorm, rownorm2, svqb_coeffs)
+
+__all__ = ["gram", "combine", "deflate", "ortho_block", "apply_chunked", "deflate_chunk_rows",
+           "SolveStats", "lobpcg_blocks", "lobpcg_blocks_resumable", "_flat",
+           "_SVQB_DROP", "_RR_DEAD_PENALTY", "_DEFLATE_CHUNK"]
+
+_SVQB_DROP = SVQB_DROP
+_RR_DEAD_PENALTY = RR_DEAD_PENALTY
+_DEFLATE_CHUNK = DEFLATE_CHUNK
+
+
+def _flat(X):
+    return X.reshape(X.shape[0], -1)
+
+
+@dataclass
+class SolveStats:
+    theta_applications: int = 0
+    rounds: list = field(default_factory=list)
+    wall_seconds: float = 0.0
+    precond_applications: int = 0
+    iterations: int = 0
+
+
+def _deflate_inplace(Y, L, nl):
+    """Y ← Y − L(LᴴY) against the first nl locked rows, in place."""
+    if L is None or nl == 0:
+        return Y
+    Lc = L[:nl]
+    Cg = gram(Lc, Y)
+    combine_into(-Cg, Lc, Y, X=Y, alpha=1.0)
+    return Y
+
+
+def _svqb_into(Y, tmp, drop=SVQB_DROP, passes=2):
+    """SVQB of Y (two sweeps), result written back into Y; tmp has Y's shape."""
+    for _ in range(passes):
+        Cw = svqb_coeffs(gram(Y, Y), drop)
+        combine_into(Cw, Y, tmp)
+        Y.copy_(tmp)
+    return Y
+
+
+def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
+                  maxit: int = 400, seed: int = 0, precond_shift: str = "median",
+                  verbose: bool = True, log_every: int = 0, initial_locked=None,
+                  initial_vals=None, on_block=None, locked_storage: str = "auto",
+                  theta_chunk: int = 0, normalized_rq: bool = True):
+    """Bottom-up deflated block LOBPCG.  Returns (eigenvalues, locked_vectors, stats).
+
+    Locks the lowest (m − guard) Ritz pairs of each block once their relative
+    residuals pass `tol`; the guard Ritz vectors seed the next block.
+    `initial_locked`/`initial_vals` resume a partial solve;
+    `on_block(n_locked_total, lam_block, X_block)` fires after each lock.
+    `locked_storage` and `theta_chunk` are accepted for API compatibility: the
+    locked set is always HBM-resident and Θ is chunked by the operator."""
+    if m <= guard:
+        raise ValueError("m must exceed guard")
+    G = op.grid_size
+    shape = (2, G, G, G)
+    dev = op.random_block(0, seed).device
+    n_lock_per = m - guard
+    cap = int(np.ceil(nev / n_lock_per)) * n_lock_per + m
+    n_init = 0
+    if initial_locked is not None:
+        init = torch.as_tensor(initial_locked) if not isinstance(initial_locked, torch.Tensor) \
+            else initial_locked
+        n_init = int(init.shape[0])
+        cap += n_init
+    locked_buf = torch.empty((cap,) + shape, dtype=torch.complex64, device=dev)
+    n_locked_now = 0
+    if n_init:
+        locked_buf[:n_init] = init.to(device=dev, dtype=torch.complex64)
+        n_locked_now = n_init
+    locked_vals = (np.empty((0,)) if initial_vals is None
+                   else np.asarray(initial_vals, np.float64))
+    if verbose:
+        print(f"  locked storage: device ({cap * 16 * G ** 3 / 2 ** 30:.1f} GiB capacity, "
+              f"{cap} rows)", flush=True)
+    # fixed device buffers
+    S = torch.empty((3 * m,) + shape, dtype=torch.complex64, device=dev)    # [X; W; P]
+    HS = torch.empty_like(S)                                                 # [ΘX; ΘW; ΘP]
+    T1 = torch.empty((m,) + shape, dtype=torch.complex64, device=dev)
+    T2 = torch.empty_like(T1)
+    X, W, P = S[:m], S[m:2 * m], S[2 * m:]
+    HX = HS[:m]
+    stats = SolveStats()
+    t0 = time.perf_counter()
+    carry = None
+    block_idx = 0
+
+    def rq(Xb, HXb):
+        lam = rowdot(Xb, HXb).real
+        if normalized_rq:
+            lam = lam / torch.clamp(rownorm2(Xb), min=1e-30)
+        return lam
+
+    while locked_vals.size < nev:
+        L = locked_buf if n_locked_now > 0 else None
+        nl = n_locked_now
+        # X = [carry, masked random unit rows]
+        nc = 0 if carry is None else int(carry.shape[0])
+        if nc:
+            X[:nc] = carry
+        if nc < m:
+            Z = op.random_block(m, seed, kind="normal", row0=block_idx * m)[:m - nc]
+            normalize_rows(Z, out=X[nc:])
+        block_idx += 1
+        _deflate_inplace(X, L, nl)
+        _svqb_into(X, T1)
+        op.theta_into(X, HX)
+        stats.theta_applications += m
+        have_p = False
+        n_lock = min(n_lock_per, nev - locked_vals.size)
+        it = 0
+        for it in range(maxit):
+            lam = rq(X, HX)
+            R = row_axpby(HX, torch.ones_like(lam), X, -lam, out=T2)
+            xnorm = rownorm(X)
+            rn = rownorm(R) / torch.clamp(rownorm(HX), min=1e-30)
+            lam_h = lam.cpu().numpy()
+            rn_h = rn.cpu().numpy()
+            dead_rows = xnorm.cpu().numpy() < 0.5
+            lam_h = np.where(dead_rows, np.inf, lam_h)
+            rn_h = np.where(dead_rows, np.inf, rn_h)
+            order = np.argsort(lam_h)
+            if (rn_h[order[:n_lock]] < tol).all():
+                break
+            if log_every and it % log_every == 0:
+                leak = float(gram(L[:nl], X).abs().max().item()) if nl else 0.0
+                lo = np.sort(lam_h)
+                print(f"    it {it:3d}  lam[0,{n_lock - 1},{m - 1}] = {lo[0]:.4f} "
+                      f"{lo[n_lock - 1]:.4f} {lo[-1]:.4f}  res q50/q90 "
+                      f"{np.quantile(rn_h, 0.5):.1e}/{np.quantile(rn_h, 0.9):.1e}  "
+                      f"leak {leak:.1e}", flush=True)
+            # W = K⁻¹R, deflated, projected out of X twice, SVQB (solver.py:251-257)
+            op.precondition_into(R, W)
+            stats.precond_applications += m
+            _deflate_inplace(W, L, nl)
+            for _ in range(2):
+                combine_into(-gram(X, W), X, W, X=W, alpha=1.0)
+            _svqb_into(W, T1)
+            # P projected out of [X, W] twice, fixed-shape whitening (solver.py:258-274)
+            if have_p:
+                XW = S[:2 * m]
+                for _ in range(2):
+                    combine_into(-gram(XW, P), XW, P, X=P, alpha=1.0)
+                wp, Vp = eigh(gram(P, P))
+                keepp = wp > SVQB_DROP * torch.clamp(wp.max(), min=1e-300)
+                if not bool(keepp.any()):
+                    have_p = False
+                else:
+                    scale = torch.where(keepp, 1.0 / torch.sqrt(torch.where(keepp, wp,
+                                                                            torch.ones_like(wp))),
+                                        torch.zeros_like(wp))
+                    combine_into(Vp * scale[None, :], P, T1)
+                    P.copy_(T1)
+            nb = (3 if have_p else 2) * m
+            Sb, HSb = S[:nb], HS[:nb]
+            op.theta_into(Sb[m:], HSb[m:])
+            stats.theta_applications += nb - m
+            A = gram(Sb, HSb)
+            A = (A + A.conj().T) / 2
+            dead = rownorm(Sb) < 0.5
+            penalty = max(RR_DEAD_PENALTY, 100.0 * float(torch.diagonal(A).abs().max().item()))
+            A = A + torch.diag(torch.where(dead, penalty, 0.0).to(A.dtype))
+            wA, VA = eigh(A)
+            Cx = VA[:, :m].contiguous()
+            combine_into(Cx, Sb, T1)                   # Xn
+            Cp = Cx.clone()
+            Cp[:m] = 0
+            combine_into(Cp, Sb, T2)                   # Pn (X rows zeroed)
+            _deflate_inplace(T1, L, nl)
+            normalize_rows(T1, floor=1e-20, out=X)
+            normalize_rows(T2, floor=1e-20, out=P)
+            have_p = True
+            op.theta_into(X, HX)
+            stats.theta_applications += m
+            stats.iterations += 1
+        # honest final values with a fresh ΘX (solver.py:316-330)
+        op.theta_into(X, HX)
+        stats.theta_applications += m
+        lam = rq(X, HX)
+        Rf = row_axpby(HX, torch.ones_like(lam), X, -lam, out=T2)
+        rn = (rownorm(Rf) / torch.clamp(rownorm(HX), min=1e-30)).cpu().numpy()
+        lam = lam.cpu().numpy()
+        xnorm = rownorm(X).cpu().numpy()
+        lam = np.where(xnorm < 0.5, np.inf, lam)
+        rn = np.where(xnorm < 0.5, np.inf, rn)
+        order = np.argsort(lam)
+        if not np.isfinite(lam[order[:n_lock]]).all():
+            raise RuntimeError(
+                f"block produced only {int(np.isfinite(lam).sum())} live Ritz pairs but needs "
+                f"{n_lock} to lock — rank collapse")
+        lock_idx = torch.as_tensor(order[:n_lock], device=dev)
+        carry_idx = torch.as_tensor(order[n_lock:], device=dev)
+        locked_buf[n_locked_now:n_locked_now + n_lock] = X[lock_idx]
+        carry = X[carry_idx].clone()
+        n_locked_now += n_lock
+        locked_vals = np.concatenate([locked_vals, lam[order[:n_lock]]])
+        stats.rounds.append({"iters": it + 1, "locked": int(locked_vals.size),
+                             "max_res_locked": float(rn[order[:n_lock]].max())})
+        if on_block is not None:
+            on_block(int(locked_vals.size), lam[order[:n_lock]],
+                     locked_buf[n_locked_now - n_lock:n_locked_now].cpu().numpy())
+        if verbose:
+            print(f"  locked {locked_vals.size:4d}/{nev} (+{n_lock})  iters {it + 1:3d}  "
+                  f"lam [{lam[order[0]]:.4f}..{lam[order[n_lock - 1]]:.4f}]  "
+                  f"worst res {rn[order[:n_lock]].max():.1e}  "
+                  f"elapsed {time.perf_counter() - t0:6.1f}s", flush=True)
+    stats.wall_seconds = time.perf_counter() - t0
+    if (np.diff(locked_vals) < -1e-6 * np.abs(locked_vals[1:])).any():
+        worst = float(np.min(np.diff(locked_vals)))
+        print(f"WARNING: locked eigenvalues non-monotone (worst step {worst:.3e}) — a band was "
+              f"recovered out of order; re-sorting.", flush=True)
+    order = np.argsort(locked_vals, kind="stable")
+    locked = locked_buf[:n_locked_now][torch.as_tensor(order, device=dev)]
+    del S, HS, T1, T2, locked_buf
+    gc.collect()
+    return locked_vals[order][:nev], locked, stats
+
+
+def lobpcg_blocks_resumable(op, nev, checkpointer=None, resume=False, **kw):
+    """`lobpcg_blocks` with per-block checkpointing and resume (solver.py:385-411).
+
+    `checkpointer` follows the reference's BlockCheckpointer protocol:
+    ``load() -> (vals, vecs, n_blocks, meta)`` and
+    ``save_block(i, lam_block, X_block, extra=...)``."""
+    if checkpointer is None:
+        return lobpcg_blocks(op, nev, **kw)
+    init_vals = init_vecs = None
+    n_blocks = 0
+    if resume:
+        init_vals, init_vecs, n_blocks, _ = checkpointer.load()
+        if init_vals is not None:
+            print(f"  resuming: {len(init_vals)} bands from {n_blocks} checkpointed blocks",
+                  flush=True)
+            if len(init_vals) >= nev:
+                return (np.asarray(init_vals)[:nev],
+                        torch.as_tensor(np.asarray(init_vecs)[:nev]).cuda(), SolveStats())
+    state = {"i": n_blocks}
+
+    def on_block(n_locked_total, lam_block, X_block):
+        checkpointer.save_block(state["i"], lam_block, X_block, extra={"n_locked": n_locked_total})
+        state["i"] += 1
+
+    return lobpcg_blocks(op, nev, initial_locked=init_vecs, initial_vals=init_vals,
+                         on_block=on_block, **kw)

@@ -1,0 +1,138 @@
+/* eigenfns_hip.h — C ABI of the MI355X (gfx950) Maxwell eigensolver kernels.
+ *
+ * The reference (fhernandez93/eigenfns) has no FFI: its boundary is the Python
+ * operator protocol of eigenfns/operator.py and the module functions of
+ * solver.py, chebyshev.py and interior.py, whose device work XLA generates.
+ * Each entry point below replaces one of those device computations; the
+ * citation names the reference code it stands in for.  The Python package
+ * eigenfns_amd binds these with ctypes and keeps the reference's Python API.
+ *
+ * Conventions
+ *  - Blocks of vectors are (m, 2, G, G, G) complex64 in device memory, C order,
+ *    z fastest, frequencies in np.fft.fftfreq order; "c64" pointers point to
+ *    interleaved float pairs, "c128" to interleaved double pairs.
+ *  - The caller owns every buffer.  A context owns only its tables and a
+ *    workspace allocated at creation; hot calls allocate nothing and do not
+ *    synchronise the host.  Calls sharing one context must be ordered on one
+ *    stream (they share its workspace).
+ *  - stream is a hipStream_t (NULL = the default stream).
+ *  - Every call returns 0 on success or a negative EFN_E* code; efn_last_error()
+ *    returns the calling thread's last message.  No C++ exception crosses the ABI.
+ */
+#ifndef EIGENFNS_HIP_H
+#define EIGENFNS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EFN_OK 0
+#define EFN_EINVAL (-1)       /* invalid argument        -> ValueError   */
+#define EFN_EUNSUPPORTED (-2) /* grid size not compiled  -> ValueError   */
+#define EFN_EHIP (-3)         /* HIP runtime failure     -> RuntimeError */
+#define EFN_ENOMEM (-4)       /* device allocation failed -> MemoryError */
+
+typedef struct efn_ctx efn_ctx;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int efn_version(void);
+/* Last error message of the calling thread ("" if none). */
+const char* efn_last_error(void);
+/* Non-zero if the grid edge G has compiled kernels. */
+int efn_grid_supported(int grid_size);
+
+/* Operator context for one ε grid and Bloch k-point.
+ * Replaces MaxwellOperator.__init__ + make_basis (eigenfns/operator.py:136-151,
+ * :44-79): builds the FP32 |k+G| axis tables, the FP64 reference-axis switch of
+ * operator.py:57 as a bit mask, ε⁻¹/G³ and ε/G³, and a workspace for
+ * `max_chunk` vectors (3·G³ complex64 each; 0 = automatic).
+ * eps_host: G³ floats on the host, C order. k in units of 2π/L. */
+int efn_ctx_create(int grid_size, double box_size, double kx, double ky, double kz,
+                   const float* eps_host, int max_chunk, efn_ctx** out);
+int efn_ctx_destroy(efn_ctx* ctx);
+/* kn_floor = 0.5·min nonzero FP32 |k+G| (operator.py:149-151), eps_mean_inv
+ * = mean(1/ε) (operator.py:148), chunk = vectors per workspace pass. */
+int efn_ctx_query(const efn_ctx* ctx, double* kn_floor, double* eps_mean_inv, int* chunk,
+                  size_t* workspace_bytes);
+/* |k+G| (G³ floats) and the frame t (2,3,G³ floats) on the device, as the
+ * kernels rebuild them (TransverseBasis.kn / .t, operator.py:29-41). */
+int efn_ctx_basis(const efn_ctx* ctx, float* kn_dev, float* t_dev, void* stream);
+
+/* out = alpha·Θ·in + beta·in for m vectors.  Θ: operator.py:82-95 / :153-155
+ * (alpha = 1, beta = 0); the Chebyshev map B = (2/λmax)Θ − I of
+ * interior.py:341-342 and exp_kpm_dos.py:95-96 is alpha = 2/λmax, beta = −1.
+ * in and out must not overlap. */
+int efn_theta(efn_ctx* ctx, const void* in_c64, void* out_c64, int m, float alpha, float beta,
+              void* stream);
+
+/* MPB transverse-projection preconditioner, operator.py:104-127 / :157-161. */
+int efn_precondition(efn_ctx* ctx, const void* in_c64, void* out_c64, int m, void* stream);
+
+/* Diagonal kinetic preconditioner out = in/(|k+G|² + scale), operator.py:98-101. */
+int efn_precondition_simple(efn_ctx* ctx, const void* in_c64, void* out_c64, int m, float scale,
+                            void* stream);
+
+/* Fused Chebyshev step (interior.py:347-349):
+ *   T0 ← 2·(alpha·Θ·T1 − T1) − T0,   Y ← Y + ck·T0(new).
+ * One operator application, T0 and Y updated in place. */
+int efn_cheb_step(efn_ctx* ctx, const void* T1_c64, void* T0_c64, void* Y_c64, float alpha,
+                  float ck, int m, void* stream);
+
+/* Fused KPM doubling step (scripts/exp/exp_kpm_dos.py:145-152), moments on the
+ * device in FP64:  T0 ← 2·(alpha·Θ·T1 − T1) − T0 and, for every row v,
+ *   mom[v·ld + 2k]   = 2⟨T1,T1⟩ − mom[v·ld + 0]
+ *   mom[v·ld + 2k+1] = 2·Re⟨T0(new),T1⟩ − mom[v·ld + 1]. */
+int efn_kpm_step(efn_ctx* ctx, const void* T1_c64, void* T0_c64, double* mom_dev, int ld, int k,
+                 float alpha, int m, void* stream);
+
+/* Fused KPM-count step (chebyshev.py:260-263):
+ *   T0 ← 2·(alpha·Θ·T1 − T1) − T0,   est[v] += ck·Re⟨Z_v, T0(new)_v⟩ (FP64). */
+int efn_kpm_count_step(efn_ctx* ctx, const void* T1_c64, void* T0_c64, const void* Z_c64,
+                       double* est_dev, double ck, float alpha, int m, void* stream);
+
+/* Gram block out[i][j] = Σ_k conj(A[i][k])·B[j][k] (solver.py:35-43), products
+ * of the FP32 inputs summed in FP64; out is ma×mb complex128 on the device.
+ * n = complex entries per row (2·G³ for a block).  work is a caller-provided
+ * device buffer of at least efn_gram_work_bytes(ma, mb, n) bytes (split
+ * partials, summed in a fixed order, so results are bitwise reproducible). */
+size_t efn_gram_work_bytes(int ma, int mb, int64_t n);
+int efn_gram(const void* A_c64, int ma, const void* B_c64, int mb, int64_t n, void* out_c128,
+             void* work, size_t work_bytes, void* stream);
+
+/* Linear combination out_i = alpha·X_i + Σ_j C[j][i]·S_j (solver.py:49-57,
+ * :106, :125): C is m_in×m_out complex64 on the device, S m_in rows, X
+ * (optional, NULL = none) and out m_out rows of n complex64.  out must not
+ * overlap S. */
+int efn_combine(const void* C_c64, int m_in, int m_out, const void* S_c64, int64_t n,
+                const void* X_c64, float alpha, void* out_c64, void* stream);
+
+/* Row reductions in FP64 (solver.py:228-231, interior.py:75-82):
+ *   op 0: out[i] = (Σ_k |A[i][k]|², 0)      op 1: out[i] = Σ_k conj(A[i][k])·B[i][k]
+ * out is m complex128 on the device. */
+size_t efn_rowreduce_work_bytes(int m, int64_t n);
+int efn_rowreduce(int op, const void* A_c64, const void* B_c64, int m, int64_t n, void* out_c128,
+                  void* work, size_t work_bytes, void* stream);
+
+/* out_i = a[i]·X_i + b[i]·Y_i with per-row FP64 coefficients on the device
+ * (Y, b may be NULL).  R = HX − λX (solver.py:229), row scaling (:306-307). */
+int efn_row_axpby(const void* X_c64, const double* a_dev, const void* Y_c64, const double* b_dev,
+                  int m, int64_t n, void* out_c64, void* stream);
+
+/* out = a·X + b·Y elementwise over count complex entries (Y may be NULL). */
+int efn_axpby(const void* X_c64, float a, const void* Y_c64, float b, int64_t count, void* out_c64,
+              void* stream);
+
+/* Random block of m rows (rows row0 .. row0+m−1 of the stream `seed`), Γ-masked.
+ * kind 0: complex normal (Box–Muller), kind 1: Rademacher ±1.  Replaces the JAX
+ * threefry draws of solver.py:196-205, chebyshev.py:243-248, exp_kpm_dos.py:138-140. */
+int efn_random_block(const efn_ctx* ctx, int kind, uint64_t seed, int64_t row0, int m,
+                     void* out_c64, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EIGENFNS_HIP_H */
