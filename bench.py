@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--m", type=int, default=M_BENCH)
     ap.add_argument("--chunk", type=int, default=0, help="vectors per workspace pass (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one plain copy kernel (efn_axpby, 16 B per complex) of the block "
+                         "before timing, to calibrate rocprof FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every GPU box "
@@ -99,6 +102,12 @@ def main():
     X = normalize_rows(op.random_block(m, seed=0))
     Y = torch.empty_like(X)
     stream = torch.cuda.current_stream()
+    if args.calib_copy:
+        from eigenfns_amd.linalg import axpby
+        axpby(X, 1.0, out=Y)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy: {X.numel() * 8} B read, {X.numel() * 8} B written",
+              file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         op.theta_into(X, Y)
     torch.cuda.synchronize()

@@ -1,9 +1,11 @@
 """Block linear algebra on the device: Gram, combine, deflation, SVQB, row reductions.
 
 Kernels: libeigenfns_hip.so (efn_gram / efn_combine / efn_rowreduce /
-efn_row_axpby).  Small dense FP64 problems (Gram eigendecompositions,
-Rayleigh–Ritz) run in FP64 on the GPU through torch.linalg (rocSOLVER), or on
-the host when EFN_SMALL_DENSE=host.
+efn_row_axpby).  Small dense FP64 eigenproblems (SVQB Gram matrices,
+Rayleigh–Ritz, ≤ 3m × 3m) run in host LAPACK by default (as the reference
+does, solver.py:119-121); EFN_SMALL_DENSE=gpu routes them to torch.linalg.eigh
+on the device, which failed to converge on well-conditioned complex128 Gram
+matrices on this ROCm stack and is therefore not the default.
 
 Reference: eigenfns/solver.py:27-126 and the row reductions of
 solver.py:228-238 / interior.py:75-82.
@@ -28,7 +30,7 @@ def _flat_n(X: torch.Tensor) -> int:
 
 
 def _small_dense_on_host() -> bool:
-    return os.environ.get("EFN_SMALL_DENSE", "gpu").lower() == "host"
+    return os.environ.get("EFN_SMALL_DENSE", "host").lower() != "gpu"
 
 
 def gram(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
@@ -143,7 +145,7 @@ def normalize_rows(X, floor: float = 1e-30, out=None) -> torch.Tensor:
 
 # ---- small dense FP64 ------------------------------------------------------
 def eigh(A: torch.Tensor):
-    """Hermitian eigendecomposition in FP64 (ascending), on the GPU by default."""
+    """Hermitian eigendecomposition in FP64 (ascending); host LAPACK unless EFN_SMALL_DENSE=gpu."""
     A = (A + A.conj().T) / 2
     if _small_dense_on_host():
         w, V = np.linalg.eigh(A.detach().cpu().numpy())
