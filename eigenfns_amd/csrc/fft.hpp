@@ -213,35 +213,44 @@ struct Plan {
   static constexpr int kStride = (kStrideRaw % 2 == 0) ? kStrideRaw + 1 : kStrideRaw;
 };
 
-// Four-step FFT over NP pencils resident in LDS (padded layout, natural order
-// in and out).  twid[e] = exp(-2πi e/N) for e in [0, N) (conjugated for S = +1).
-// Must be called by all NT threads of the block; contains __syncthreads().
+// Four-step building blocks on one pencil (padded LDS layout):
+//   step1: the task n2 owns inputs x[N2·n1 + n2] (n1 < N1) in registers; DFT_N1,
+//          twiddle by ω_N^{n2·k1}, store A[n2][k1] at pos(N2·k1 + n2).
+//   step3: the task k1 loads A[·][k1] (the contiguous run pos(N2·k1 + n2)) and
+//          leaves X[k1 + N1·k2] in r[k2].
+// twid[e] = exp(-2πi e/N) for e in [0, N) (conjugated for S = +1).
+template <int N, int S>
+__device__ __forceinline__ void step1_store(float2 (&v)[Plan<N>::N1], int n2, float2* __restrict__ pencil,
+                                            const float2* __restrict__ twid) {
+  using P = Plan<N>;
+  constexpr int N1 = P::N1, N2 = P::N2;
+  DFT<N1, S>::run(v);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) {
+    float2 w = twid[n2 * k1];  // n2·k1 < N
+    if (S > 0) w.y = -w.y;
+    v[k1] = cmul(v[k1], w);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) pencil[P::pos(N2 * k1 + n2)] = v[k1];
+}
+
+template <int N, int S>
+__device__ __forceinline__ void step3_load(float2 (&r)[Plan<N>::N2], int k1, const float2* __restrict__ pencil) {
+  using P = Plan<N>;
+  constexpr int N2 = P::N2;
+#pragma unroll
+  for (int n2 = 0; n2 < N2; ++n2) r[n2] = pencil[P::pos(N2 * k1 + n2)];
+  DFT<N2, S>::run(r);
+}
+
+// Step 3 of the four-step FFT over NP pencils in LDS, results written back in
+// natural order (read all, barrier, write all, barrier).
 template <int N, int S, int NP, int NT>
-__device__ __forceinline__ void lds_fft(float2* __restrict__ buf, const float2* __restrict__ twid) {
+__device__ __forceinline__ void lds_step3_natural(float2* __restrict__ buf) {
   using P = Plan<N>;
   constexpr int N1 = P::N1, N2 = P::N2;
   const int tid = threadIdx.x;
-  // step 1: task (p, n2) — DFT_N1 over x[N2·n1 + n2], twiddle by ω_N^{n2·k1}, in place
-  constexpr int T1 = NP * N2;
-  for (int t = tid; t < T1; t += NT) {
-    const int p = t / N2, n2 = t - p * N2;
-    float2* base = buf + p * P::kStride;
-    float2 v[N1];
-#pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[P::pos(N2 * n1 + n2)];
-    DFT<N1, S>::run(v);
-#pragma unroll
-    for (int k1 = 1; k1 < N1; ++k1) {
-      int e = n2 * k1;  // < N
-      float2 w = twid[e];
-      if (S > 0) w.y = -w.y;
-      v[k1] = cmul(v[k1], w);
-    }
-#pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) base[P::pos(N2 * k1 + n2)] = v[k1];
-  }
-  __syncthreads();
-  // step 3: task (p, k1) — DFT_N2 over A[n2][k1] (contiguous run), output X[k1 + N1·k2]
   constexpr int T3 = NP * N1;
   constexpr int TPT = (T3 + NT - 1) / NT;
   float2 r[TPT][N2];
@@ -250,10 +259,7 @@ __device__ __forceinline__ void lds_fft(float2* __restrict__ buf, const float2* 
     const int t = tid + j * NT;
     if (t < T3) {
       const int p = t / N1, k1 = t - p * N1;
-      const float2* base = buf + p * P::kStride;
-#pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) r[j][n2] = base[P::pos(N2 * k1 + n2)];
-      DFT<N2, S>::run(r[j]);
+      step3_load<N, S>(r[j], k1, buf + p * P::kStride);
     }
   }
   __syncthreads();
@@ -268,6 +274,25 @@ __device__ __forceinline__ void lds_fft(float2* __restrict__ buf, const float2* 
     }
   }
   __syncthreads();
+}
+
+// Four-step FFT over NP pencils resident in LDS (padded layout, natural order
+// in and out).  Must be called by all NT threads of the block; contains __syncthreads().
+template <int N, int S, int NP, int NT>
+__device__ __forceinline__ void lds_fft(float2* __restrict__ buf, const float2* __restrict__ twid) {
+  using P = Plan<N>;
+  constexpr int N1 = P::N1, N2 = P::N2;
+  constexpr int T1 = NP * N2;
+  for (int t = threadIdx.x; t < T1; t += NT) {
+    const int p = t / N2, n2 = t - p * N2;
+    float2* base = buf + p * P::kStride;
+    float2 v[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[P::pos(N2 * n1 + n2)];
+    step1_store<N, S>(v, n2, base, twid);
+  }
+  __syncthreads();
+  lds_step3_natural<N, S, NP, NT>(buf);
 }
 
 }  // namespace efn

@@ -14,7 +14,8 @@
 //                                 curl back, fused epilogue (store / Chebyshev
 //                                 step / KPM step with moment partials)
 // The frame (t̂₁, t̂₂, |k+G|) is rebuilt per voxel from three 1-D tables and a
-// bit-packed FP64 reference-axis switch computed on the host.
+// bit-packed FP64 reference-axis switch computed on the host.  The x-pass
+// multiply maps the three components of a tile to one XCD so ε is fetched once.
 #pragma once
 
 #include "fft.hpp"
@@ -37,55 +38,72 @@ struct Frame {
   float kn, t1x, t1y, t1z, t2x, t2y, t2z;
 };
 
-// operator.py:53-71: t̂₁ = K×ref/|K×ref|, t̂₂ = K×t̂₁/|K×t̂₁|, zero at K = 0.
-__device__ __forceinline__ Frame make_frame(const FrameTables& ft, int x, int y, int z, long long vox) {
-  Frame f;
-  const float Kx = ft.kx[x], Ky = ft.ky[y], Kz = ft.kz[z];
-  const float kn2 = fmaf(Kx, Kx, fmaf(Ky, Ky, Kz * Kz));
-  if (vox == ft.sing_voxel) {
-    f.kn = 0.f;
-    f.t1x = f.t1y = f.t1z = f.t2x = f.t2y = f.t2z = 0.f;
-    return f;
-  }
-  f.kn = sqrtf(kn2);
+// The transverse frame of operator.py:53-71 (t̂₁ = K×ref/|K×ref|,
+// t̂₂ = K×t̂₁/|K×t̂₁|, zero at K = 0; ref = x̂ where the FP64 switch bit is set,
+// else ẑ) in branch-free closed forms, algebraically identical, in FP32:
+//   reference axis ẑ:  t̂₁ = (K_y, −K_x, 0)/ρ,   t̂₂ = (K_x·K_z/ρ, K_y·K_z/ρ, −ρ)/|K|,  ρ = √(K_x²+K_y²)
+//   reference axis x̂:  t̂₁ = (0, K_z, −K_y)/σ,   t̂₂ = (−σ, K_x·K_y/σ, K_x·K_z/σ)/|K|,  σ = √(K_y²+K_z²)
+// ρ > 0 wherever ẑ is used and σ > 0 wherever x̂ is used (away from K = 0).
+struct RowFrame {
+  float Kx, Ky, rho2, rrho;
+};
+
+__device__ __forceinline__ RowFrame row_frame(const FrameTables& ft, int x, int y) {
+  RowFrame rf;
+  rf.Kx = ft.kx[x];
+  rf.Ky = ft.ky[y];
+  rf.rho2 = fmaf(rf.Kx, rf.Kx, rf.Ky * rf.Ky);
+  rf.rrho = 1.0f / sqrtf(rf.rho2);
+  return rf;
+}
+
+__device__ __forceinline__ Frame point_frame(const FrameTables& ft, const RowFrame& rf, int z, long long vox) {
   const bool sw = (ft.swap[vox >> 5] >> (vox & 31)) & 1u;
-  float ax, ay, az;
-  if (!sw) {  // ref = ẑ: K×ẑ = (Ky, −Kx, 0)
-    ax = Ky; ay = -Kx; az = 0.f;
-  } else {    // ref = x̂: K×x̂ = (0, Kz, −Ky)
-    ax = 0.f; ay = Kz; az = -Ky;
-  }
-  const float r1 = 1.0f / sqrtf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
-  f.t1x = ax * r1; f.t1y = ay * r1; f.t1z = az * r1;
-  const float bx = fmaf(Ky, f.t1z, -Kz * f.t1y);
-  const float by = fmaf(Kz, f.t1x, -Kx * f.t1z);
-  const float bz = fmaf(Kx, f.t1y, -Ky * f.t1x);
-  const float r2 = 1.0f / sqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz)));
-  f.t2x = bx * r2; f.t2y = by * r2; f.t2z = bz * r2;
+  const bool sing = vox == ft.sing_voxel;
+  const float Kx = rf.Kx, Ky = rf.Ky, Kz = ft.kz[z];
+  const float kn2 = fmaf(Kz, Kz, rf.rho2);
+  const float rkn = 1.0f / sqrtf(kn2);
+  const float sig2 = fmaf(Ky, Ky, Kz * Kz);
+  const float rsig = 1.0f / sqrtf(sig2);
+  Frame f;
+  const float a = rf.rrho * rkn, b = rsig * rkn;
+  f.kn = sing ? 0.f : kn2 * rkn;
+  f.t1x = sing ? 0.f : (sw ? 0.f : Ky * rf.rrho);
+  f.t1y = sing ? 0.f : (sw ? Kz * rsig : -Kx * rf.rrho);
+  f.t1z = sing ? 0.f : (sw ? -Ky * rsig : 0.f);
+  f.t2x = sing ? 0.f : (sw ? -sig2 * b : Kx * Kz * a);
+  f.t2y = sing ? 0.f : (sw ? Kx * Ky * b : Ky * Kz * a);
+  f.t2z = sing ? 0.f : (sw ? Kx * Kz * b : -rf.rho2 * a);
   return f;
 }
 
-// Pencils per block for the contiguous-axis passes: a power of two dividing G²
-// so a block never spans two vectors.
-__host__ __device__ constexpr int zrows_for(int G) {
+// Rows per zin/zout block when all three components are in LDS at once: a
+// power of two dividing G² (a block never spans two vectors), 8–32 rows.
+__host__ __device__ constexpr int zrows3_for(int G) {
   int target = G >= 128 ? 8 : (G >= 48 ? 16 : 32);
   int p = 1;
   while (p * 2 <= target && (G * G) % (p * 2) == 0) p *= 2;
   return p;
 }
-__host__ __device__ constexpr int ztile_for(int G) { return G >= 16 ? 16 : G; }
+template <int G>
+__host__ __device__ constexpr int zrows3() { return zrows3_for(G); }
 
 constexpr int kThreads = 256;
 
+
 // ---------------------------------------------------------------------------
-// zin: (nvec, 2, G³) → W (nvec, 3, G³), curl into the Cartesian frame + inverse z-FFT
+// zin: (nvec, 2, G³) → W (nvec, 3, G³), curl into the Cartesian frame + inverse
+// z-FFT.  A coalesced prologue reads each input point once and writes the three
+// Cartesian components in natural order to LDS; the four-step FFT runs on the
+// 3·R pencils; step-3 outputs go straight to HBM.
 template <int G, int KIND>
 __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict__ in,
                                                        float2* __restrict__ ws, int nvec,
                                                        FrameTables ft,
                                                        const float2* __restrict__ twid_g) {
   using P = Plan<G>;
-  constexpr int R = zrows_for(G);
+  constexpr int N1 = P::N1, N2 = P::N2;
+  constexpr int R = zrows3<G>();
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
   __shared__ float2 buf[3 * R * P::kStride];
   __shared__ float2 twid[G];
@@ -101,7 +119,7 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
     const int x = int(xy / G), y = int(xy - (long long)x * G);
     const long long vox = xy * G + z;
     const float2 a = a_in[vox], b = b_in[vox];
-    const Frame f = make_frame(ft, x, y, z, vox);
+    const Frame f = point_frame(ft, row_frame(ft, x, y), z, vox);
     float2 c1, c2;
     if (KIND == kTheta) {  // c1 = −i·kn·b, c2 = i·kn·a
       c1 = make_float2(f.kn * b.y, -f.kn * b.x);
@@ -117,31 +135,43 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
     buf[(2 * R + r) * P::kStride + p] = make_float2(fmaf(c1.x, f.t1z, c2.x * f.t2z), fmaf(c1.y, f.t1z, c2.y * f.t2z));
   }
   __syncthreads();
-  lds_fft<G, +1, 3 * R, kThreads>(buf, twid);
-  float2* w = ws + (long long)v * 3 * G3;
-  for (int e = threadIdx.x; e < R * G; e += kThreads) {
-    const int r = e / G, z = e - r * G;
-    const long long vox = (xy0 + r) * G + z;
-    const int p = P::pos(z);
+  for (int t = threadIdx.x; t < 3 * R * N2; t += kThreads) {
+    const int p = t / N2, n2 = t - p * N2;
+    float2* pen = buf + p * P::kStride;
+    float2 vv[N1];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) w[c * G3 + vox] = buf[(c * R + r) * P::kStride + p];
+    for (int n1 = 0; n1 < N1; ++n1) vv[n1] = pen[P::pos(N2 * n1 + n2)];
+    step1_store<G, +1>(vv, n2, pen, twid);
+  }
+  __syncthreads();
+  float2* w = ws + (long long)v * 3 * G3;
+  for (int t = threadIdx.x; t < 3 * R * N1; t += kThreads) {
+    const int p = t / N1, k1 = t - p * N1;
+    const int c = p / R, r = p - c * R;
+    float2 o[N2];
+    step3_load<G, +1>(o, k1, buf + p * P::kStride);
+    float2* dst = w + c * G3 + (xy0 + r) * G;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) dst[k1 + N1 * k2] = o[k2];
   }
 }
 
 // ---------------------------------------------------------------------------
-// Strided pass over W (nvec·3 fields) in place.  AXIS 1 = y (stride G), 0 = x
-// (stride G²).  MID: inverse FFT, multiply by rmul (real, per voxel), forward FFT.
-template <int G, int AXIS, int S, bool MID>
+// Strided pass over W (nvec·3 fields) in place.  AXIS 1 = y (stride G),
+// 0 = x (stride G²).  Tile = ZT consecutive z × all G positions on the axis;
+// HBM → registers (step 1) → LDS → registers (step 3) → HBM.
+template <int G, int AXIS, int S>
 __global__ __launch_bounds__(kThreads) void strided_kernel(float2* __restrict__ ws,
-                                                           const float* __restrict__ rmul,
                                                            const float2* __restrict__ twid_g) {
   using P = Plan<G>;
+  constexpr int N1 = P::N1, N2 = P::N2;
   constexpr int ZT = ztile_for(G);
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
   constexpr long long stride = AXIS == 1 ? G : G2;
   __shared__ float2 buf[ZT * P::kStride];
   __shared__ float2 twid[G];
   for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+  __syncthreads();
   constexpr int nzt = (G + ZT - 1) / ZT;
   const long long fo = blockIdx.x / nzt;  // (field, other-axis index)
   const int zt = blockIdx.x - int(fo * nzt);
@@ -149,30 +179,122 @@ __global__ __launch_bounds__(kThreads) void strided_kernel(float2* __restrict__ 
   const int o = int(fo - field * G);
   const int z0 = zt * ZT;
   float2* base = ws + field * G3 + (AXIS == 1 ? (long long)o * G2 : (long long)o * G) + z0;
-  for (int e = threadIdx.x; e < ZT * G; e += kThreads) {
-    const int n = e / ZT, z = e - n * ZT;
-    if (z0 + z < G) buf[z * P::kStride + P::pos(n)] = base[n * stride + z];
+  for (int t = threadIdx.x; t < ZT * N2; t += kThreads) {
+    const int z = t % ZT, n2 = t / ZT;
+    if (z0 + z < G) {
+      float2 v[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[(N2 * n1 + n2) * stride + z];
+      step1_store<G, S>(v, n2, buf + z * P::kStride, twid);
+    }
   }
   __syncthreads();
-  if (!MID) {
-    lds_fft<G, S, ZT, kThreads>(buf, twid);
+  for (int t = threadIdx.x; t < ZT * N1; t += kThreads) {
+    const int z = t % ZT, k1 = t / ZT;
+    if (z0 + z < G) {
+      float2 r[N2];
+      step3_load<G, S>(r, k1, buf + z * P::kStride);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) base[(k1 + N1 * k2) * stride + z] = r[k2];
+    }
+  }
+}
+
+// x-pass with the real-space multiply: inverse x-FFT, ×rmul (ε⁻¹/G³ or ε/G³),
+// forward x-FFT.  The three Cartesian components of one tile run on blocks of
+// the same XCD, back to back, so the rmul tile is fetched from HBM once and
+// served from that XCD's L2 to the other two.
+template <int G>
+__global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
+                                                        const float* __restrict__ rmul,
+                                                        const float2* __restrict__ twid_g,
+                                                        long long tiles) {
+  using P = Plan<G>;
+  constexpr int N1 = P::N1, N2 = P::N2;
+  constexpr int ZT = ztile_for(G);
+  constexpr long long G2 = (long long)G * G, G3 = G2 * G;
+  constexpr int nzt = (G + ZT - 1) / ZT;
+  __shared__ float2 buf[ZT * P::kStride];
+  __shared__ float2 twid[G];
+  for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+  __syncthreads();
+  const long long b = blockIdx.x;
+  long long tile;
+  int c;
+  if ((tiles & 7) == 0) {  // XCD-aware: blocks b ≡ xcd (mod 8) run on one XCD
+    const long long q = b >> 3;
+    c = int(q % 3);
+    tile = (q / 3) * 8 + (b & 7);
   } else {
-    lds_fft<G, +1, ZT, kThreads>(buf, twid);
-    for (int e = threadIdx.x; e < ZT * G; e += kThreads) {
-      const int n = e / ZT, z = e - n * ZT;
-      if (z0 + z < G) {
-        // x-pass: voxel (x = n, y = o, z)
-        const float s = rmul[(long long)n * G2 + (long long)o * G + z0 + z];
-        float2& d = buf[z * P::kStride + P::pos(n)];
-        d = make_float2(d.x * s, d.y * s);
+    c = int(b % 3);
+    tile = b / 3;
+  }
+  const long long v = tile / (G * nzt);
+  const int rem = int(tile - v * (G * nzt));
+  const int o = rem / nzt;  // y
+  const int z0 = (rem - o * nzt) * ZT;
+  float2* base = ws + (v * 3 + c) * G3 + (long long)o * G + z0;
+  const float* eps = rmul + (long long)o * G + z0;
+  // inverse step 1: HBM → registers → LDS
+  for (int t = threadIdx.x; t < ZT * N2; t += kThreads) {
+    const int z = t % ZT, n2 = t / ZT;
+    if (z0 + z < G) {
+      float2 vv[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) vv[n1] = base[(N2 * n1 + n2) * G2 + z];
+      step1_store<G, +1>(vv, n2, buf + z * P::kStride, twid);
+    }
+  }
+  __syncthreads();
+  // inverse step 3, real-space multiply, back to LDS in natural order
+  constexpr int T3 = ZT * N1;
+  constexpr int TPT = (T3 + kThreads - 1) / kThreads;
+  float2 r[TPT][N2];
+#pragma unroll
+  for (int j = 0; j < TPT; ++j) {
+    const int t = threadIdx.x + j * kThreads;
+    const int z = t % ZT, k1 = t / ZT;
+    if (t < T3 && z0 + z < G) {
+      step3_load<G, +1>(r[j], k1, buf + z * P::kStride);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const float sc = eps[(long long)(k1 + N1 * k2) * G2 + z];
+        r[j][k2] = cscale(r[j][k2], sc);
       }
     }
-    __syncthreads();
-    lds_fft<G, -1, ZT, kThreads>(buf, twid);
   }
-  for (int e = threadIdx.x; e < ZT * G; e += kThreads) {
-    const int n = e / ZT, z = e - n * ZT;
-    if (z0 + z < G) base[n * stride + z] = buf[z * P::kStride + P::pos(n)];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < TPT; ++j) {
+    const int t = threadIdx.x + j * kThreads;
+    const int z = t % ZT, k1 = t / ZT;
+    if (t < T3 && z0 + z < G) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) buf[z * P::kStride + P::pos(k1 + N1 * k2)] = r[j][k2];
+    }
+  }
+  __syncthreads();
+  // forward step 1 in LDS (each task rewrites its own slots)
+  for (int t = threadIdx.x; t < ZT * N2; t += kThreads) {
+    const int z = t % ZT, n2 = t / ZT;
+    if (z0 + z < G) {
+      float2* pen = buf + z * P::kStride;
+      float2 vv[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) vv[n1] = pen[P::pos(N2 * n1 + n2)];
+      step1_store<G, -1>(vv, n2, pen, twid);
+    }
+  }
+  __syncthreads();
+  // forward step 3: LDS → registers → HBM
+  for (int t = threadIdx.x; t < T3; t += kThreads) {
+    const int z = t % ZT, k1 = t / ZT;
+    if (z0 + z < G) {
+      float2 o2[N2];
+      step3_load<G, -1>(o2, k1, buf + z * P::kStride);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) base[(k1 + N1 * k2) * G2 + z] = o2[k2];
+    }
   }
 }
 
@@ -202,88 +324,98 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;  // valid in thread 0
 }
 
-// zout: W (nvec,3,G³) → forward z-FFT → project + curl back → epilogue
+// zout: W (nvec,3,G³) → forward z-FFT → project on t̂₁, t̂₂ → curl back →
+// epilogue.  Step-1 inputs of the three components come straight from HBM;
+// step 3 writes back to LDS in natural order; the epilogue reads the three
+// components per point and applies the fused update with coalesced HBM access.
 template <int G, int KIND, int EPI>
 __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict__ ws, int nvec,
                                                         FrameTables ft,
                                                         const float2* __restrict__ twid_g,
                                                         EpiArgs ea) {
   using P = Plan<G>;
-  constexpr int R = zrows_for(G);
+  constexpr int N1 = P::N1, N2 = P::N2;
+  constexpr int R = zrows3<G>();
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
   __shared__ float2 buf[3 * R * P::kStride];
   __shared__ float2 twid[G];
   __shared__ double red[2 * kThreads / 64];
   for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+  __syncthreads();
   const long long row0 = (long long)blockIdx.x * R;
   const int v = int(row0 / G2);
   const long long xy0 = row0 - (long long)v * G2;
   const float2* w = ws + (long long)v * 3 * G3;
-  for (int e = threadIdx.x; e < R * G; e += kThreads) {
-    const int r = e / G, z = e - r * G;
-    const long long vox = (xy0 + r) * G + z;
-    const int p = P::pos(z);
+  for (int t = threadIdx.x; t < 3 * R * N2; t += kThreads) {
+    const int p = t / N2, n2 = t - p * N2;
+    const int c = p / R, r = p - c * R;
+    const float2* src = w + c * G3 + (xy0 + r) * G;
+    float2 vv[N1];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) buf[(c * R + r) * P::kStride + p] = w[c * G3 + vox];
+    for (int n1 = 0; n1 < N1; ++n1) vv[n1] = src[N2 * n1 + n2];
+    step1_store<G, -1>(vv, n2, buf + p * P::kStride, twid);
   }
   __syncthreads();
-  lds_fft<G, -1, 3 * R, kThreads>(buf, twid);
+  lds_step3_natural<G, -1, 3 * R, kThreads>(buf);
   const long long vbase = (long long)v * 2 * G3;
   double s0 = 0.0, s1 = 0.0;
   for (int e = threadIdx.x; e < R * G; e += kThreads) {
-    const int r = e / G, z = e - r * G;
-    const long long xy = xy0 + r;
-    const int x = int(xy / G), y = int(xy - (long long)x * G);
-    const long long vox = xy * G + z;
-    const int p = P::pos(z);
-    const float2 d0 = buf[(0 * R + r) * P::kStride + p];
-    const float2 d1 = buf[(1 * R + r) * P::kStride + p];
-    const float2 d2 = buf[(2 * R + r) * P::kStride + p];
-    const Frame f = make_frame(ft, x, y, z, vox);
-    const float2 f1 = make_float2(fmaf(f.t1x, d0.x, fmaf(f.t1y, d1.x, f.t1z * d2.x)),
-                                  fmaf(f.t1x, d0.y, fmaf(f.t1y, d1.y, f.t1z * d2.y)));
-    const float2 f2 = make_float2(fmaf(f.t2x, d0.x, fmaf(f.t2y, d1.x, f.t2z * d2.x)),
-                                  fmaf(f.t2x, d0.y, fmaf(f.t2y, d1.y, f.t2z * d2.y)));
-    float2 o0, o1;
-    if (KIND == kTheta) {  // (−i·kn·f2, i·kn·f1)
-      o0 = make_float2(f.kn * f2.y, -f.kn * f2.x);
-      o1 = make_float2(-f.kn * f1.y, f.kn * f1.x);
-    } else {               // (i·f2/knr, −i·f1/knr)
-      const float inv = 1.0f / fmaxf(f.kn, ft.kn_floor);
-      o0 = make_float2(-f2.y * inv, f2.x * inv);
-      o1 = make_float2(f1.y * inv, -f1.x * inv);
-    }
-    const long long i0 = vbase + vox, i1 = vbase + G3 + vox;
-    if (EPI == kStore) {
-      if (ea.beta != 0.f) {
-        const float2 x0 = ea.x[i0], x1 = ea.x[i1];
-        o0 = make_float2(fmaf(ea.alpha, o0.x, ea.beta * x0.x), fmaf(ea.alpha, o0.y, ea.beta * x0.y));
-        o1 = make_float2(fmaf(ea.alpha, o1.x, ea.beta * x1.x), fmaf(ea.alpha, o1.y, ea.beta * x1.y));
-      } else if (ea.alpha != 1.f) {
-        o0 = cscale(o0, ea.alpha);
-        o1 = cscale(o1, ea.alpha);
+    {
+      const int r = e / G, z = e - r * G;
+      const long long xy = xy0 + r;
+      const int x = int(xy / G), y = int(xy - (long long)x * G);
+      const long long vox = xy * G + z;
+      const int p = P::pos(z);
+      const float2 d0 = buf[(0 * R + r) * P::kStride + p];
+      const float2 d1 = buf[(1 * R + r) * P::kStride + p];
+      const float2 d2 = buf[(2 * R + r) * P::kStride + p];
+      const Frame f = point_frame(ft, row_frame(ft, x, y), z, vox);
+      const float2 f1 = make_float2(fmaf(f.t1x, d0.x, fmaf(f.t1y, d1.x, f.t1z * d2.x)),
+                                    fmaf(f.t1x, d0.y, fmaf(f.t1y, d1.y, f.t1z * d2.y)));
+      const float2 f2 = make_float2(fmaf(f.t2x, d0.x, fmaf(f.t2y, d1.x, f.t2z * d2.x)),
+                                    fmaf(f.t2x, d0.y, fmaf(f.t2y, d1.y, f.t2z * d2.y)));
+      const float kn = f.kn;
+      float2 o0, o1;
+      if (KIND == kTheta) {  // (−i·kn·f2, i·kn·f1)
+        o0 = make_float2(kn * f2.y, -kn * f2.x);
+        o1 = make_float2(-kn * f1.y, kn * f1.x);
+      } else {               // (i·f2/knr, −i·f1/knr)
+        const float inv = 1.0f / fmaxf(kn, ft.kn_floor);
+        o0 = make_float2(-f2.y * inv, f2.x * inv);
+        o1 = make_float2(f1.y * inv, -f1.x * inv);
       }
-      ea.out[i0] = o0;
-      ea.out[i1] = o1;
-    } else {
-      // T_{k+1} = 2·(alpha·Θ T_k − T_k) − T_{k−1}   (interior.py:341-348, exp_kpm_dos.py:148)
-      const float2 x0 = ea.x[i0], x1 = ea.x[i1];
-      const float2 p0 = ea.out[i0], p1 = ea.out[i1];
-      const float2 n0 = make_float2(2.f * fmaf(ea.alpha, o0.x, -x0.x) - p0.x, 2.f * fmaf(ea.alpha, o0.y, -x0.y) - p0.y);
-      const float2 n1 = make_float2(2.f * fmaf(ea.alpha, o1.x, -x1.x) - p1.x, 2.f * fmaf(ea.alpha, o1.y, -x1.y) - p1.y);
-      ea.out[i0] = n0;
-      ea.out[i1] = n1;
-      if (EPI == kCheb) {
-        const float2 y0 = ea.y[i0], y1 = ea.y[i1];
-        ea.y[i0] = make_float2(fmaf(ea.ck, n0.x, y0.x), fmaf(ea.ck, n0.y, y0.y));
-        ea.y[i1] = make_float2(fmaf(ea.ck, n1.x, y1.x), fmaf(ea.ck, n1.y, y1.y));
-      } else if (EPI == kKpm) {
-        // ⟨T_k, T_k⟩ and Re⟨T_{k+1}, T_k⟩ in FP64 (products of FP32 are exact in FP64)
-        s0 += (double)x0.x * x0.x + (double)x0.y * x0.y + (double)x1.x * x1.x + (double)x1.y * x1.y;
-        s1 += (double)n0.x * x0.x + (double)n0.y * x0.y + (double)n1.x * x1.x + (double)n1.y * x1.y;
-      } else {  // kDot: Re⟨z, T_{k+1}⟩
-        const float2 z0 = ea.z[i0], z1 = ea.z[i1];
-        s0 += (double)z0.x * n0.x + (double)z0.y * n0.y + (double)z1.x * n1.x + (double)z1.y * n1.y;
+      const long long i0 = vbase + vox, i1 = vbase + G3 + vox;
+      if (EPI == kStore) {
+        if (ea.beta != 0.f) {
+          const float2 x0 = ea.x[i0], x1 = ea.x[i1];
+          o0 = make_float2(fmaf(ea.alpha, o0.x, ea.beta * x0.x), fmaf(ea.alpha, o0.y, ea.beta * x0.y));
+          o1 = make_float2(fmaf(ea.alpha, o1.x, ea.beta * x1.x), fmaf(ea.alpha, o1.y, ea.beta * x1.y));
+        } else if (ea.alpha != 1.f) {
+          o0 = cscale(o0, ea.alpha);
+          o1 = cscale(o1, ea.alpha);
+        }
+        ea.out[i0] = o0;
+        ea.out[i1] = o1;
+      } else {
+        // T_{k+1} = 2·(alpha·Θ T_k − T_k) − T_{k−1}   (interior.py:341-348, exp_kpm_dos.py:148)
+        const float2 x0 = ea.x[i0], x1 = ea.x[i1];
+        const float2 p0 = ea.out[i0], p1 = ea.out[i1];
+        const float2 n0 = make_float2(2.f * fmaf(ea.alpha, o0.x, -x0.x) - p0.x, 2.f * fmaf(ea.alpha, o0.y, -x0.y) - p0.y);
+        const float2 n1 = make_float2(2.f * fmaf(ea.alpha, o1.x, -x1.x) - p1.x, 2.f * fmaf(ea.alpha, o1.y, -x1.y) - p1.y);
+        ea.out[i0] = n0;
+        ea.out[i1] = n1;
+        if (EPI == kCheb) {
+          const float2 y0 = ea.y[i0], y1 = ea.y[i1];
+          ea.y[i0] = make_float2(fmaf(ea.ck, n0.x, y0.x), fmaf(ea.ck, n0.y, y0.y));
+          ea.y[i1] = make_float2(fmaf(ea.ck, n1.x, y1.x), fmaf(ea.ck, n1.y, y1.y));
+        } else if (EPI == kKpm) {
+          // ⟨T_k, T_k⟩ and Re⟨T_{k+1}, T_k⟩ in FP64 (products of FP32 are exact in FP64)
+          s0 += (double)x0.x * x0.x + (double)x0.y * x0.y + (double)x1.x * x1.x + (double)x1.y * x1.y;
+          s1 += (double)n0.x * x0.x + (double)n0.y * x0.y + (double)n1.x * x1.x + (double)n1.y * x1.y;
+        } else {  // kDot: Re⟨z, T_{k+1}⟩
+          const float2 z0 = ea.z[i0], z1 = ea.z[i1];
+          s0 += (double)z0.x * n0.x + (double)z0.y * n0.y + (double)z1.x * n1.x + (double)z1.y * n1.y;
+        }
       }
     }
   }
