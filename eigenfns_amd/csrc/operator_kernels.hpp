@@ -77,6 +77,9 @@ __device__ __forceinline__ Frame point_frame(const FrameTables& ft, const RowFra
   return f;
 }
 
+// z-tile width of the strided (y, x) passes: 16 complex = 128 B runs.
+__host__ __device__ constexpr int ztile_for(int G) { return G >= 16 ? 16 : G; }
+
 // Rows per zin/zout block when all three components are in LDS at once: a
 // power of two dividing G² (a block never spans two vectors), 8–32 rows.
 __host__ __device__ constexpr int zrows3_for(int G) {
