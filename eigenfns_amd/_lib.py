@@ -36,6 +36,8 @@ _SIGNATURES = {
     "efn_ctx_basis": (_i, [_vp, _vp, _vp, _vp]),
     "efn_theta": (_i, [_vp, _vp, _vp, _i, _f, _f, _vp]),
     "efn_precondition": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "efn_theta_multi": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _vp]),
+    "efn_precondition_multi": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
     "efn_precondition_simple": (_i, [_vp, _vp, _vp, _i, _f, _vp]),
     "efn_cheb_step": (_i, [_vp, _vp, _vp, _vp, _f, _f, _i, _vp]),
     "efn_kpm_step": (_i, [_vp, _vp, _vp, _vp, _i, _i, _f, _i, _vp]),
@@ -43,10 +45,16 @@ _SIGNATURES = {
     "efn_gram_work_bytes": (_sz, [_i, _i, _i64]),
     "efn_gram": (_i, [_vp, _i, _vp, _i, _i64, _vp, _vp, _sz, _vp]),
     "efn_combine": (_i, [_vp, _i, _i, _vp, _i64, _vp, _f, _vp, _vp]),
+    "efn_gram_multi_work_bytes": (_sz, [_i, _i, _i, _i64]),
+    "efn_gram_multi": (_i, [_i, _vp, _vp, _i, _i, _i64, _vp, _vp, _sz, _vp]),
+    "efn_combine_multi": (_i, [_i, _vp, _vp, _i, _i, _i64, _vp, _f, _vp, _vp]),
     "efn_rowreduce_work_bytes": (_sz, [_i, _i64]),
     "efn_rowreduce": (_i, [_i, _vp, _vp, _i, _i64, _vp, _vp, _sz, _vp]),
     "efn_row_axpby": (_i, [_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp]),
+    "efn_rowreduce_strided": (_i, [_i, _vp, _i64, _vp, _i64, _i, _i, _i64, _vp, _vp, _sz, _vp]),
+    "efn_row_axpby_strided": (_i, [_vp, _i64, _vp, _vp, _i64, _vp, _i, _i, _i64, _vp, _i64, _vp]),
     "efn_axpby": (_i, [_vp, _f, _vp, _f, _i64, _vp, _vp]),
+    "efn_copy_multi": (_i, [_i, _vp, _vp, _i64, _vp]),
     "efn_random_block": (_i, [_vp, _i, C.c_uint64, _i64, _i, _vp, _vp]),
 }
 

@@ -1,0 +1,380 @@
+"""Batched, continuously admitted deflated block LOBPCG for many small problems.
+
+Serving path of the north star: many (structure, Bloch k-point, band count)
+solves of one grid size advance in lockstep, one LOBPCG iteration per engine
+step, through shared launches — multi-operator Θ / preconditioner
+(efn_theta_multi: each slot's vectors carry their own frame tables and ε),
+multi-problem Gram and combine (efn_gram_multi / efn_combine_multi), strided
+row reductions over every slot's rows at once — and new requests take a slot
+as soon as one frees (continuous batching).  Each slot runs the reference's
+algorithm (eigenfns/solver.py:136-382): guard warm start, re-deflation every
+iteration, SVQB (1e-5 drop), fixed-shape P whitening, Rayleigh–Ritz with the
+dead-row penalty, lock the lowest m − guard.  A slot without P keeps zero P
+rows, which the dead-row penalty removes from the Rayleigh–Ritz — the
+reference's [X, W] basis.
+
+Batch invariance: every per-slot quantity is computed from that slot's data
+only, with reduction orders that depend only on the grid size and block width
+(Gram splits depend on the row length only, Θ is per vector, random rows come
+from the request's own counter-based stream), so a request's eigenvalues are
+bitwise the same alone and in any mix.
+
+Storage per slot (fixed, HBM-resident): [X; W; P] and [ΘX; ΘW; ΘP] (3m rows
+each), R, two m-row temporaries, the carry and the locked rows (zero-padded to
+the engine capacity so deflation runs at one shape).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .linalg import (RR_DEAD_PENALTY, SVQB_DROP, combine_multi, copy_multi, eigh_batched_host,
+                     gram_multi, rowreduce_strided, row_axpby_strided, MAX_PROBLEMS)
+from .operator import MaxwellOperator, precondition_multi, theta_multi
+
+
+@dataclass
+class Request:
+    """One eigenproblem: lowest `nev` bands of `op`."""
+
+    op: MaxwellOperator
+    nev: int
+    seed: int = 0
+    rid: int = 0
+    meta: dict = field(default_factory=dict)
+    # filled by the engine
+    vals: np.ndarray | None = None
+    res: np.ndarray | None = None
+    vecs: torch.Tensor | None = None
+    iterations: int = 0
+    theta_applications: int = 0
+    t_arrive: float = 0.0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+
+
+class _Slot:
+    def __init__(self, req: Request, m: int, guard: int):
+        self.req = req
+        self.op = req.op
+        self.nev = req.nev
+        self.m = m
+        self.guard = guard
+        self.locked_vals = np.empty((0,))
+        self.locked_res = np.empty((0,))
+        self.n_locked = 0
+        self.nc = 0                  # carry rows waiting in the carry buffer
+        self.block_idx = 0
+        self.phase = "init"
+        self.it = 0
+        self.n_lock = 0
+
+
+class BatchedLOBPCG:
+    """Lockstep LOBPCG over up to `nslots` (≤ 32) problems of grid size G.
+
+    Parameters follow lobpcg_blocks (m, guard, tol, maxit); `max_nev` sizes the
+    per-slot locked storage.  `keep_vectors` copies each finished request's
+    locked vectors out of the engine (otherwise only eigenvalues and residuals
+    are returned)."""
+
+    def __init__(self, grid_size: int, m: int = 24, guard: int = 8, nslots: int = 8,
+                 max_nev: int = 64, tol: float = 1e-4, maxit: int = 400,
+                 keep_vectors: bool = False, normalized_rq: bool = True):
+        if not 0 < guard < m:
+            raise ValueError("need 0 < guard < m")
+        if not 1 <= nslots <= MAX_PROBLEMS:
+            raise ValueError(f"nslots must be in [1, {MAX_PROBLEMS}]")
+        self.G = int(grid_size)
+        self.m = int(m)
+        self.guard = int(guard)
+        self.nslots = int(nslots)
+        self.tol = float(tol)
+        self.maxit = int(maxit)
+        self.keep_vectors = keep_vectors
+        self.normalized_rq = normalized_rq
+        per = self.m - self.guard
+        self.cap = int(np.ceil(max_nev / per)) * per + self.m
+        self.max_nev = int(max_nev)
+        G = self.G
+        shape = (2, G, G, G)
+        self.n = 2 * G ** 3
+        # workspace owner for the multi launches (ε = 1; only its workspace is used)
+        self.ws = MaxwellOperator(np.ones((G, G, G), np.float32), 1.0)
+        dev = self.ws._dev
+        c64 = torch.complex64
+        m3 = 3 * self.m
+        self.S = torch.zeros((nslots * m3,) + shape, dtype=c64, device=dev)
+        self.HS = torch.zeros_like(self.S)
+        self.R = torch.zeros((nslots * self.m,) + shape, dtype=c64, device=dev)
+        self.T1 = torch.zeros_like(self.R)
+        self.T2 = torch.zeros_like(self.R)
+        self.CARRY = torch.zeros_like(self.R)
+        self.L = torch.zeros((nslots * self.cap,) + shape, dtype=c64, device=dev)
+        self.slots: list[_Slot | None] = [None] * nslots
+        self.steps = 0
+
+    # ---- bookkeeping --------------------------------------------------------
+    def free_slots(self) -> int:
+        return sum(s is None for s in self.slots)
+
+    def active(self) -> int:
+        return self.nslots - self.free_slots()
+
+    def bytes_per_slot(self) -> int:
+        return (2 * 3 * self.m + 4 * self.m + self.cap) * self.n * 8
+
+    def admit(self, req: Request) -> int:
+        if req.op.grid_size != self.G:
+            raise ValueError("request grid size does not match the engine")
+        if req.nev > self.max_nev:
+            raise ValueError(f"nev {req.nev} exceeds the engine capacity {self.max_nev}")
+        for s, sl in enumerate(self.slots):
+            if sl is None:
+                self.slots[s] = _Slot(req, self.m, self.guard)
+                self.L[s * self.cap:(s + 1) * self.cap].zero_()
+                req.t_admit = time.perf_counter()
+                return s
+        raise RuntimeError("no free slot")
+
+    # ---- row views ----------------------------------------------------------
+    def _X(self, s):
+        return self.S[s * 3 * self.m:s * 3 * self.m + self.m]
+
+    def _W(self, s):
+        return self.S[s * 3 * self.m + self.m:s * 3 * self.m + 2 * self.m]
+
+    def _P(self, s):
+        return self.S[s * 3 * self.m + 2 * self.m:(s + 1) * 3 * self.m]
+
+    def _XW(self, s):
+        return self.S[s * 3 * self.m:s * 3 * self.m + 2 * self.m]
+
+    def _Sb(self, s):
+        return self.S[s * 3 * self.m:(s + 1) * 3 * self.m]
+
+    def _HSb(self, s):
+        return self.HS[s * 3 * self.m:(s + 1) * 3 * self.m]
+
+    def _rows(self, buf, s, width):
+        return buf[s * width:(s + 1) * width]
+
+    def _Lk(self, s, k):
+        return self.L[s * self.cap:s * self.cap + k]
+
+    # ---- batched building blocks ---------------------------------------------
+    def _deflate(self, idx, target):
+        """target(s) ← target(s) − L_s(L_sᴴ target(s)) for slots idx (zero-padded locked rows)."""
+        nl = max(self.slots[s].n_locked for s in idx)
+        if nl == 0:
+            return
+        Ls = [self._Lk(s, nl) for s in idx]
+        Ts = [target(s) for s in idx]
+        C = gram_multi(Ls, Ts)
+        combine_multi(-C, Ls, Ts, Xs=Ts, alpha=1.0)
+
+    def _svqb(self, idx, block, passes=2):
+        """SVQB of block(s) for slots idx, two sweeps, written back in place."""
+        for _ in range(passes):
+            Bs = [block(s) for s in idx]
+            w, V = eigh_batched_host(gram_multi(Bs, Bs))
+            wmax = np.maximum(w.max(axis=1, keepdims=True), 1e-300)
+            keep = w > SVQB_DROP * wmax
+            scale = np.where(keep, 1.0 / np.sqrt(np.where(keep, w, 1.0)), 0.0)
+            Cw = torch.as_tensor(V * scale[:, None, :], device=self.S.device)
+            Ts = [self._rows(self.T1, s, self.m) for s in idx]
+            combine_multi(Cw, Bs, Ts)
+            copy_multi(Ts, Bs)
+
+    def _theta(self, idx, src, row0_src, dst, row0_dst, stride_src, stride_dst, nrow):
+        for c in range(0, len(idx), MAX_PROBLEMS):
+            part = idx[c:c + MAX_PROBLEMS]
+            theta_multi(self.ws, [self.slots[s].op for s in part], src,
+                        [s * stride_src + row0_src for s in part], dst,
+                        [s * stride_dst + row0_dst for s in part], nrow)
+        for s in idx:
+            self.slots[s].req.theta_applications += nrow
+
+    # ---- phases -------------------------------------------------------------
+    def _init_blocks(self, idx):
+        m = self.m
+        for s in idx:
+            sl = self.slots[s]
+            X = self._X(s)
+            if sl.nc:
+                X[:sl.nc].copy_(self.CARRY[s * m:s * m + sl.nc])
+            if sl.nc < m:
+                Z = sl.op.random_block(m, sl.req.seed, kind="normal", row0=sl.block_idx * m)
+                Z = Z[:m - sl.nc]
+                nrm = torch.sqrt(torch.sum(torch.abs(Z.reshape(Z.shape[0], -1)) ** 2, dim=1))
+                X[sl.nc:] = Z / torch.clamp(nrm, min=1e-30)[:, None, None, None, None]
+            sl.block_idx += 1
+            self._P(s).zero_()
+        self._deflate(idx, self._X)
+        self._svqb(idx, self._X)
+        self._theta(idx, self.S, 0, self.HS, 0, 3 * m, 3 * m, m)
+        for s in idx:
+            sl = self.slots[s]
+            sl.phase = "iter"
+            sl.it = 0
+            sl.n_lock = min(m - self.guard, sl.nev - sl.locked_vals.size)
+
+    def _iterate(self, idx, lam):
+        m = self.m
+        # W = K⁻¹R, deflated, projected out of X twice, SVQB (solver.py:251-257)
+        for c in range(0, len(idx), MAX_PROBLEMS):
+            part = idx[c:c + MAX_PROBLEMS]
+            precondition_multi(self.ws, [self.slots[s].op for s in part], self.R,
+                               [s * m for s in part], self.S, [s * 3 * m + m for s in part], m)
+        self._deflate(idx, self._W)
+        for _ in range(2):
+            Xs = [self._X(s) for s in idx]
+            Ws = [self._W(s) for s in idx]
+            combine_multi(-gram_multi(Xs, Ws), Xs, Ws, Xs=Ws, alpha=1.0)
+        self._svqb(idx, self._W)
+        # P projected out of [X, W] twice, fixed-shape whitening (solver.py:258-274)
+        for _ in range(2):
+            XWs = [self._XW(s) for s in idx]
+            Ps = [self._P(s) for s in idx]
+            combine_multi(-gram_multi(XWs, Ps), XWs, Ps, Xs=Ps, alpha=1.0)
+        Ps = [self._P(s) for s in idx]
+        wp, Vp = eigh_batched_host(gram_multi(Ps, Ps))
+        wpmax = np.maximum(wp.max(axis=1, keepdims=True), 1e-300)
+        keepp = (wp > SVQB_DROP * wpmax) & (wpmax > 0)
+        scale = np.where(keepp, 1.0 / np.sqrt(np.where(keepp, wp, 1.0)), 0.0)
+        Ts = [self._rows(self.T1, s, m) for s in idx]
+        combine_multi(torch.as_tensor(Vp * scale[:, None, :], device=self.S.device), Ps, Ts)
+        copy_multi(Ts, Ps)
+        # Θ(W), Θ(P) fresh; Rayleigh–Ritz on [X; W; P] (solver.py:275-302)
+        self._theta(idx, self.S, m, self.HS, m, 3 * m, 3 * m, 2 * m)
+        Sbs = [self._Sb(s) for s in idx]
+        A = gram_multi(Sbs, [self._HSb(s) for s in idx]).cpu().numpy()
+        rn2 = rowreduce_strided(0, self.S, None, 3 * m, 3 * m, 3 * m, self.nslots * 3 * m,
+                                self.n).real.cpu().numpy().reshape(self.nslots, 3 * m)
+        C = np.empty((len(idx), 3 * m, m), np.complex128)
+        for j, s in enumerate(idx):
+            a = (A[j] + A[j].conj().T) / 2
+            dead = np.sqrt(rn2[s]) < 0.5
+            penalty = max(RR_DEAD_PENALTY, 100.0 * float(np.abs(np.diag(a)).max()))
+            a[np.diag_indices_from(a)] += np.where(dead, penalty, 0.0)
+            _, VA = np.linalg.eigh(a)
+            C[j] = VA[:, :m]
+        Ct = torch.as_tensor(C, device=self.S.device)
+        Cp = Ct.clone()
+        Cp[:, :m] = 0
+        T1s = [self._rows(self.T1, s, m) for s in idx]
+        T2s = [self._rows(self.T2, s, m) for s in idx]
+        combine_multi(Ct, Sbs, T1s)
+        combine_multi(Cp, Sbs, T2s)
+        self._deflate(idx, lambda s: self._rows(self.T1, s, m))
+        # normalised rows back into X and P of every slot (rows of slots not
+        # iterating hold nothing that is read again before they are rebuilt)
+        nrm1 = rowreduce_strided(0, self.T1, None, m, m, m, self.nslots * m, self.n).real
+        nrm2 = rowreduce_strided(0, self.T2, None, m, m, m, self.nslots * m, self.n).real
+        live = torch.zeros(self.nslots * m, dtype=torch.bool, device=self.S.device)
+        for s in idx:
+            live[s * m:(s + 1) * m] = True
+        a1 = torch.where(live, 1.0 / torch.clamp(torch.sqrt(nrm1), min=1e-20), torch.ones_like(nrm1))
+        a2 = torch.where(live, 1.0 / torch.clamp(torch.sqrt(nrm2), min=1e-20), torch.ones_like(nrm2))
+        # X rows of non-iterating slots are preserved: copy them through T1 first
+        keep = [s for s in range(self.nslots) if s not in set(idx)]
+        if keep:
+            copy_multi([self._X(s) for s in keep], [self._rows(self.T1, s, m) for s in keep])
+            copy_multi([self._P(s) for s in keep], [self._rows(self.T2, s, m) for s in keep])
+        row_axpby_strided(self.T1, m, a1, None, m, None, m, self.nslots * m, self.n,
+                          self.S, 3 * m)
+        P0 = self.S[2 * m:]
+        row_axpby_strided(self.T2, m, a2, None, m, None, m, self.nslots * m, self.n, P0, 3 * m)
+        self._theta(idx, self.S, 0, self.HS, 0, 3 * m, 3 * m, m)
+        for s in idx:
+            self.slots[s].it += 1
+            self.slots[s].req.iterations += 1
+
+    def _finish_block(self, s, lam, rn, order):
+        """Lock the lowest n_lock rows, keep the rest as carry (solver.py:316-345)."""
+        sl = self.slots[s]
+        m = self.m
+        if not np.isfinite(lam[order[:sl.n_lock]]).all():
+            raise RuntimeError(f"request {sl.req.rid}: rank collapse in block {sl.block_idx}")
+        X = self._X(s)
+        lock_idx = torch.as_tensor(order[:sl.n_lock], device=X.device)
+        carry_idx = torch.as_tensor(order[sl.n_lock:], device=X.device)
+        base = s * self.cap
+        self.L[base + sl.n_locked:base + sl.n_locked + sl.n_lock] = X[lock_idx]
+        sl.nc = int(carry_idx.numel())
+        self.CARRY[s * m:s * m + sl.nc] = X[carry_idx]
+        sl.n_locked += sl.n_lock
+        sl.locked_vals = np.concatenate([sl.locked_vals, lam[order[:sl.n_lock]]])
+        sl.locked_res = np.concatenate([sl.locked_res, rn[order[:sl.n_lock]]])
+        if sl.locked_vals.size >= sl.nev:
+            req = sl.req
+            o = np.argsort(sl.locked_vals, kind="stable")
+            req.vals = sl.locked_vals[o][:sl.nev]
+            req.res = sl.locked_res[o][:sl.nev]
+            if self.keep_vectors:
+                req.vecs = self.L[base:base + sl.n_locked][torch.as_tensor(o, device=X.device)][
+                    :sl.nev].clone()
+            req.t_done = time.perf_counter()
+            self.slots[s] = None
+            return req
+        sl.phase = "init"
+        return None
+
+    # ---- one engine step = one LOBPCG iteration for every active slot -------
+    def step(self) -> list:
+        finished = []
+        act = [s for s, sl in enumerate(self.slots) if sl is not None]
+        if not act:
+            return finished
+        self.steps += 1
+        m = self.m
+        init = [s for s in act if self.slots[s].phase == "init"]
+        if init:
+            self._init_blocks(init)
+        # Rayleigh quotients, residuals, dead rows for the X rows of every slot
+        M = self.nslots * m
+        lam = rowreduce_strided(1, self.S, self.HS, m, 3 * m, 3 * m, M, self.n).real
+        xn2 = rowreduce_strided(0, self.S, None, m, 3 * m, 3 * m, M, self.n).real
+        if self.normalized_rq:
+            lam = lam / torch.clamp(xn2, min=1e-30)
+        row_axpby_strided(self.HS, 3 * m, torch.ones_like(lam), self.S, 3 * m, -lam, m, M,
+                          self.n, self.R, m)
+        rr2 = rowreduce_strided(0, self.R, None, m, m, m, M, self.n).real
+        hx2 = rowreduce_strided(0, self.HS, None, m, 3 * m, 3 * m, M, self.n).real
+        stats = torch.stack([lam, xn2, rr2, hx2]).cpu().numpy()
+        lam_h = stats[0].reshape(self.nslots, m)
+        xn = np.sqrt(stats[1]).reshape(self.nslots, m)
+        rn_h = (np.sqrt(stats[2]) / np.maximum(np.sqrt(stats[3]), 1e-30)).reshape(self.nslots, m)
+        cont = []
+        for s in act:
+            sl = self.slots[s]
+            dead = xn[s] < 0.5
+            l = np.where(dead, np.inf, lam_h[s])
+            r = np.where(dead, np.inf, rn_h[s])
+            order = np.argsort(l)
+            if (r[order[:sl.n_lock]] < self.tol).all() or sl.it >= self.maxit:
+                req = self._finish_block(s, l, r, order)
+                if req is not None:
+                    finished.append(req)
+            else:
+                cont.append(s)
+        if cont:
+            self._iterate(cont, lam)
+        return finished
+
+
+def solve_batched(requests, grid_size, m=24, guard=8, nslots=8, max_nev=None, tol=1e-4,
+                  maxit=400, keep_vectors=False):
+    """Solve a list of same-grid requests through one engine (admission as slots free)."""
+    max_nev = max_nev or max(r.nev for r in requests)
+    eng = BatchedLOBPCG(grid_size, m, guard, nslots, max_nev, tol, maxit, keep_vectors)
+    pending = list(requests)
+    done = []
+    while pending or eng.active():
+        while pending and eng.free_slots():
+            eng.admit(pending.pop(0))
+        done.extend(eng.step())
+    return done

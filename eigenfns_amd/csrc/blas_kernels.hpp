@@ -16,19 +16,32 @@ namespace efn {
 
 // ---------------------------------------------------------------------------
 // Gram partials: part[s][i][j] = Σ_{k in split s} conj(A[i][k])·B[j][k]
+constexpr int kMaxProb = 32;  // problems per multi-problem launch (pointer lists in kernel args)
+struct PairPtrs {
+  const float2* a[kMaxProb];
+  const float2* b[kMaxProb];
+};
+struct CombinePtrs {
+  const float2* s[kMaxProb];
+  const float2* x[kMaxProb];
+  float2* out[kMaxProb];
+};
+
 constexpr int kGT = 64;   // output tile edge
 constexpr int kGK = 32;   // k-chunk staged in LDS
 constexpr int kGPad = kGT + 1;
 
-__global__ __launch_bounds__(256) void gram_partial_kernel(const float2* __restrict__ A, int ma,
-                                                           const float2* __restrict__ B, int mb,
+// grid: (splits, tiles_i·tiles_j, problems); part[(prob·splits + s)·ma·mb + i·mb + j]
+__global__ __launch_bounds__(256) void gram_partial_kernel(PairPtrs pp, int ma, int mb, int tiles_j,
                                                            long long n, long long kper,
                                                            double2* __restrict__ part) {
+  const float2* __restrict__ A = pp.a[blockIdx.z];
+  const float2* __restrict__ B = pp.b[blockIdx.z];
   __shared__ float2 As[kGK * kGPad];
   __shared__ float2 Bs[kGK * kGPad];
   const int tid = threadIdx.x;
   const int ti = tid >> 4, tj = tid & 15;
-  const int i0 = blockIdx.y * kGT, j0 = blockIdx.z * kGT;
+  const int i0 = (blockIdx.y / tiles_j) * kGT, j0 = (blockIdx.y % tiles_j) * kGT;
   const long long kb = (long long)blockIdx.x * kper;
   const long long ke = kb + kper < n ? kb + kper : n;
   double ar[4][4], ai[4][4];
@@ -72,7 +85,7 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(const float2* __restr
     }
     __syncthreads();
   }
-  double2* out = part + (long long)blockIdx.x * ma * mb;
+  double2* out = part + ((long long)blockIdx.z * gridDim.x + blockIdx.x) * ma * mb;
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -82,11 +95,13 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(const float2* __restr
     }
 }
 
-// out[e] = Σ_s part[s][e] in fixed order
+// out[prob][e] = Σ_s part[prob][s][e] in fixed order (grid.y = problem)
 __global__ void sum_partials_kernel(const double2* __restrict__ part, int nsplit, long long count,
                                     double2* __restrict__ out) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= count) return;
+  part += (long long)blockIdx.y * nsplit * count;
+  out += (long long)blockIdx.y * count;
   double re = 0.0, im = 0.0;
   for (int s = 0; s < nsplit; ++s) {
     const double2 p = part[(long long)s * count + e];
@@ -98,11 +113,14 @@ __global__ void sum_partials_kernel(const double2* __restrict__ part, int nsplit
 
 // ---------------------------------------------------------------------------
 // combine: out[i][k] = alpha·X[i][k] + Σ_j C[j][i]·S[j][k]   (C is m_in × m_out, c64)
+// grid: (columns/256, row tiles, problems); C is problems × m_in × m_out
 template <int TI>
-__global__ __launch_bounds__(256) void combine_kernel(const float2* __restrict__ C, int m_in, int m_out,
-                                                      const float2* __restrict__ S, long long n,
-                                                      const float2* __restrict__ X, float alpha,
-                                                      float2* __restrict__ out) {
+__global__ __launch_bounds__(256) void combine_kernel(const float2* __restrict__ Call, int m_in, int m_out,
+                                                      CombinePtrs cp, long long n, float alpha) {
+  const float2* __restrict__ C = Call + (long long)blockIdx.z * m_in * m_out;
+  const float2* __restrict__ S = cp.s[blockIdx.z];
+  const float2* __restrict__ X = cp.x[blockIdx.z];
+  float2* __restrict__ out = cp.out[blockIdx.z];
   const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
   const int i0 = blockIdx.y * TI;
   if (k >= n) return;
@@ -147,17 +165,23 @@ __device__ __forceinline__ double2 wave_sum2(double2 v) {
   return v;
 }
 
+// Logical row i of a strided row set: physical row (i / nrow)·stride + i % nrow.
+__device__ __forceinline__ long long phys_row(int i, int nrow, long long stride) {
+  const int slot = i / nrow;
+  return (long long)slot * stride + (i - slot * nrow);
+}
+
 template <int OP>
-__global__ __launch_bounds__(256) void rowreduce_partial_kernel(const float2* __restrict__ A,
-                                                                const float2* __restrict__ B,
-                                                                long long n, long long kper,
+__global__ __launch_bounds__(256) void rowreduce_partial_kernel(const float2* __restrict__ A, long long sa,
+                                                                const float2* __restrict__ B, long long sb,
+                                                                int nrow, long long n, long long kper,
                                                                 double2* __restrict__ part) {
   __shared__ double2 red[4];
   const int i = blockIdx.y, s = blockIdx.x, nsplit = gridDim.x;
   const long long kb = (long long)s * kper;
   const long long ke = kb + kper < n ? kb + kper : n;
-  const float2* a = A + (long long)i * n;
-  const float2* b = B + (long long)i * n;
+  const float2* a = A + phys_row(i, nrow, sa) * n;
+  const float2* b = B + phys_row(i, nrow, sb) * n;
   double re = 0.0, im = 0.0;
   for (long long k = kb + threadIdx.x; k < ke; k += 256) {
     const float2 x = a[k];
@@ -194,22 +218,24 @@ __global__ void rowreduce_final_kernel(const double2* __restrict__ part, int nsp
 // ---------------------------------------------------------------------------
 // out[i][k] = a[i]·X[i][k] + b[i]·Y[i][k] (a, b real per row, FP64 on device,
 // applied in FP32).  Y may be null.
-__global__ void row_axpby_kernel(const float2* __restrict__ X, const double* __restrict__ a,
-                                 const float2* __restrict__ Y, const double* __restrict__ b,
-                                 long long n, float2* __restrict__ out) {
+__global__ void row_axpby_kernel(const float2* __restrict__ X, long long sx, const double* __restrict__ a,
+                                 const float2* __restrict__ Y, long long sy, const double* __restrict__ b,
+                                 int nrow, long long n, float2* __restrict__ out, long long so) {
   const int i = blockIdx.y;
   const float ai = (float)a[i];
   const float bi = Y ? (float)b[i] : 0.f;
-  const long long base = (long long)i * n;
+  const float2* xr = X + phys_row(i, nrow, sx) * n;
+  const float2* yr = Y ? Y + phys_row(i, nrow, sy) * n : nullptr;
+  float2* orow = out + phys_row(i, nrow, so) * n;
   for (long long k = (long long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long long)gridDim.x * 256) {
-    const float2 x = X[base + k];
+    const float2 x = xr[k];
     float2 r = make_float2(ai * x.x, ai * x.y);
-    if (Y) {
-      const float2 y = Y[base + k];
+    if (yr) {
+      const float2 y = yr[k];
       r.x = fmaf(bi, y.x, r.x);
       r.y = fmaf(bi, y.y, r.y);
     }
-    out[base + k] = r;
+    orow[k] = r;
   }
 }
 
@@ -226,6 +252,18 @@ __global__ void axpby_kernel(const float2* __restrict__ X, float a, const float2
     }
     out[e] = r;
   }
+}
+
+// dst_p[e] = src_p[e] for e < count, problem p = blockIdx.y (NULL src = zero fill)
+struct CopyPtrs {
+  const float2* src[kMaxProb];
+  float2* dst[kMaxProb];
+};
+__global__ void copy_multi_kernel(CopyPtrs cp, long long count) {
+  const float2* __restrict__ src = cp.src[blockIdx.y];
+  float2* __restrict__ dst = cp.dst[blockIdx.y];
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < count; e += (long long)gridDim.x * 256)
+    dst[e] = src ? src[e] : make_float2(0.f, 0.f);
 }
 
 // ---------------------------------------------------------------------------

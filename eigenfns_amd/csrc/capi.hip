@@ -41,9 +41,10 @@ using efn::FrameTables;
 struct OpRequest {
   int kind;              // efn::kTheta / kPrecond
   int epi;               // efn::Epi
-  const float2* in;      // T_k (operator input)
-  int m;
-  EpiArgs ea;            // pointers for row 0; advanced per chunk
+  const float2* in;      // operator input base (T_k)
+  int m;                 // vectors in the launch (slots × rows)
+  efn::VecMap vm;        // per-slot operator and block offsets
+  EpiArgs ea;            // output / epilogue base pointers
   int reduce_mode;       // 0 none, 1 KPM moments, 2 KPM count
   double* mom;           // KPM: moments base
   int ld;
@@ -71,6 +72,7 @@ struct efn_ctx {
   float2* d_ws = nullptr;        // chunk·3·G³ complex64
   double* d_partial = nullptr;   // chunk·(G²/R)·2
   double2* d_red = nullptr;      // chunk·2 doubles of reduced sums
+  efn::OpDesc* d_desc = nullptr; // this operator's descriptor for the pass kernels
   size_t ws_bytes = 0;
   int (*run)(efn_ctx*, const OpRequest&, hipStream_t) = nullptr;
 
@@ -126,48 +128,41 @@ template <int G>
 int run_op(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
   using namespace efn;
   constexpr int R = zrows3<G>();
-  constexpr long long G3 = (long long)G * G * G;
   constexpr int nzt = (G + ztile_for(G) - 1) / ztile_for(G);
   constexpr int nb = G * G / R;  // zin/zout blocks per vector
-  const FrameTables ft = c->tables();
-  const float* rmul = rq.kind == kTheta ? c->d_inv_eps : c->d_eps;
+  const int use_eps = rq.kind == kTheta ? 0 : 1;
   for (int v0 = 0; v0 < rq.m; v0 += c->chunk) {
     const int nv = rq.m - v0 < c->chunk ? rq.m - v0 : c->chunk;
-    const long long off = (long long)v0 * 2 * G3;
-    const float2* x = rq.in + off;
     const long long tiles = (long long)nv * G * nzt;  // x-pass tiles per component
-    const dim3 gin((unsigned)(nv * nb)), gz((unsigned)(nv * nb)), gs((unsigned)(nv * 3 * G * nzt)),
-        gx((unsigned)(3 * tiles)), blk(kThreads);
+    const dim3 gz((unsigned)(nv * nb)), gs((unsigned)(nv * 3 * G * nzt)), gx((unsigned)(3 * tiles)),
+        blk(kThreads);
     if (rq.kind == kTheta)
-      hipLaunchKernelGGL((zin_kernel<G, kTheta>), gin, blk, 0, s, x, c->d_ws, nv, ft, c->d_twid);
+      hipLaunchKernelGGL((zin_kernel<G, kTheta>), gz, blk, 0, s, rq.in, c->d_ws, rq.vm, (long long)v0, c->d_twid);
     else
-      hipLaunchKernelGGL((zin_kernel<G, kPrecond>), gin, blk, 0, s, x, c->d_ws, nv, ft, c->d_twid);
+      hipLaunchKernelGGL((zin_kernel<G, kPrecond>), gz, blk, 0, s, rq.in, c->d_ws, rq.vm, (long long)v0, c->d_twid);
     EFN_LAUNCH_CHECK();
     hipLaunchKernelGGL((strided_kernel<G, 1, +1>), gs, blk, 0, s, c->d_ws, c->d_twid);
-    hipLaunchKernelGGL((xmid_kernel<G>), gx, blk, 0, s, c->d_ws, rmul, c->d_twid, tiles);
+    hipLaunchKernelGGL((xmid_kernel<G>), gx, blk, 0, s, c->d_ws, rq.vm, (long long)v0, use_eps, c->d_twid, tiles);
     hipLaunchKernelGGL((strided_kernel<G, 1, -1>), gs, blk, 0, s, c->d_ws, c->d_twid);
     EFN_LAUNCH_CHECK();
     EpiArgs ea = rq.ea;
-    ea.x = rq.in + off;
-    ea.out = rq.ea.out + off;
-    if (rq.ea.y) ea.y = rq.ea.y + off;
-    if (rq.ea.z) ea.z = rq.ea.z + off;
+    ea.x = rq.in;
     ea.partial = c->d_partial;
     if (rq.kind == kPrecond) {
-      hipLaunchKernelGGL((zout_kernel<G, kPrecond, kStore>), gz, blk, 0, s, c->d_ws, nv, ft, c->d_twid, ea);
+      hipLaunchKernelGGL((zout_kernel<G, kPrecond, kStore>), gz, blk, 0, s, c->d_ws, rq.vm, (long long)v0, c->d_twid, ea);
     } else {
       switch (rq.epi) {
         case kStore:
-          hipLaunchKernelGGL((zout_kernel<G, kTheta, kStore>), gz, blk, 0, s, c->d_ws, nv, ft, c->d_twid, ea);
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kStore>), gz, blk, 0, s, c->d_ws, rq.vm, (long long)v0, c->d_twid, ea);
           break;
         case kCheb:
-          hipLaunchKernelGGL((zout_kernel<G, kTheta, kCheb>), gz, blk, 0, s, c->d_ws, nv, ft, c->d_twid, ea);
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kCheb>), gz, blk, 0, s, c->d_ws, rq.vm, (long long)v0, c->d_twid, ea);
           break;
         case kKpm:
-          hipLaunchKernelGGL((zout_kernel<G, kTheta, kKpm>), gz, blk, 0, s, c->d_ws, nv, ft, c->d_twid, ea);
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kKpm>), gz, blk, 0, s, c->d_ws, rq.vm, (long long)v0, c->d_twid, ea);
           break;
         default:
-          hipLaunchKernelGGL((zout_kernel<G, kTheta, kDot>), gz, blk, 0, s, c->d_ws, nv, ft, c->d_twid, ea);
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kDot>), gz, blk, 0, s, c->d_ws, rq.vm, (long long)v0, c->d_twid, ea);
           break;
       }
     }
@@ -219,14 +214,16 @@ void destroy(efn_ctx* c) {
   free_dev(c->d_ws);
   free_dev(c->d_partial);
   free_dev(c->d_red);
+  free_dev(c->d_desc);
   delete c;
 }
 
-int gram_nsplit(int ma, int mb, long long n) {
-  const long long tiles = (long long)((ma + efn::kGT - 1) / efn::kGT) * ((mb + efn::kGT - 1) / efn::kGT);
-  long long want = 1024 / (tiles > 0 ? tiles : 1);
-  long long maxsplit = (n + 4095) / 4096;  // at least 4096 entries per split
-  long long ns = want < maxsplit ? want : maxsplit;
+// Splits of the k range depend on n only, so a problem's Gram is summed in the
+// same order whether it runs alone or batched with others (and whatever its
+// row counts): results are bitwise batch-invariant.
+int gram_nsplit(int, int, long long n) {
+  long long ns = (n + 4095) / 4096;  // at least 4096 entries per split
+  if (ns > 256) ns = 256;
   return (int)(ns < 1 ? 1 : ns);
 }
 
@@ -320,7 +317,7 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   if (chunk <= 0) {
     chunk = (int)((size_t(192) << 20) / per_vec);
     if (chunk < 1) chunk = 1;
-    if (chunk > 64) chunk = 64;
+    if (chunk > 1024) chunk = 1024;
   }
   c->chunk = chunk;
   c->ws_bytes = per_vec * chunk;
@@ -342,6 +339,7 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   EFN_ALLOC(c->d_ws, c->ws_bytes);
   EFN_ALLOC(c->d_partial, (size_t)chunk * nb * 2 * sizeof(double));
   EFN_ALLOC(c->d_red, (size_t)chunk * sizeof(double2));
+  EFN_ALLOC(c->d_desc, sizeof(efn::OpDesc));
 #undef EFN_ALLOC
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMemcpy(c->d_axes, axes.data(), axes.size() * sizeof(float), hipMemcpyHostToDevice);
@@ -350,6 +348,13 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   if (e == hipSuccess) e = hipMemcpy(c->d_eps, epsv.data(), G3 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_kn2, kn2f.data(), G3 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_twid, twid.data(), G * sizeof(float2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    efn::OpDesc d;
+    d.ft = c->tables();
+    d.inv_eps = c->d_inv_eps;
+    d.eps = c->d_eps;
+    e = hipMemcpy(c->d_desc, &d, sizeof(d), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) {
     destroy(c);
     return fail(EFN_EHIP, std::string("upload of operator tables: ") + hipGetErrorString(e));
@@ -419,6 +424,16 @@ int efn_ctx_basis(const efn_ctx* c, float* kn_dev, float* t_dev, void* stream) {
   return EFN_OK;
 }
 
+static efn::VecMap single_map(const efn_ctx* c, int m) {
+  efn::VecMap vm;
+  memset(&vm, 0, sizeof(vm));
+  vm.desc[0] = c->d_desc;
+  vm.in_off[0] = 0;
+  vm.out_off[0] = 0;
+  vm.nrow = m > 0 ? m : 1;
+  return vm;
+}
+
 static int check_block(const efn_ctx* c, const void* p, int m, const char* what) {
   if (!c) return fail(EFN_EINVAL, "ctx is NULL");
   if (m < 0) return fail(EFN_EINVAL, "negative block size");
@@ -436,6 +451,7 @@ int efn_theta(efn_ctx* c, const void* in, void* out, int m, float alpha, float b
   rq.epi = efn::kStore;
   rq.in = (const float2*)in;
   rq.m = m;
+  rq.vm = single_map(c, m);
   rq.ea.alpha = alpha;
   rq.ea.beta = beta;
   rq.ea.out = (float2*)out;
@@ -452,6 +468,7 @@ int efn_precondition(efn_ctx* c, const void* in, void* out, int m, void* stream)
   rq.epi = efn::kStore;
   rq.in = (const float2*)in;
   rq.m = m;
+  rq.vm = single_map(c, m);
   rq.ea.alpha = 1.f;
   rq.ea.beta = 0.f;
   rq.ea.out = (float2*)out;
@@ -483,6 +500,7 @@ int efn_cheb_step(efn_ctx* c, const void* T1, void* T0, void* Y, float alpha, fl
   rq.epi = efn::kCheb;
   rq.in = (const float2*)T1;
   rq.m = m;
+  rq.vm = single_map(c, m);
   rq.ea.alpha = alpha;
   rq.ea.ck = ck;
   rq.ea.out = (float2*)T0;
@@ -502,6 +520,7 @@ int efn_kpm_step(efn_ctx* c, const void* T1, void* T0, double* mom, int ld, int 
   rq.epi = efn::kKpm;
   rq.in = (const float2*)T1;
   rq.m = m;
+  rq.vm = single_map(c, m);
   rq.ea.alpha = alpha;
   rq.ea.out = (float2*)T0;
   rq.reduce_mode = 1;
@@ -525,6 +544,7 @@ int efn_kpm_count_step(efn_ctx* c, const void* T1, void* T0, const void* Z, doub
   rq.epi = efn::kDot;
   rq.in = (const float2*)T1;
   rq.m = m;
+  rq.vm = single_map(c, m);
   rq.ea.alpha = alpha;
   rq.ea.out = (float2*)T0;
   rq.ea.z = (const float2*)Z;
@@ -534,56 +554,151 @@ int efn_kpm_count_step(efn_ctx* c, const void* T1, void* T0, const void* Z, doub
   return c->run(c, rq, (hipStream_t)stream);
 }
 
+static int build_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
+                       const int64_t* in_off, void* out_base, const int64_t* out_off, int nrow,
+                       OpRequest& rq) {
+  if (!ws) return fail(EFN_EINVAL, "workspace ctx is NULL");
+  if (nslot < 0 || nslot > efn::kMaxSlots)
+    return fail(EFN_EINVAL, "nslot must be in [0, " + std::to_string(efn::kMaxSlots) + "]");
+  if (nrow < 0) return fail(EFN_EINVAL, "negative rows per slot");
+  if (nslot == 0 || nrow == 0) return 1;  // nothing to do
+  if (!ctxs || !in_off || !out_off || !in_base || !out_base) return fail(EFN_EINVAL, "NULL pointer");
+  memset(&rq.vm, 0, sizeof(rq.vm));
+  for (int i = 0; i < nslot; ++i) {
+    if (!ctxs[i]) return fail(EFN_EINVAL, "NULL ctx in slot list");
+    if (ctxs[i]->G != ws->G) return fail(EFN_EINVAL, "all slots must share the workspace grid size");
+    if (in_off[i] < 0 || out_off[i] < 0) return fail(EFN_EINVAL, "negative slot offset");
+    rq.vm.desc[i] = ctxs[i]->d_desc;
+    rq.vm.in_off[i] = in_off[i];
+    rq.vm.out_off[i] = out_off[i];
+  }
+  rq.vm.nrow = nrow;
+  rq.in = (const float2*)in_base;
+  rq.m = nslot * nrow;
+  rq.ea.out = (float2*)out_base;
+  return 0;
+}
+
+int efn_theta_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
+                    const int64_t* in_off, void* out_base, const int64_t* out_off, int nrow,
+                    float alpha, float beta, void* stream) {
+  OpRequest rq{};
+  rq.kind = efn::kTheta;
+  rq.epi = efn::kStore;
+  const int rc = build_multi(ws, nslot, ctxs, in_base, in_off, out_base, out_off, nrow, rq);
+  if (rc) return rc > 0 ? EFN_OK : rc;
+  if (in_base == out_base) {
+    for (int i = 0; i < nslot; ++i)
+      for (int j = 0; j < nslot; ++j)
+        if (in_off[i] < out_off[j] + nrow && out_off[j] < in_off[i] + nrow)
+          return fail(EFN_EINVAL, "input and output rows overlap");
+  }
+  rq.ea.alpha = alpha;
+  rq.ea.beta = beta;
+  return ws->run(ws, rq, (hipStream_t)stream);
+}
+
+int efn_precondition_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
+                           const int64_t* in_off, void* out_base, const int64_t* out_off, int nrow,
+                           void* stream) {
+  OpRequest rq{};
+  rq.kind = efn::kPrecond;
+  rq.epi = efn::kStore;
+  const int rc = build_multi(ws, nslot, ctxs, in_base, in_off, out_base, out_off, nrow, rq);
+  if (rc) return rc > 0 ? EFN_OK : rc;
+  if (in_base == out_base) {
+    for (int i = 0; i < nslot; ++i)
+      for (int j = 0; j < nslot; ++j)
+        if (in_off[i] < out_off[j] + nrow && out_off[j] < in_off[i] + nrow)
+          return fail(EFN_EINVAL, "input and output rows overlap");
+  }
+  rq.ea.alpha = 1.f;
+  rq.ea.beta = 0.f;
+  return ws->run(ws, rq, (hipStream_t)stream);
+}
+
 size_t efn_gram_work_bytes(int ma, int mb, int64_t n) {
   if (ma <= 0 || mb <= 0 || n <= 0) return 0;
   return (size_t)gram_nsplit(ma, mb, n) * ma * mb * sizeof(double2);
 }
 
-int efn_gram(const void* A, int ma, const void* B, int mb, int64_t n, void* out, void* work,
-             size_t work_bytes, void* stream) {
+size_t efn_gram_multi_work_bytes(int nprob, int ma, int mb, int64_t n) {
+  return nprob <= 0 ? 0 : (size_t)nprob * efn_gram_work_bytes(ma, mb, n);
+}
+
+int efn_gram_multi(int nprob, const void* const* A, const void* const* B, int ma, int mb, int64_t n,
+                   void* out, void* work, size_t work_bytes, void* stream) {
+  if (nprob < 0 || nprob > efn::kMaxProb)
+    return fail(EFN_EINVAL, "nprob must be in [0, " + std::to_string(efn::kMaxProb) + "]");
   if (ma < 0 || mb < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
-  if (ma == 0 || mb == 0) return EFN_OK;
+  if (nprob == 0 || ma == 0 || mb == 0) return EFN_OK;
   if (!A || !B || !out) return fail(EFN_EINVAL, "NULL pointer");
   hipStream_t s = (hipStream_t)stream;
   if (n == 0) {
-    EFN_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)ma * mb * sizeof(double2), s));
+    EFN_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)nprob * ma * mb * sizeof(double2), s));
     return EFN_OK;
   }
-  const size_t need = efn_gram_work_bytes(ma, mb, n);
+  efn::PairPtrs pp;
+  memset(&pp, 0, sizeof(pp));
+  for (int i = 0; i < nprob; ++i) {
+    if (!A[i] || !B[i]) return fail(EFN_EINVAL, "NULL operand pointer");
+    pp.a[i] = (const float2*)A[i];
+    pp.b[i] = (const float2*)B[i];
+  }
+  const size_t need = efn_gram_multi_work_bytes(nprob, ma, mb, n);
   if (!work || work_bytes < need) return fail(EFN_EINVAL, "gram work buffer too small");
   const int ns = gram_nsplit(ma, mb, n);
   long long kper = (n + ns - 1) / ns;
   kper = (kper + efn::kGK - 1) / efn::kGK * efn::kGK;
-  const dim3 g((unsigned)ns, (unsigned)((ma + efn::kGT - 1) / efn::kGT), (unsigned)((mb + efn::kGT - 1) / efn::kGT));
-  hipLaunchKernelGGL(efn::gram_partial_kernel, g, dim3(256), 0, s, (const float2*)A, ma,
-                     (const float2*)B, mb, (long long)n, kper, (double2*)work);
+  const int ti = (ma + efn::kGT - 1) / efn::kGT, tj = (mb + efn::kGT - 1) / efn::kGT;
+  hipLaunchKernelGGL(efn::gram_partial_kernel, dim3((unsigned)ns, (unsigned)(ti * tj), (unsigned)nprob),
+                     dim3(256), 0, s, pp, ma, mb, tj, (long long)n, kper, (double2*)work);
   EFN_LAUNCH_CHECK();
   const long long count = (long long)ma * mb;
-  hipLaunchKernelGGL(efn::sum_partials_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
-                     (const double2*)work, ns, count, (double2*)out);
+  hipLaunchKernelGGL(efn::sum_partials_kernel, dim3((unsigned)((count + 255) / 256), (unsigned)nprob),
+                     dim3(256), 0, s, (const double2*)work, ns, count, (double2*)out);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+int efn_gram(const void* A, int ma, const void* B, int mb, int64_t n, void* out, void* work,
+             size_t work_bytes, void* stream) {
+  return efn_gram_multi(1, &A, &B, ma, mb, n, out, work, work_bytes, stream);
+}
+
+int efn_combine_multi(int nprob, const void* C, const void* const* S, int m_in, int m_out, int64_t n,
+                      const void* const* X, float alpha, void* const* out, void* stream) {
+  if (nprob < 0 || nprob > efn::kMaxProb)
+    return fail(EFN_EINVAL, "nprob must be in [0, " + std::to_string(efn::kMaxProb) + "]");
+  if (m_in < 0 || m_out < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
+  if (nprob == 0 || m_out == 0 || n == 0) return EFN_OK;
+  if (!out || (m_in > 0 && (!C || !S))) return fail(EFN_EINVAL, "NULL pointer");
+  efn::CombinePtrs cp;
+  memset(&cp, 0, sizeof(cp));
+  for (int i = 0; i < nprob; ++i) {
+    if (!out[i] || (m_in > 0 && !S[i])) return fail(EFN_EINVAL, "NULL block pointer");
+    if (m_in > 0 && out[i] == S[i]) return fail(EFN_EINVAL, "out must not alias S");
+    cp.s[i] = m_in > 0 ? (const float2*)S[i] : nullptr;
+    cp.x[i] = X ? (const float2*)X[i] : nullptr;
+    cp.out[i] = (float2*)out[i];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned gx = (unsigned)((n + 255) / 256);
+  if (m_out <= 8) {
+    hipLaunchKernelGGL((efn::combine_kernel<8>), dim3(gx, (m_out + 7) / 8, nprob), dim3(256), 0, s,
+                       (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
+  } else {
+    hipLaunchKernelGGL((efn::combine_kernel<16>), dim3(gx, (m_out + 15) / 16, nprob), dim3(256), 0, s,
+                       (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
+  }
   EFN_LAUNCH_CHECK();
   return EFN_OK;
 }
 
 int efn_combine(const void* C, int m_in, int m_out, const void* S, int64_t n, const void* X,
                 float alpha, void* out, void* stream) {
-  if (m_in < 0 || m_out < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
-  if (m_out == 0 || n == 0) return EFN_OK;
-  if (!out || (m_in > 0 && (!C || !S))) return fail(EFN_EINVAL, "NULL pointer");
-  if (out == S) return fail(EFN_EINVAL, "out must not alias S");
-  hipStream_t s = (hipStream_t)stream;
-  const unsigned gx = (unsigned)((n + 255) / 256);
-  if (m_out <= 8) {
-    hipLaunchKernelGGL((efn::combine_kernel<8>), dim3(gx, (m_out + 7) / 8), dim3(256), 0, s,
-                       (const float2*)C, m_in, m_out, (const float2*)S, (long long)n,
-                       (const float2*)X, alpha, (float2*)out);
-  } else {
-    hipLaunchKernelGGL((efn::combine_kernel<16>), dim3(gx, (m_out + 15) / 16), dim3(256), 0, s,
-                       (const float2*)C, m_in, m_out, (const float2*)S, (long long)n,
-                       (const float2*)X, alpha, (float2*)out);
-  }
-  EFN_LAUNCH_CHECK();
-  return EFN_OK;
+  void* o = out;
+  return efn_combine_multi(1, C, &S, m_in, m_out, n, X ? &X : nullptr, alpha, &o, stream);
 }
 
 size_t efn_rowreduce_work_bytes(int m, int64_t n) {
@@ -591,11 +706,14 @@ size_t efn_rowreduce_work_bytes(int m, int64_t n) {
   return (size_t)rowreduce_nsplit(n) * m * sizeof(double2);
 }
 
-int efn_rowreduce(int op, const void* A, const void* B, int m, int64_t n, void* out, void* work,
-                  size_t work_bytes, void* stream) {
+int efn_rowreduce_strided(int op, const void* A, int64_t stride_a, const void* B, int64_t stride_b,
+                          int nrow, int m, int64_t n, void* out, void* work, size_t work_bytes,
+                          void* stream) {
   if (op != 0 && op != 1) return fail(EFN_EINVAL, "op must be 0 or 1");
   if (m < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
   if (m == 0) return EFN_OK;
+  if (nrow <= 0 || stride_a < nrow || (op == 1 && stride_b < nrow))
+    return fail(EFN_EINVAL, "row map needs 0 < nrow <= stride");
   if (!A || !out || (op == 1 && !B)) return fail(EFN_EINVAL, "NULL pointer");
   hipStream_t s = (hipStream_t)stream;
   if (n == 0) {
@@ -608,10 +726,12 @@ int efn_rowreduce(int op, const void* A, const void* B, int m, int64_t n, void* 
   const long long kper = (n + ns - 1) / ns;
   if (op == 0)
     hipLaunchKernelGGL((efn::rowreduce_partial_kernel<0>), dim3(ns, m), dim3(256), 0, s,
-                       (const float2*)A, (const float2*)A, (long long)n, kper, (double2*)work);
+                       (const float2*)A, (long long)stride_a, (const float2*)A, (long long)stride_a,
+                       nrow, (long long)n, kper, (double2*)work);
   else
     hipLaunchKernelGGL((efn::rowreduce_partial_kernel<1>), dim3(ns, m), dim3(256), 0, s,
-                       (const float2*)A, (const float2*)B, (long long)n, kper, (double2*)work);
+                       (const float2*)A, (long long)stride_a, (const float2*)B, (long long)stride_b,
+                       nrow, (long long)n, kper, (double2*)work);
   EFN_LAUNCH_CHECK();
   hipLaunchKernelGGL(efn::rowreduce_final_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
                      (const double2*)work, ns, m, (double2*)out);
@@ -619,18 +739,33 @@ int efn_rowreduce(int op, const void* A, const void* B, int m, int64_t n, void* 
   return EFN_OK;
 }
 
-int efn_row_axpby(const void* X, const double* a, const void* Y, const double* b, int m, int64_t n,
-                  void* out, void* stream) {
+int efn_rowreduce(int op, const void* A, const void* B, int m, int64_t n, void* out, void* work,
+                  size_t work_bytes, void* stream) {
+  const int nrow = m > 0 ? m : 1;
+  return efn_rowreduce_strided(op, A, nrow, B, nrow, nrow, m, n, out, work, work_bytes, stream);
+}
+
+int efn_row_axpby_strided(const void* X, int64_t stride_x, const double* a, const void* Y,
+                          int64_t stride_y, const double* b, int nrow, int m, int64_t n, void* out,
+                          int64_t stride_out, void* stream) {
   if (m < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
   if (m == 0 || n == 0) return EFN_OK;
   if (!X || !a || !out || (Y && !b)) return fail(EFN_EINVAL, "NULL pointer");
+  if (nrow <= 0 || stride_x < nrow || stride_out < nrow || (Y && stride_y < nrow))
+    return fail(EFN_EINVAL, "row map needs 0 < nrow <= stride");
   long long gx = (n + 255) / 256;
   if (gx > 4096) gx = 4096;
   hipLaunchKernelGGL(efn::row_axpby_kernel, dim3((unsigned)gx, (unsigned)m), dim3(256), 0,
-                     (hipStream_t)stream, (const float2*)X, a, (const float2*)Y, b, (long long)n,
-                     (float2*)out);
+                     (hipStream_t)stream, (const float2*)X, (long long)stride_x, a, (const float2*)Y,
+                     (long long)stride_y, b, nrow, (long long)n, (float2*)out, (long long)stride_out);
   EFN_LAUNCH_CHECK();
   return EFN_OK;
+}
+
+int efn_row_axpby(const void* X, const double* a, const void* Y, const double* b, int m, int64_t n,
+                  void* out, void* stream) {
+  const int nrow = m > 0 ? m : 1;
+  return efn_row_axpby_strided(X, nrow, a, Y, nrow, b, nrow, m, n, out, nrow, stream);
 }
 
 int efn_axpby(const void* X, float a, const void* Y, float b, int64_t count, void* out, void* stream) {
@@ -639,6 +774,25 @@ int efn_axpby(const void* X, float a, const void* Y, float b, int64_t count, voi
   if (!X || !out) return fail(EFN_EINVAL, "NULL pointer");
   hipLaunchKernelGGL(efn::axpby_kernel, dim3(grid_for(count)), dim3(256), 0, (hipStream_t)stream,
                      (const float2*)X, a, (const float2*)Y, b, (long long)count, (float2*)out);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+int efn_copy_multi(int nprob, const void* const* src, void* const* dst, int64_t count, void* stream) {
+  if (nprob < 0 || nprob > efn::kMaxProb)
+    return fail(EFN_EINVAL, "nprob must be in [0, " + std::to_string(efn::kMaxProb) + "]");
+  if (count < 0) return fail(EFN_EINVAL, "negative size");
+  if (nprob == 0 || count == 0) return EFN_OK;
+  if (!dst) return fail(EFN_EINVAL, "NULL pointer");
+  efn::CopyPtrs cp;
+  memset(&cp, 0, sizeof(cp));
+  for (int i = 0; i < nprob; ++i) {
+    if (!dst[i]) return fail(EFN_EINVAL, "NULL destination");
+    cp.src[i] = src ? (const float2*)src[i] : nullptr;
+    cp.dst[i] = (float2*)dst[i];
+  }
+  hipLaunchKernelGGL(efn::copy_multi_kernel, dim3(grid_for(count), (unsigned)nprob), dim3(256), 0,
+                     (hipStream_t)stream, cp, (long long)count);
   EFN_LAUNCH_CHECK();
   return EFN_OK;
 }

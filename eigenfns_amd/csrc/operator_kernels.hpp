@@ -38,6 +38,41 @@ struct Frame {
   float kn, t1x, t1y, t1z, t2x, t2y, t2z;
 };
 
+// Per-operator device descriptor (one per context, in device memory).
+struct OpDesc {
+  FrameTables ft;
+  const float* inv_eps;     // ε⁻¹/G³ per voxel
+  const float* eps;         // ε/G³ per voxel
+};
+
+// Vector map of one launch: launch-local vector v belongs to slot v / nrow and
+// is row v % nrow of that slot; the slot's operator, input and output blocks
+// are given per slot (offsets in vectors from the launch's base pointers).
+// One slot is the plain single-operator call; several slots batch operators of
+// one grid size (k-points, structures, requests) into the same passes.
+constexpr int kMaxSlots = 32;
+struct VecMap {
+  const OpDesc* desc[kMaxSlots];
+  long long in_off[kMaxSlots];
+  long long out_off[kMaxSlots];
+  int nrow;
+};
+
+struct VecRef {
+  const OpDesc* desc;
+  long long in_vec, out_vec;  // absolute vector indices from the base pointers
+};
+
+__device__ __forceinline__ VecRef map_vector(const VecMap& vm, long long gv) {
+  const int slot = int(gv / vm.nrow);
+  const long long row = gv - (long long)slot * vm.nrow;
+  VecRef r;
+  r.desc = vm.desc[slot];
+  r.in_vec = vm.in_off[slot] + row;
+  r.out_vec = vm.out_off[slot] + row;
+  return r;
+}
+
 // The transverse frame of operator.py:53-71 (t̂₁ = K×ref/|K×ref|,
 // t̂₂ = K×t̂₁/|K×t̂₁|, zero at K = 0; ref = x̂ where the FP64 switch bit is set,
 // else ẑ) in branch-free closed forms, algebraically identical, in FP32:
@@ -101,8 +136,8 @@ constexpr int kThreads = 256;
 // 3·R pencils; step-3 outputs go straight to HBM.
 template <int G, int KIND>
 __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict__ in,
-                                                       float2* __restrict__ ws, int nvec,
-                                                       FrameTables ft,
+                                                       float2* __restrict__ ws, VecMap vm,
+                                                       long long v0,
                                                        const float2* __restrict__ twid_g) {
   using P = Plan<G>;
   constexpr int N1 = P::N1, N2 = P::N2;
@@ -114,7 +149,9 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
   const long long row0 = (long long)blockIdx.x * R;  // flattened (v, x, y)
   const int v = int(row0 / G2);
   const long long xy0 = row0 - (long long)v * G2;
-  const float2* a_in = in + (long long)v * 2 * G3;
+  const VecRef vr = map_vector(vm, v0 + v);
+  const FrameTables ft = vr.desc->ft;
+  const float2* a_in = in + vr.in_vec * 2 * G3;
   const float2* b_in = a_in + G3;
   for (int e = threadIdx.x; e < R * G; e += kThreads) {
     const int r = e / G, z = e - r * G;
@@ -208,8 +245,8 @@ __global__ __launch_bounds__(kThreads) void strided_kernel(float2* __restrict__ 
 // the same XCD, back to back, so the rmul tile is fetched from HBM once and
 // served from that XCD's L2 to the other two.
 template <int G>
-__global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
-                                                        const float* __restrict__ rmul,
+__global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws, VecMap vm,
+                                                        long long v0, int use_eps,
                                                         const float2* __restrict__ twid_g,
                                                         long long tiles) {
   using P = Plan<G>;
@@ -237,7 +274,8 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   const int o = rem / nzt;  // y
   const int z0 = (rem - o * nzt) * ZT;
   float2* base = ws + (v * 3 + c) * G3 + (long long)o * G + z0;
-  const float* eps = rmul + (long long)o * G + z0;
+  const OpDesc* dsc = map_vector(vm, v0 + v).desc;
+  const float* eps = (use_eps ? dsc->eps : dsc->inv_eps) + (long long)o * G + z0;
   // inverse step 1: HBM → registers → LDS
   for (int t = threadIdx.x; t < ZT * N2; t += kThreads) {
     const int z = t % ZT, n2 = t / ZT;
@@ -332,8 +370,8 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // step 3 writes back to LDS in natural order; the epilogue reads the three
 // components per point and applies the fused update with coalesced HBM access.
 template <int G, int KIND, int EPI>
-__global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict__ ws, int nvec,
-                                                        FrameTables ft,
+__global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict__ ws, VecMap vm,
+                                                        long long v0,
                                                         const float2* __restrict__ twid_g,
                                                         EpiArgs ea) {
   using P = Plan<G>;
@@ -348,6 +386,8 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
   const long long row0 = (long long)blockIdx.x * R;
   const int v = int(row0 / G2);
   const long long xy0 = row0 - (long long)v * G2;
+  const VecRef vr = map_vector(vm, v0 + v);
+  const FrameTables ft = vr.desc->ft;
   const float2* w = ws + (long long)v * 3 * G3;
   for (int t = threadIdx.x; t < 3 * R * N2; t += kThreads) {
     const int p = t / N2, n2 = t - p * N2;
@@ -360,7 +400,8 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
   }
   __syncthreads();
   lds_step3_natural<G, -1, 3 * R, kThreads>(buf);
-  const long long vbase = (long long)v * 2 * G3;
+  const long long vbase = vr.out_vec * 2 * G3;   // out, y, z blocks
+  const long long xbase = vr.in_vec * 2 * G3;    // the operator's input block
   double s0 = 0.0, s1 = 0.0;
   for (int e = threadIdx.x; e < R * G; e += kThreads) {
     {
@@ -388,9 +429,10 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
         o1 = make_float2(f1.y * inv, -f1.x * inv);
       }
       const long long i0 = vbase + vox, i1 = vbase + G3 + vox;
+      const long long j0 = xbase + vox, j1 = xbase + G3 + vox;
       if (EPI == kStore) {
         if (ea.beta != 0.f) {
-          const float2 x0 = ea.x[i0], x1 = ea.x[i1];
+          const float2 x0 = ea.x[j0], x1 = ea.x[j1];
           o0 = make_float2(fmaf(ea.alpha, o0.x, ea.beta * x0.x), fmaf(ea.alpha, o0.y, ea.beta * x0.y));
           o1 = make_float2(fmaf(ea.alpha, o1.x, ea.beta * x1.x), fmaf(ea.alpha, o1.y, ea.beta * x1.y));
         } else if (ea.alpha != 1.f) {
@@ -401,7 +443,7 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
         ea.out[i1] = o1;
       } else {
         // T_{k+1} = 2·(alpha·Θ T_k − T_k) − T_{k−1}   (interior.py:341-348, exp_kpm_dos.py:148)
-        const float2 x0 = ea.x[i0], x1 = ea.x[i1];
+        const float2 x0 = ea.x[j0], x1 = ea.x[j1];
         const float2 p0 = ea.out[i0], p1 = ea.out[i1];
         const float2 n0 = make_float2(2.f * fmaf(ea.alpha, o0.x, -x0.x) - p0.x, 2.f * fmaf(ea.alpha, o0.y, -x0.y) - p0.y);
         const float2 n1 = make_float2(2.f * fmaf(ea.alpha, o1.x, -x1.x) - p1.x, 2.f * fmaf(ea.alpha, o1.y, -x1.y) - p1.y);

@@ -210,3 +210,89 @@ def apply_chunked(fn, X, rows: int):
     if X.shape[0] <= rows:
         return fn(X)
     return torch.cat([fn(X[s:s + rows]) for s in range(0, X.shape[0], rows)], dim=0)
+
+
+# ---- multi-problem forms (one launch for up to 32 independent problems) ------
+MAX_PROBLEMS = 32
+
+
+def _ptrs(ts):
+    import ctypes as C
+    return (C.c_void_p * len(ts))(*[None if t is None else t.data_ptr() for t in ts])
+
+
+def gram_multi(As, Bs) -> torch.Tensor:
+    """(nprob, m_a, m_b) complex128: A_pᴴ·B_p for each pair (FP64-accumulated)."""
+    nprob = len(As)
+    if nprob == 0:
+        raise ValueError("gram_multi: no problems")
+    if nprob > MAX_PROBLEMS:
+        raise ValueError(f"gram_multi: at most {MAX_PROBLEMS} problems per launch")
+    ma, mb = int(As[0].shape[0]), int(Bs[0].shape[0])
+    n = _flat_n(As[0])
+    for A, B in zip(As, Bs):
+        if int(A.shape[0]) != ma or int(B.shape[0]) != mb or not A.is_contiguous() \
+                or not B.is_contiguous():
+            raise ValueError("gram_multi: blocks must be contiguous and uniform in shape")
+    out = torch.empty((nprob, ma, mb), dtype=torch.complex128, device=As[0].device)
+    lib = _lib.load()
+    need = lib.efn_gram_multi_work_bytes(nprob, ma, mb, n)
+    work = SCRATCH.get(need, "gram")
+    _lib.check(lib.efn_gram_multi(nprob, _ptrs(As), _ptrs(Bs), ma, mb, n, out.data_ptr(),
+                                  work.data_ptr(), work.numel(), stream_ptr()), "gram_multi")
+    return out
+
+
+def combine_multi(C: torch.Tensor, Ss, outs, Xs=None, alpha: float = 0.0) -> None:
+    """out_p = alpha·X_p + C_pᵀ·S_p for each problem (C: (nprob, m_in, m_out))."""
+    nprob = len(Ss)
+    if nprob > MAX_PROBLEMS:
+        raise ValueError(f"combine_multi: at most {MAX_PROBLEMS} problems per launch")
+    C = C.to(device=outs[0].device, dtype=torch.complex64).contiguous()
+    m_in, m_out = int(C.shape[1]), int(C.shape[2])
+    n = _flat_n(outs[0])
+    _lib.check(_lib.load().efn_combine_multi(nprob, C.data_ptr(), _ptrs(Ss), m_in, m_out, n,
+                                             None if Xs is None else _ptrs(Xs), float(alpha),
+                                             _ptrs(outs), stream_ptr()), "combine_multi")
+
+
+def copy_multi(srcs, dsts) -> None:
+    """dst_p ← src_p (src None: zero fill) for equally sized contiguous blocks."""
+    nprob = len(dsts)
+    count = int(dsts[0].numel())
+    _lib.check(_lib.load().efn_copy_multi(nprob, None if srcs is None else _ptrs(srcs),
+                                          _ptrs(dsts), count, stream_ptr()), "copy_multi")
+
+
+def rowreduce_strided(op: int, A, B, nrow: int, stride_a: int, stride_b: int, m: int, n: int):
+    """Row reductions over logical rows i → physical (i // nrow)·stride + i % nrow."""
+    out = torch.empty(m, dtype=torch.complex128, device=A.device)
+    lib = _lib.load()
+    need = lib.efn_rowreduce_work_bytes(m, n)
+    work = SCRATCH.get(need, "rowred")
+    _lib.check(lib.efn_rowreduce_strided(op, A.data_ptr(), stride_a,
+                                         None if B is None else B.data_ptr(), stride_b, nrow, m, n,
+                                         out.data_ptr(), work.data_ptr(), work.numel(),
+                                         stream_ptr()), "rowreduce_strided")
+    return out
+
+
+def row_axpby_strided(X, stride_x, a, Y, stride_y, b, nrow, m, n, out, stride_out):
+    """out_i = a_i·X_i + b_i·Y_i over strided logical rows (see rowreduce_strided)."""
+    a = torch.as_tensor(a, dtype=torch.float64, device=X.device).reshape(-1).contiguous()
+    if Y is not None:
+        b = torch.as_tensor(b, dtype=torch.float64, device=X.device).reshape(-1).contiguous()
+    _lib.check(_lib.load().efn_row_axpby_strided(X.data_ptr(), stride_x, a.data_ptr(),
+                                                 None if Y is None else Y.data_ptr(), stride_y,
+                                                 None if Y is None else b.data_ptr(), nrow, m, n,
+                                                 out.data_ptr(), stride_out, stream_ptr()),
+               "row_axpby_strided")
+    return out
+
+
+def eigh_batched_host(A: torch.Tensor):
+    """FP64 Hermitian eigendecomposition of a (nprob, k, k) stack on the host (LAPACK)."""
+    a = A.detach().cpu().numpy()
+    a = (a + np.conj(np.swapaxes(a, -1, -2))) / 2
+    w, V = np.linalg.eigh(a)
+    return w, V

@@ -258,3 +258,38 @@ class MaxwellOperator:
         E = self.inv_eps * torch.fft.ifftn(Ec, dim=(2, 3, 4))
         lam = torch.as_tensor(np.asarray(eigvals), dtype=torch.float32, device=self._dev)
         return E / torch.sqrt(lam)[:, None, None, None, None]
+
+
+def _multi(kind, ws_op, ops, in_base, in_off, out_base, out_off, nrow, alpha=1.0, beta=0.0):
+    import ctypes as C
+    n = len(ops)
+    if n > 32:
+        raise ValueError("at most 32 operators per multi launch")
+    for op in ops:
+        if op.grid_size != ws_op.grid_size:
+            raise ValueError("multi launch: all operators must share the grid size")
+    ctxs = (C.c_void_p * n)(*[op._ctx.value for op in ops])
+    io = (C.c_int64 * n)(*[int(x) for x in in_off])
+    oo = (C.c_int64 * n)(*[int(x) for x in out_off])
+    lib = _lib.load()
+    if kind == "theta":
+        rc = lib.efn_theta_multi(ws_op._ctx, n, ctxs, in_base.data_ptr(), io, out_base.data_ptr(),
+                                 oo, int(nrow), float(alpha), float(beta), stream_ptr())
+    else:
+        rc = lib.efn_precondition_multi(ws_op._ctx, n, ctxs, in_base.data_ptr(), io,
+                                        out_base.data_ptr(), oo, int(nrow), stream_ptr())
+    _lib.check(rc, f"{kind}_multi")
+
+
+def theta_multi(ws_op, ops, in_base, in_off, out_base, out_off, nrow, alpha=1.0, beta=0.0):
+    """One launch set applying Θ of several operators (same G) to row ranges of two buffers.
+
+    Slot i: rows in_off[i] .. in_off[i]+nrow−1 of `in_base` (a contiguous
+    (rows, 2, G, G, G) buffer) → rows out_off[i] .. of `out_base`; `ws_op`'s
+    workspace is used for the passes."""
+    _multi("theta", ws_op, ops, in_base, in_off, out_base, out_off, nrow, alpha, beta)
+
+
+def precondition_multi(ws_op, ops, in_base, in_off, out_base, out_off, nrow):
+    """Multi-operator transverse-projection preconditioner (see theta_multi)."""
+    _multi("precond", ws_op, ops, in_base, in_off, out_base, out_off, nrow)

@@ -68,6 +68,20 @@ int efn_ctx_basis(const efn_ctx* ctx, float* kn_dev, float* t_dev, void* stream)
 int efn_theta(efn_ctx* ctx, const void* in_c64, void* out_c64, int m, float alpha, float beta,
               void* stream);
 
+/* Multi-operator Θ: one launch set over `nslot` (≤ 32) operators of the same
+ * grid size — Bloch k-points, structures or independent requests.  Slot i
+ * applies ctxs[i] to rows in_off[i] .. in_off[i]+nrow−1 of in_base and writes
+ * rows out_off[i] .. of out_base (offsets in vectors of 2·G³ complex64).  The
+ * passes run in ws's workspace.  Replaces a loop of operator.py:153-155 calls
+ * over operators; no reference counterpart as a single call. */
+int efn_theta_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
+                    const int64_t* in_off, void* out_base, const int64_t* out_off, int nrow,
+                    float alpha, float beta, void* stream);
+/* Multi-operator preconditioner (operator.py:157-161 over several operators). */
+int efn_precondition_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
+                           const int64_t* in_off, void* out_base, const int64_t* out_off, int nrow,
+                           void* stream);
+
 /* MPB transverse-projection preconditioner, operator.py:104-127 / :157-161. */
 int efn_precondition(efn_ctx* ctx, const void* in_c64, void* out_c64, int m, void* stream);
 
@@ -102,6 +116,13 @@ size_t efn_gram_work_bytes(int ma, int mb, int64_t n);
 int efn_gram(const void* A_c64, int ma, const void* B_c64, int mb, int64_t n, void* out_c128,
              void* work, size_t work_bytes, void* stream);
 
+/* Gram blocks of `nprob` (≤ 32) independent problems in one launch:
+ * out[p] = A_pᴴ·B_p (ma×mb complex128 each, contiguous), A_p / B_p device
+ * pointers given in host arrays.  solver.py:35-43 per problem. */
+size_t efn_gram_multi_work_bytes(int nprob, int ma, int mb, int64_t n);
+int efn_gram_multi(int nprob, const void* const* A_c64, const void* const* B_c64, int ma, int mb,
+                   int64_t n, void* out_c128, void* work, size_t work_bytes, void* stream);
+
 /* Linear combination out_i = alpha·X_i + Σ_j C[j][i]·S_j (solver.py:49-57,
  * :106, :125): C is m_in×m_out complex64 on the device, S m_in rows, X
  * (optional, NULL = none) and out m_out rows of n complex64.  out must not
@@ -109,12 +130,30 @@ int efn_gram(const void* A_c64, int ma, const void* B_c64, int mb, int64_t n, vo
 int efn_combine(const void* C_c64, int m_in, int m_out, const void* S_c64, int64_t n,
                 const void* X_c64, float alpha, void* out_c64, void* stream);
 
+/* combine for `nprob` (≤ 32) problems in one launch: C is nprob×m_in×m_out
+ * complex64 on the device; S, X (optional, NULL = none) and out are host
+ * arrays of per-problem device pointers.  solver.py:49-57 per problem. */
+int efn_combine_multi(int nprob, const void* C_c64, const void* const* S_c64, int m_in, int m_out,
+                      int64_t n, const void* const* X_c64, float alpha, void* const* out_c64,
+                      void* stream);
+
 /* Row reductions in FP64 (solver.py:228-231, interior.py:75-82):
  *   op 0: out[i] = (Σ_k |A[i][k]|², 0)      op 1: out[i] = Σ_k conj(A[i][k])·B[i][k]
  * out is m complex128 on the device. */
 size_t efn_rowreduce_work_bytes(int m, int64_t n);
 int efn_rowreduce(int op, const void* A_c64, const void* B_c64, int m, int64_t n, void* out_c128,
                   void* work, size_t work_bytes, void* stream);
+
+/* Strided-row forms for blocks that hold several problems' rows: logical row
+ * i (0 ≤ i < m) is physical row (i / nrow)·stride + i % nrow of its array
+ * (e.g. the X rows of every slot in slot-major [X; W; P] storage:
+ * nrow = block width, stride = 3·block width). */
+int efn_rowreduce_strided(int op, const void* A_c64, int64_t stride_a, const void* B_c64,
+                          int64_t stride_b, int nrow, int m, int64_t n, void* out_c128, void* work,
+                          size_t work_bytes, void* stream);
+int efn_row_axpby_strided(const void* X_c64, int64_t stride_x, const double* a_dev,
+                          const void* Y_c64, int64_t stride_y, const double* b_dev, int nrow, int m,
+                          int64_t n, void* out_c64, int64_t stride_out, void* stream);
 
 /* out_i = a[i]·X_i + b[i]·Y_i with per-row FP64 coefficients on the device
  * (Y, b may be NULL).  R = HX − λX (solver.py:229), row scaling (:306-307). */
@@ -124,6 +163,13 @@ int efn_row_axpby(const void* X_c64, const double* a_dev, const void* Y_c64, con
 /* out = a·X + b·Y elementwise over count complex entries (Y may be NULL). */
 int efn_axpby(const void* X_c64, float a, const void* Y_c64, float b, int64_t count, void* out_c64,
               void* stream);
+
+/* Batched copy of `count` complex64 entries for `nprob` (≤ 32) problems:
+ * dst[p][e] = src[p][e] (src NULL: zero fill).  Moves rows between the
+ * fixed per-slot buffers of the batched solver (solver.py:65-71's locked-block
+ * write, per slot). */
+int efn_copy_multi(int nprob, const void* const* src_c64, void* const* dst_c64, int64_t count,
+                   void* stream);
 
 /* Random block of m rows (rows row0 .. row0+m−1 of the stream `seed`), Γ-masked.
  * kind 0: complex normal (Box–Muller), kind 1: Rademacher ±1.  Replaces the JAX
