@@ -116,6 +116,12 @@ class BatchedLOBPCG:
         self.L = torch.zeros((nslots * self.cap,) + shape, dtype=c64, device=dev)
         self.slots: list[_Slot | None] = [None] * nslots
         self.steps = 0
+        self.timing = {}  # host seconds per phase (includes waits at the phase's syncs)
+
+    def _tick(self, name, t0):
+        t1 = time.perf_counter()
+        self.timing[name] = self.timing.get(name, 0.0) + (t1 - t0)
+        return t1
 
     # ---- bookkeeping --------------------------------------------------------
     def free_slots(self) -> int:
@@ -224,6 +230,7 @@ class BatchedLOBPCG:
 
     def _iterate(self, idx, lam):
         m = self.m
+        t = time.perf_counter()
         # W = K⁻¹R, deflated, projected out of X twice, SVQB (solver.py:251-257)
         for c in range(0, len(idx), MAX_PROBLEMS):
             part = idx[c:c + MAX_PROBLEMS]
@@ -234,7 +241,9 @@ class BatchedLOBPCG:
             Xs = [self._X(s) for s in idx]
             Ws = [self._W(s) for s in idx]
             combine_multi(-gram_multi(Xs, Ws), Xs, Ws, Xs=Ws, alpha=1.0)
+        t = self._tick("precond+deflate+proj", t)
         self._svqb(idx, self._W)
+        t = self._tick("svqb_W", t)
         # P projected out of [X, W] twice, fixed-shape whitening (solver.py:258-274)
         for _ in range(2):
             XWs = [self._XW(s) for s in idx]
@@ -248,20 +257,22 @@ class BatchedLOBPCG:
         Ts = [self._rows(self.T1, s, m) for s in idx]
         combine_multi(torch.as_tensor(Vp * scale[:, None, :], device=self.S.device), Ps, Ts)
         copy_multi(Ts, Ps)
+        t = self._tick("P", t)
         # Θ(W), Θ(P) fresh; Rayleigh–Ritz on [X; W; P] (solver.py:275-302)
         self._theta(idx, self.S, m, self.HS, m, 3 * m, 3 * m, 2 * m)
+        t = self._tick("theta_WP", t)
         Sbs = [self._Sb(s) for s in idx]
         A = gram_multi(Sbs, [self._HSb(s) for s in idx]).cpu().numpy()
         rn2 = rowreduce_strided(0, self.S, None, 3 * m, 3 * m, 3 * m, self.nslots * 3 * m,
                                 self.n).real.cpu().numpy().reshape(self.nslots, 3 * m)
-        C = np.empty((len(idx), 3 * m, m), np.complex128)
         for j, s in enumerate(idx):
-            a = (A[j] + A[j].conj().T) / 2
+            A[j] = (A[j] + A[j].conj().T) / 2
             dead = np.sqrt(rn2[s]) < 0.5
-            penalty = max(RR_DEAD_PENALTY, 100.0 * float(np.abs(np.diag(a)).max()))
-            a[np.diag_indices_from(a)] += np.where(dead, penalty, 0.0)
-            _, VA = np.linalg.eigh(a)
-            C[j] = VA[:, :m]
+            penalty = max(RR_DEAD_PENALTY, 100.0 * float(np.abs(np.diag(A[j])).max()))
+            A[j][np.diag_indices(3 * m)] += np.where(dead, penalty, 0.0)
+        from .linalg import eigh_stack_host
+        _, C = eigh_stack_host(A, lowest=m)
+        t = self._tick("RR_gram+eigh", t)
         Ct = torch.as_tensor(C, device=self.S.device)
         Cp = Ct.clone()
         Cp[:, :m] = 0
@@ -288,7 +299,9 @@ class BatchedLOBPCG:
                           self.S, 3 * m)
         P0 = self.S[2 * m:]
         row_axpby_strided(self.T2, m, a2, None, m, None, m, self.nslots * m, self.n, P0, 3 * m)
+        t = self._tick("update_XP", t)
         self._theta(idx, self.S, 0, self.HS, 0, 3 * m, 3 * m, m)
+        t = self._tick("theta_X", t)
         for s in idx:
             self.slots[s].it += 1
             self.slots[s].req.iterations += 1
@@ -331,9 +344,11 @@ class BatchedLOBPCG:
             return finished
         self.steps += 1
         m = self.m
+        t = time.perf_counter()
         init = [s for s in act if self.slots[s].phase == "init"]
         if init:
             self._init_blocks(init)
+            t = self._tick("init", t)
         # Rayleigh quotients, residuals, dead rows for the X rows of every slot
         M = self.nslots * m
         lam = rowreduce_strided(1, self.S, self.HS, m, 3 * m, 3 * m, M, self.n).real
@@ -345,6 +360,7 @@ class BatchedLOBPCG:
         rr2 = rowreduce_strided(0, self.R, None, m, m, m, M, self.n).real
         hx2 = rowreduce_strided(0, self.HS, None, m, 3 * m, 3 * m, M, self.n).real
         stats = torch.stack([lam, xn2, rr2, hx2]).cpu().numpy()
+        t = self._tick("residuals+sync", t)
         lam_h = stats[0].reshape(self.nslots, m)
         xn = np.sqrt(stats[1]).reshape(self.nslots, m)
         rn_h = (np.sqrt(stats[2]) / np.maximum(np.sqrt(stats[3]), 1e-30)).reshape(self.nslots, m)
@@ -361,6 +377,7 @@ class BatchedLOBPCG:
                     finished.append(req)
             else:
                 cont.append(s)
+        t = self._tick("lock", t)
         if cont:
             self._iterate(cont, lam)
         return finished

@@ -95,6 +95,200 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(PairPtrs pp, int ma, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Gram on the matrix cores.  Complex AᴴB as two real products over k' = 2k
+// interleaved reals:
+//   Re = Σ_k' A'[i][k']·B'[j][k'],  Im = Σ_k' A'[i][k']·B''[j][k'],
+//   A' = (a_r, a_i), B' = (b_r, b_i), B'' = (b_i, −b_r) per complex entry.
+// A wave owns a 32×32 output tile (2×2 MFMA tiles of 16×16) over a quarter of
+// its block's k range.  Per step lane l reads one float4 (2 complex) of rows
+// l%16 and l%16+16 at k' offset 4·(l/16); component q feeds MFMA q, whose
+// operand k index is l/16 for A and B alike, so each step covers 16 k' once.
+// MFMA 16x16x4 layouts: A[i][k] in lane i+16k, B[k][j] in lane j+16k; D[i][j]
+// in lane j+16·(i/4), element i%4 for f32, and in lane j+16·(i%4), element
+// i/4 for f64 (P::row maps lane group and element to the output row).
+//   GramF64: v_mfma_f64_16x16x4f64 on the exact FP64 images of the FP32
+//            inputs — products exact, sums in FP64 (same precision class as
+//            the VALU kernel).
+//   GramF32: v_mfma_f32_16x16x4f32, FP32 accumulators added into FP64 every
+//            kMFlush steps (≤ 4·kMFlush·4 products per FP32 partial sum).
+// The four waves' tiles are summed through LDS in the fixed order
+// (w0 + w2) + (w1 + w3) and each block writes one split partial, as the
+// VALU kernel does, so the work-buffer layout is shared.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+constexpr int kMWaveTile = 32;
+constexpr int kMFlush = 4;
+constexpr int kMStep = 8;  // complex entries per wave step
+
+struct GramF64 {
+  typedef double in_t;
+  typedef doublex4 acc_t;
+  static __device__ __forceinline__ acc_t mfma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int q) { return g + 4 * q; }
+  static constexpr bool kFlush = false;
+};
+struct GramF32 {
+  typedef float in_t;
+  typedef floatx4 acc_t;
+  static __device__ __forceinline__ acc_t mfma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int q) { return 4 * g + q; }
+  static constexpr bool kFlush = true;
+};
+
+__device__ __forceinline__ float4 load_c2(const float2* __restrict__ row, long long k, long long ke) {
+  // complex entries k, k+1 of a row as a float4, zero at and beyond ke
+  if (k + 1 < ke) return *reinterpret_cast<const float4*>(row + k);
+  if (k < ke) {
+    const float2 v = row[k];
+    return make_float4(v.x, v.y, 0.f, 0.f);
+  }
+  return make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// grid: (splits, tiles_i·tiles_j, problems) with 32×32 tiles; kper a multiple
+// of 4·kMStep; rows 16-byte aligned (n even, aligned bases).
+template <class P>
+__global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int mb, int tiles_j,
+                                                        long long n, long long kper,
+                                                        double2* __restrict__ part) {
+  typedef typename P::in_t T;
+  typedef typename P::acc_t Acc;
+  __shared__ double2 red[2][kMWaveTile * kMWaveTile];
+  const float2* __restrict__ A = pp.a[blockIdx.z];
+  const float2* __restrict__ B = pp.b[blockIdx.z];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int i0 = (blockIdx.y / tiles_j) * kMWaveTile, j0 = (blockIdx.y % tiles_j) * kMWaveTile;
+  const long long kb_blk = (long long)blockIdx.x * kper;
+  const long long ke_blk = kb_blk + kper < n ? kb_blk + kper : n;
+  const long long kw = kper / 4;
+  const long long kb = kb_blk + wave * kw;
+  const long long ke = kb + kw < ke_blk ? kb + kw : ke_blk;
+  const bool ra[2] = {i0 + r < ma, i0 + 16 + r < ma};
+  const bool rb[2] = {j0 + r < mb, j0 + 16 + r < mb};
+  const float2* ap[2] = {A + (long long)(ra[0] ? i0 + r : 0) * n, A + (long long)(ra[1] ? i0 + 16 + r : 0) * n};
+  const float2* bp[2] = {B + (long long)(rb[0] ? j0 + r : 0) * n, B + (long long)(rb[1] ? j0 + 16 + r : 0) * n};
+  Acc cre[2][2], cim[2][2];
+  double dre[2][2][4], dim_[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        cre[a][b][q] = 0;
+        cim[a][b][q] = 0;
+        dre[a][b][q] = dim_[a][b][q] = 0.0;
+      }
+    }
+  int steps = 0;
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  // register double buffer: the next step's loads are in flight during this step's MFMAs
+  float4 an[2], bn[2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) an[a] = ra[a] ? load_c2(ap[a], kb + 2 * g, ke) : z4;
+#pragma unroll
+  for (int b = 0; b < 2; ++b) bn[b] = rb[b] ? load_c2(bp[b], kb + 2 * g, ke) : z4;
+  for (long long k = kb; k < ke; k += kMStep) {
+    float4 av[2] = {an[0], an[1]}, bv[2] = {bn[0], bn[1]};
+    const long long kn = k + kMStep + 2 * g;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) an[a] = ra[a] ? load_c2(ap[a], kn, ke) : z4;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) bn[b] = rb[b] ? load_c2(bp[b], kn, ke) : z4;
+    T x[2][4], y[2][4], w[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      x[a][0] = (T)av[a].x; x[a][1] = (T)av[a].y; x[a][2] = (T)av[a].z; x[a][3] = (T)av[a].w;
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      y[b][0] = (T)bv[b].x; y[b][1] = (T)bv[b].y; y[b][2] = (T)bv[b].z; y[b][3] = (T)bv[b].w;
+      w[b][0] = y[b][1]; w[b][1] = -y[b][0]; w[b][2] = y[b][3]; w[b][3] = -y[b][2];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          cre[a][b] = P::mfma(x[a][q], y[b][q], cre[a][b]);
+          cim[a][b] = P::mfma(x[a][q], w[b][q], cim[a][b]);
+        }
+    if (P::kFlush && ++steps == kMFlush) {
+      steps = 0;
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            dre[a][b][q] += (double)cre[a][b][q];
+            dim_[a][b][q] += (double)cim[a][b][q];
+            cre[a][b][q] = 0;
+            cim[a][b][q] = 0;
+          }
+    }
+  }
+  // this wave's tile: element (P::row(g, q), r) of sub-tile (a, b)
+  double2 v[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        v[a][b][q] = make_double2(dre[a][b][q] + (double)cre[a][b][q],
+                                  dim_[a][b][q] + (double)cim[a][b][q]);
+  // fixed-order reduction over the waves: (w0 + w2) + (w1 + w3)
+  auto lds_at = [&](int a, int b, int q) { return (16 * a + P::row(g, q)) * kMWaveTile + 16 * b + r; };
+  if (wave >= 2)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[wave - 2][lds_at(a, b, q)] = v[a][b][q];
+  __syncthreads();
+  if (wave < 2)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double2 o = red[wave][lds_at(a, b, q)];
+          v[a][b][q].x += o.x;
+          v[a][b][q].y += o.y;
+        }
+  __syncthreads();
+  if (wave == 1)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[0][lds_at(a, b, q)] = v[a][b][q];
+  __syncthreads();
+  if (wave != 0) return;
+  double2* out = part + ((long long)blockIdx.z * gridDim.x + blockIdx.x) * ma * mb;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + 16 * a + P::row(g, q), j = j0 + 16 * b + r;
+        const double2 o = red[0][lds_at(a, b, q)];
+        if (i < ma && j < mb) out[(long long)i * mb + j] = make_double2(v[a][b][q].x + o.x, v[a][b][q].y + o.y);
+      }
+}
+
 // out[prob][e] = Σ_s part[prob][s][e] in fixed order (grid.y = problem)
 __global__ void sum_partials_kernel(const double2* __restrict__ part, int nsplit, long long count,
                                     double2* __restrict__ out) {

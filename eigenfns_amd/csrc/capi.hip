@@ -2,6 +2,7 @@
 // Declarations and reference citations: include/eigenfns_hip.h.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -227,6 +228,8 @@ int gram_nsplit(int, int, long long n) {
   return (int)(ns < 1 ? 1 : ns);
 }
 
+std::atomic<int> g_gram_engine{EFN_GRAM_MFMA_F64};
+
 int rowreduce_nsplit(long long n) {
   long long ns = (n + 65535) / 65536;
   return (int)(ns < 1 ? 1 : (ns > 256 ? 256 : ns));
@@ -314,7 +317,9 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   // (one vector at 192³ = 170 MB); measured +16% over 1 GB chunks at 192³.
   const size_t per_vec = (size_t)3 * G3 * sizeof(float2);
   int chunk = max_chunk;
-  if (chunk <= 0) {
+  if (chunk < 0) {
+    chunk = 0;  // table-only context: usable in multi launches that bring a workspace
+  } else if (chunk == 0) {
     chunk = (int)((size_t(192) << 20) / per_vec);
     if (chunk < 1) chunk = 1;
     if (chunk > 1024) chunk = 1024;
@@ -336,9 +341,11 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   EFN_ALLOC(c->d_eps, G3 * sizeof(float));
   EFN_ALLOC(c->d_kn2, G3 * sizeof(float));
   EFN_ALLOC(c->d_twid, G * sizeof(float2));
-  EFN_ALLOC(c->d_ws, c->ws_bytes);
-  EFN_ALLOC(c->d_partial, (size_t)chunk * nb * 2 * sizeof(double));
-  EFN_ALLOC(c->d_red, (size_t)chunk * sizeof(double2));
+  if (chunk > 0) {
+    EFN_ALLOC(c->d_ws, c->ws_bytes);
+    EFN_ALLOC(c->d_partial, (size_t)chunk * nb * 2 * sizeof(double));
+    EFN_ALLOC(c->d_red, (size_t)chunk * sizeof(double2));
+  }
   EFN_ALLOC(c->d_desc, sizeof(efn::OpDesc));
 #undef EFN_ALLOC
   hipError_t e = hipSuccess;
@@ -441,9 +448,16 @@ static int check_block(const efn_ctx* c, const void* p, int m, const char* what)
   return EFN_OK;
 }
 
+// operator passes need the context's own workspace
+static int check_ws(const efn_ctx* c) {
+  if (c->chunk <= 0) return fail(EFN_EINVAL, "context was created without a workspace (max_chunk < 0)");
+  return EFN_OK;
+}
+
 int efn_theta(efn_ctx* c, const void* in, void* out, int m, float alpha, float beta, void* stream) {
   int rc;
-  if ((rc = check_block(c, in, m, "in")) || (rc = check_block(c, out, m, "out"))) return rc;
+  if ((rc = check_block(c, in, m, "in")) || (rc = check_block(c, out, m, "out")) || (rc = check_ws(c)))
+    return rc;
   if (m == 0) return EFN_OK;
   if (in == out) return fail(EFN_EINVAL, "in and out must not alias");
   OpRequest rq{};
@@ -460,7 +474,8 @@ int efn_theta(efn_ctx* c, const void* in, void* out, int m, float alpha, float b
 
 int efn_precondition(efn_ctx* c, const void* in, void* out, int m, void* stream) {
   int rc;
-  if ((rc = check_block(c, in, m, "in")) || (rc = check_block(c, out, m, "out"))) return rc;
+  if ((rc = check_block(c, in, m, "in")) || (rc = check_block(c, out, m, "out")) || (rc = check_ws(c)))
+    return rc;
   if (m == 0) return EFN_OK;
   if (in == out) return fail(EFN_EINVAL, "in and out must not alias");
   OpRequest rq{};
@@ -491,7 +506,7 @@ int efn_cheb_step(efn_ctx* c, const void* T1, void* T0, void* Y, float alpha, fl
                   void* stream) {
   int rc;
   if ((rc = check_block(c, T1, m, "T1")) || (rc = check_block(c, T0, m, "T0")) ||
-      (rc = check_block(c, Y, m, "Y")))
+      (rc = check_block(c, Y, m, "Y")) || (rc = check_ws(c)))
     return rc;
   if (m == 0) return EFN_OK;
   if (T1 == T0 || T1 == Y || T0 == Y) return fail(EFN_EINVAL, "T1, T0 and Y must be distinct");
@@ -511,7 +526,8 @@ int efn_cheb_step(efn_ctx* c, const void* T1, void* T0, void* Y, float alpha, fl
 int efn_kpm_step(efn_ctx* c, const void* T1, void* T0, double* mom, int ld, int k, float alpha,
                  int m, void* stream) {
   int rc;
-  if ((rc = check_block(c, T1, m, "T1")) || (rc = check_block(c, T0, m, "T0"))) return rc;
+  if ((rc = check_block(c, T1, m, "T1")) || (rc = check_block(c, T0, m, "T0")) || (rc = check_ws(c)))
+    return rc;
   if (m == 0) return EFN_OK;
   if (!mom || k < 1 || 2 * k + 1 >= ld) return fail(EFN_EINVAL, "bad moment buffer / index");
   if (T1 == T0) return fail(EFN_EINVAL, "T1 and T0 must be distinct");
@@ -534,7 +550,7 @@ int efn_kpm_count_step(efn_ctx* c, const void* T1, void* T0, const void* Z, doub
                        float alpha, int m, void* stream) {
   int rc;
   if ((rc = check_block(c, T1, m, "T1")) || (rc = check_block(c, T0, m, "T0")) ||
-      (rc = check_block(c, Z, m, "Z")))
+      (rc = check_block(c, Z, m, "Z")) || (rc = check_ws(c)))
     return rc;
   if (m == 0) return EFN_OK;
   if (!est) return fail(EFN_EINVAL, "est is NULL");
@@ -558,6 +574,7 @@ static int build_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void*
                        const int64_t* in_off, void* out_base, const int64_t* out_off, int nrow,
                        OpRequest& rq) {
   if (!ws) return fail(EFN_EINVAL, "workspace ctx is NULL");
+  if (ws->chunk <= 0) return fail(EFN_EINVAL, "workspace ctx was created without a workspace");
   if (nslot < 0 || nslot > efn::kMaxSlots)
     return fail(EFN_EINVAL, "nslot must be in [0, " + std::to_string(efn::kMaxSlots) + "]");
   if (nrow < 0) return fail(EFN_EINVAL, "negative rows per slot");
@@ -617,6 +634,12 @@ int efn_precondition_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const v
   return ws->run(ws, rq, (hipStream_t)stream);
 }
 
+int efn_gram_engine(int engine) {
+  if (engine < 0) return g_gram_engine.load();
+  if (engine > EFN_GRAM_MFMA_F32) return fail(EFN_EINVAL, "unknown Gram engine");
+  return g_gram_engine.exchange(engine);
+}
+
 size_t efn_gram_work_bytes(int ma, int mb, int64_t n) {
   if (ma <= 0 || mb <= 0 || n <= 0) return 0;
   return (size_t)gram_nsplit(ma, mb, n) * ma * mb * sizeof(double2);
@@ -650,9 +673,24 @@ int efn_gram_multi(int nprob, const void* const* A, const void* const* B, int ma
   const int ns = gram_nsplit(ma, mb, n);
   long long kper = (n + ns - 1) / ns;
   kper = (kper + efn::kGK - 1) / efn::kGK * efn::kGK;
-  const int ti = (ma + efn::kGT - 1) / efn::kGT, tj = (mb + efn::kGT - 1) / efn::kGT;
-  hipLaunchKernelGGL(efn::gram_partial_kernel, dim3((unsigned)ns, (unsigned)(ti * tj), (unsigned)nprob),
-                     dim3(256), 0, s, pp, ma, mb, tj, (long long)n, kper, (double2*)work);
+  bool aligned = (n % 2) == 0;
+  for (int i = 0; i < nprob; ++i)
+    aligned = aligned && ((uintptr_t)A[i] % 16 == 0) && ((uintptr_t)B[i] % 16 == 0);
+  const int engine = aligned ? g_gram_engine.load() : EFN_GRAM_VALU_F64;
+  if (engine == EFN_GRAM_VALU_F64) {
+    const int ti = (ma + efn::kGT - 1) / efn::kGT, tj = (mb + efn::kGT - 1) / efn::kGT;
+    hipLaunchKernelGGL(efn::gram_partial_kernel, dim3((unsigned)ns, (unsigned)(ti * tj), (unsigned)nprob),
+                       dim3(256), 0, s, pp, ma, mb, tj, (long long)n, kper, (double2*)work);
+  } else {
+    const int ti = (ma + efn::kMWaveTile - 1) / efn::kMWaveTile, tj = (mb + efn::kMWaveTile - 1) / efn::kMWaveTile;
+    const dim3 grid((unsigned)ns, (unsigned)(ti * tj), (unsigned)nprob);
+    if (engine == EFN_GRAM_MFMA_F64)
+      hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF64>, grid, dim3(256), 0, s, pp, ma, mb, tj,
+                         (long long)n, kper, (double2*)work);
+    else
+      hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF32>, grid, dim3(256), 0, s, pp, ma, mb, tj,
+                         (long long)n, kper, (double2*)work);
+  }
   EFN_LAUNCH_CHECK();
   const long long count = (long long)ma * mb;
   hipLaunchKernelGGL(efn::sum_partials_kernel, dim3((unsigned)((count + 255) / 256), (unsigned)nprob),

@@ -33,6 +33,23 @@ def _small_dense_on_host() -> bool:
     return os.environ.get("EFN_SMALL_DENSE", "host").lower() != "gpu"
 
 
+GRAM_ENGINES = {"valu_f64": 0, "mfma_f64": 1, "mfma_f32": 2}
+
+
+def gram_engine(name: str | None = None) -> str:
+    """Select the process-wide Gram engine by name; returns the previous one.
+
+    "mfma_f64" (default): matrix cores on the exact FP64 images of the FP32
+    inputs; "valu_f64": vector ALU, same precision; "mfma_f32": FP32 matrix
+    cores with FP32 partial sums of ≤64 products added into FP64."""
+    inv = {v: k for k, v in GRAM_ENGINES.items()}
+    code = -1 if name is None else GRAM_ENGINES[name]
+    rc = _lib.load().efn_gram_engine(code)
+    if rc < 0:
+        _lib.check(rc, "gram_engine")
+    return inv[rc]
+
+
 def gram(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
     """⟨A_i, B_j⟩ as an (m_a, m_b) complex128 device tensor (solver.py:35-43).
 
@@ -290,9 +307,37 @@ def row_axpby_strided(X, stride_x, a, Y, stride_y, b, nrow, m, n, out, stride_ou
     return out
 
 
-def eigh_batched_host(A: torch.Tensor):
-    """FP64 Hermitian eigendecomposition of a (nprob, k, k) stack on the host (LAPACK)."""
-    a = A.detach().cpu().numpy()
+_POOL = None
+
+
+def _pool():
+    global _POOL
+    if _POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _POOL = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+    return _POOL
+
+
+def eigh_stack_host(a: np.ndarray, lowest: int | None = None):
+    """FP64 Hermitian eigendecompositions of a (nprob, k, k) numpy stack, one LAPACK
+    call per matrix spread over a thread pool (LAPACK releases the GIL).  With
+    `lowest`, only the lowest eigenpairs are computed (scipy subset driver)."""
+    import scipy.linalg as sla
     a = (a + np.conj(np.swapaxes(a, -1, -2))) / 2
-    w, V = np.linalg.eigh(a)
-    return w, V
+
+    def one(x):
+        if lowest is not None and lowest < x.shape[0]:
+            return sla.eigh(x, subset_by_index=[0, lowest - 1], driver="evr",
+                            check_finite=False)
+        return np.linalg.eigh(x)
+
+    if a.shape[0] == 1:
+        res = [one(a[0])]
+    else:
+        res = list(_pool().map(one, list(a)))
+    return np.stack([r[0] for r in res]), np.stack([r[1] for r in res])
+
+
+def eigh_batched_host(A: torch.Tensor, lowest: int | None = None):
+    """FP64 Hermitian eigendecomposition of a (nprob, k, k) stack on the host (LAPACK)."""
+    return eigh_stack_host(A.detach().cpu().numpy(), lowest)

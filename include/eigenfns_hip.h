@@ -48,7 +48,8 @@ int efn_grid_supported(int grid_size);
  * Replaces MaxwellOperator.__init__ + make_basis (eigenfns/operator.py:136-151,
  * :44-79): builds the FP32 |k+G| axis tables, the FP64 reference-axis switch of
  * operator.py:57 as a bit mask, ε⁻¹/G³ and ε/G³, and a workspace for
- * `max_chunk` vectors (3·G³ complex64 each; 0 = automatic).
+ * `max_chunk` vectors (3·G³ complex64 each; 0 = automatic, ~192 MB; < 0 = no
+ * workspace: a table-only context for the slot lists of the *_multi calls).
  * eps_host: G³ floats on the host, C order. k in units of 2π/L. */
 int efn_ctx_create(int grid_size, double box_size, double kx, double ky, double kz,
                    const float* eps_host, int max_chunk, efn_ctx** out);
@@ -113,6 +114,23 @@ int efn_kpm_count_step(efn_ctx* ctx, const void* T1_c64, void* T0_c64, const voi
  * device buffer of at least efn_gram_work_bytes(ma, mb, n) bytes (split
  * partials, summed in a fixed order, so results are bitwise reproducible). */
 size_t efn_gram_work_bytes(int ma, int mb, int64_t n);
+
+/* Process-wide Gram engine for efn_gram / efn_gram_multi; returns the previous
+ * engine (engine < 0: query only).  All engines use the same split partials
+ * and the same fixed-order FP64 sum over them.
+ *   EFN_GRAM_VALU_F64  vector ALU: exact FP64 products, FP64 sums.
+ *   EFN_GRAM_MFMA_F64  matrix cores (v_mfma_f64_16x16x4f64) on the exact FP64
+ *                      images of the inputs: exact products, FP64 sums (default).
+ *   EFN_GRAM_MFMA_F32  matrix cores in FP32 (v_mfma_f32_16x16x4f32); FP32 partial
+ *                      sums of at most 64 products, compensated by adding them
+ *                      into FP64 accumulators.
+ * MFMA engines need n even and 16-byte aligned rows; other calls use the
+ * vector-ALU kernel.  No reference counterpart (XLA picks the reference's
+ * einsum kernels). */
+#define EFN_GRAM_VALU_F64 0
+#define EFN_GRAM_MFMA_F64 1
+#define EFN_GRAM_MFMA_F32 2
+int efn_gram_engine(int engine);
 int efn_gram(const void* A_c64, int ma, const void* B_c64, int mb, int64_t n, void* out_c128,
              void* work, size_t work_bytes, void* stream);
 

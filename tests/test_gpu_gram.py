@@ -1,0 +1,98 @@
+"""GPU tests of the three Gram engines (vector-ALU FP64, FP64 MFMA, FP32 MFMA
+with FP64 flushes) against the complex128 product of the same FP32 inputs.
+
+Bars: the FP64 engines within 1e-12 of max|AᴴB| (products exact, FP64 sums);
+the FP32 engine within 2e-6 of max_ij Σ_k |A_ik||B_jk| (FP32 partial sums of at
+most 64 products); every engine bitwise reproducible and bitwise identical
+alone and in a multi-problem launch.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ENGINES = ["valu_f64", "mfma_f64", "mfma_f32"]
+SHAPES = [(5, 7, 1000), (32, 32, 70000), (96, 96, 8192), (33, 65, 2 * 16 ** 3), (3, 130, 4098),
+          (96, 96, 2 * 48 ** 3)]
+
+
+def _cblock(rng, m, n):
+    return (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(np.complex64)
+
+
+@pytest.fixture
+def engine_guard():
+    from eigenfns_amd.linalg import gram_engine
+    prev = gram_engine()
+    yield gram_engine
+    gram_engine(prev)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_gram_engine_matches_complex128(engine, engine_guard):
+    import torch
+    from eigenfns_amd.linalg import gram
+    engine_guard(engine)
+    rng = np.random.default_rng(7)
+    for ma, mb, n in SHAPES:
+        A, B = _cblock(rng, ma, n), _cblock(rng, mb, n)
+        g = gram(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()).cpu().numpy()
+        A2, B2 = A.astype(np.complex128), B.astype(np.complex128)
+        gr = A2.conj() @ B2.T
+        err = np.abs(g - gr).max()
+        if engine == "mfma_f32":
+            scale = (np.abs(A2) @ np.abs(B2).T).max()
+            assert err < 2e-6 * scale, (ma, mb, n, err / scale)
+        else:
+            assert err < 1e-12 * np.abs(gr).max(), (ma, mb, n, err)
+        print(f"{engine} {ma}x{mb} n={n}: max err {err:.3e}")
+
+
+def test_gram_engines_agree_on_orthonormal_rows(engine_guard):
+    """Near-orthonormal rows (the solver's case): every engine's identity defect
+    is the FP32 rounding of the rows themselves, not the engine's."""
+    import torch
+    from eigenfns_amd.linalg import gram
+    rng = np.random.default_rng(3)
+    n = 2 * 32 ** 3
+    Q, _ = np.linalg.qr(_cblock(rng, n, 48).astype(np.complex128))
+    X = torch.from_numpy(Q.T.astype(np.complex64)).cuda()
+    Xd = X.cpu().numpy().astype(np.complex128)
+    exact = Xd.conj() @ Xd.T
+    for engine in ENGINES:
+        engine_guard(engine)
+        err = np.abs(gram(X, X).cpu().numpy() - exact).max()
+        print(f"{engine}: identity defect error {err:.2e}")
+        # FP64 engines: summation rounding over n = 65536 terms (numpy's too)
+        assert err < (1e-6 if engine == "mfma_f32" else 1e-13), (engine, err)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_gram_multi_bitwise_equals_single(engine, engine_guard):
+    import torch
+    from eigenfns_amd.linalg import gram, gram_multi
+    engine_guard(engine)
+    rng = np.random.default_rng(11)
+    n = 2 * 24 ** 3
+    As = [torch.from_numpy(_cblock(rng, 40, n)).cuda() for _ in range(3)]
+    Bs = [torch.from_numpy(_cblock(rng, 40, n)).cuda() for _ in range(3)]
+    multi = gram_multi(As, Bs)
+    for p in range(3):
+        single = gram(As[p], Bs[p])
+        assert torch.equal(multi[p], single)
+        assert torch.equal(single, gram(As[p], Bs[p]))
+
+
+def test_gram_unaligned_rows_use_vector_alu(engine_guard):
+    """Rows that start off a 16-byte boundary (odd n) take the vector-ALU path
+    under every engine and stay exact."""
+    import torch
+    from eigenfns_amd.linalg import gram
+    rng = np.random.default_rng(5)
+    n = 4097
+    A, B = _cblock(rng, 9, n), _cblock(rng, 6, n)
+    gr = A.astype(np.complex128).conj() @ B.astype(np.complex128).T
+    for engine in ENGINES:
+        engine_guard(engine)
+        g = gram(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()).cpu().numpy()
+        assert np.abs(g - gr).max() < 1e-12 * np.abs(gr).max()
