@@ -47,14 +47,17 @@ def row_len(X: torch.Tensor) -> int:
 
 
 class Scratch:
-    """Per-device cache of byte buffers for kernel work arrays (split partials)."""
+    """Cache of byte buffers for kernel work arrays (split partials), one set per
+    (device, stream): work queued on different streams never shares a buffer,
+    and a buffer is allocated while its stream is current, so the caching
+    allocator orders its reuse on that stream."""
 
     def __init__(self):
         self._bufs: dict[tuple, torch.Tensor] = {}
 
     def get(self, nbytes: int, tag: str = "w") -> torch.Tensor:
         dev = device()
-        key = (dev.index, tag)
+        key = (dev.index, torch.cuda.current_stream(dev).cuda_stream, tag)
         buf = self._bufs.get(key)
         if buf is None or buf.numel() < nbytes:
             buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=dev)

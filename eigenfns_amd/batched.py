@@ -186,7 +186,7 @@ class BatchedLOBPCG:
         """SVQB of block(s) for slots idx, two sweeps, written back in place."""
         for _ in range(passes):
             Bs = [block(s) for s in idx]
-            w, V = eigh_batched_host(gram_multi(Bs, Bs))
+            w, V = eigh_batched_host(gram_multi(Bs, Bs, hermitian=True))
             wmax = np.maximum(w.max(axis=1, keepdims=True), 1e-300)
             keep = w > SVQB_DROP * wmax
             scale = np.where(keep, 1.0 / np.sqrt(np.where(keep, w, 1.0)), 0.0)
@@ -250,7 +250,7 @@ class BatchedLOBPCG:
             Ps = [self._P(s) for s in idx]
             combine_multi(-gram_multi(XWs, Ps), XWs, Ps, Xs=Ps, alpha=1.0)
         Ps = [self._P(s) for s in idx]
-        wp, Vp = eigh_batched_host(gram_multi(Ps, Ps))
+        wp, Vp = eigh_batched_host(gram_multi(Ps, Ps, hermitian=True))
         wpmax = np.maximum(wp.max(axis=1, keepdims=True), 1e-300)
         keepp = (wp > SVQB_DROP * wpmax) & (wpmax > 0)
         scale = np.where(keepp, 1.0 / np.sqrt(np.where(keepp, wp, 1.0)), 0.0)
@@ -262,7 +262,7 @@ class BatchedLOBPCG:
         self._theta(idx, self.S, m, self.HS, m, 3 * m, 3 * m, 2 * m)
         t = self._tick("theta_WP", t)
         Sbs = [self._Sb(s) for s in idx]
-        A = gram_multi(Sbs, [self._HSb(s) for s in idx]).cpu().numpy()
+        A = gram_multi(Sbs, [self._HSb(s) for s in idx], hermitian=True).cpu().numpy()
         rn2 = rowreduce_strided(0, self.S, None, 3 * m, 3 * m, 3 * m, self.nslots * 3 * m,
                                 self.n).real.cpu().numpy().reshape(self.nslots, 3 * m)
         for j, s in enumerate(idx):

@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .linalg import (combine, deflate, gram, eigh, normalize_rows, ortho_block, rowdot,
+from .linalg import (combine, deflate, gram, gram_herm, eigh, normalize_rows, ortho_block, rowdot,
                      rownorm, rownorm2, row_axpby, axpby)
 
 
@@ -112,7 +112,7 @@ def chebsi_bottom_up(op, nev: int, guard: int = 48, degree: int = 200, tol: floa
         act = ortho_block(act)
         Hact = op.theta(act)
         stats.theta_applications += act.shape[0]
-        A = gram(act, Hact)
+        A = gram_herm(act, Hact)
         dead = rownorm(act) < 0.5
         A = (A + A.conj().T) / 2 + torch.diag(torch.where(dead, 1e4, 0.0).to(A.dtype))
         wA, VA = eigh(A)

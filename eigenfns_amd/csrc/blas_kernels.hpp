@@ -27,6 +27,24 @@ struct CombinePtrs {
   float2* out[kMaxProb];
 };
 
+// Output tile of blockIdx.y: tiles_j > 0 — row-major over a tiles_i × tiles_j
+// grid; tiles_j = −T — the upper block triangle ti ≤ tj of a T × T grid, row by
+// row (for Hermitian products; the sum kernel mirrors the lower triangle).
+__device__ __forceinline__ void tile_of(int by, int tiles_j, int& ti, int& tj) {
+  if (tiles_j > 0) {
+    ti = by / tiles_j;
+    tj = by - ti * tiles_j;
+    return;
+  }
+  const int T = -tiles_j;
+  ti = 0;
+  while (by >= T - ti) {
+    by -= T - ti;
+    ++ti;
+  }
+  tj = ti + by;
+}
+
 constexpr int kGT = 64;   // output tile edge
 constexpr int kGK = 32;   // k-chunk staged in LDS
 constexpr int kGPad = kGT + 1;
@@ -41,7 +59,9 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(PairPtrs pp, int ma, 
   __shared__ float2 Bs[kGK * kGPad];
   const int tid = threadIdx.x;
   const int ti = tid >> 4, tj = tid & 15;
-  const int i0 = (blockIdx.y / tiles_j) * kGT, j0 = (blockIdx.y % tiles_j) * kGT;
+  int ti_, tj_;
+  tile_of(blockIdx.y, tiles_j, ti_, tj_);
+  const int i0 = ti_ * kGT, j0 = tj_ * kGT;
   const long long kb = (long long)blockIdx.x * kper;
   const long long ke = kb + kper < n ? kb + kper : n;
   double ar[4][4], ai[4][4];
@@ -163,7 +183,9 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
   const float2* __restrict__ B = pp.b[blockIdx.z];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, g = lane >> 4;
-  const int i0 = (blockIdx.y / tiles_j) * kMWaveTile, j0 = (blockIdx.y % tiles_j) * kMWaveTile;
+  int ti_, tj_;
+  tile_of(blockIdx.y, tiles_j, ti_, tj_);
+  const int i0 = ti_ * kMWaveTile, j0 = tj_ * kMWaveTile;
   const long long kb_blk = (long long)blockIdx.x * kper;
   const long long ke_blk = kb_blk + kper < n ? kb_blk + kper : n;
   const long long kw = kper / 4;
@@ -289,20 +311,34 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
       }
 }
 
-// out[prob][e] = Σ_s part[prob][s][e] in fixed order (grid.y = problem)
+// out[prob][e] = Σ_s part[prob][s][e] in fixed order (grid.y = problem).
+// herm: only tiles ti ≤ tj were computed; the result is made exactly
+// Hermitian — entry (i, j) with i > j is the conjugate of (j, i) and the
+// diagonal is real.
 __global__ void sum_partials_kernel(const double2* __restrict__ part, int nsplit, long long count,
-                                    double2* __restrict__ out) {
+                                    int mb, int herm, double2* __restrict__ out) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= count) return;
   part += (long long)blockIdx.y * nsplit * count;
   out += (long long)blockIdx.y * count;
+  long long src = e;
+  double sign = 1.0;
+  if (herm) {
+    const int i = (int)(e / mb), j = (int)(e - (long long)i * mb);
+    if (i > j) {
+      src = (long long)j * mb + i;
+      sign = -1.0;
+    } else if (i == j) {
+      sign = 0.0;
+    }
+  }
   double re = 0.0, im = 0.0;
   for (int s = 0; s < nsplit; ++s) {
-    const double2 p = part[(long long)s * count + e];
+    const double2 p = part[(long long)s * count + src];
     re += p.x;
     im += p.y;
   }
-  out[e] = make_double2(re, im);
+  out[e] = make_double2(re, sign * im);
 }
 
 // ---------------------------------------------------------------------------

@@ -649,8 +649,11 @@ size_t efn_gram_multi_work_bytes(int nprob, int ma, int mb, int64_t n) {
   return nprob <= 0 ? 0 : (size_t)nprob * efn_gram_work_bytes(ma, mb, n);
 }
 
-int efn_gram_multi(int nprob, const void* const* A, const void* const* B, int ma, int mb, int64_t n,
-                   void* out, void* work, size_t work_bytes, void* stream) {
+}  // extern "C"
+
+namespace {
+int gram_multi_impl(int nprob, const void* const* A, const void* const* B, int ma, int mb, int64_t n,
+                    void* out, void* work, size_t work_bytes, bool upper, void* stream) {
   if (nprob < 0 || nprob > efn::kMaxProb)
     return fail(EFN_EINVAL, "nprob must be in [0, " + std::to_string(efn::kMaxProb) + "]");
   if (ma < 0 || mb < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
@@ -677,31 +680,44 @@ int efn_gram_multi(int nprob, const void* const* A, const void* const* B, int ma
   for (int i = 0; i < nprob; ++i)
     aligned = aligned && ((uintptr_t)A[i] % 16 == 0) && ((uintptr_t)B[i] % 16 == 0);
   const int engine = aligned ? g_gram_engine.load() : EFN_GRAM_VALU_F64;
-  if (engine == EFN_GRAM_VALU_F64) {
-    const int ti = (ma + efn::kGT - 1) / efn::kGT, tj = (mb + efn::kGT - 1) / efn::kGT;
-    hipLaunchKernelGGL(efn::gram_partial_kernel, dim3((unsigned)ns, (unsigned)(ti * tj), (unsigned)nprob),
-                       dim3(256), 0, s, pp, ma, mb, tj, (long long)n, kper, (double2*)work);
-  } else {
-    const int ti = (ma + efn::kMWaveTile - 1) / efn::kMWaveTile, tj = (mb + efn::kMWaveTile - 1) / efn::kMWaveTile;
-    const dim3 grid((unsigned)ns, (unsigned)(ti * tj), (unsigned)nprob);
-    if (engine == EFN_GRAM_MFMA_F64)
-      hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF64>, grid, dim3(256), 0, s, pp, ma, mb, tj,
-                         (long long)n, kper, (double2*)work);
-    else
-      hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF32>, grid, dim3(256), 0, s, pp, ma, mb, tj,
-                         (long long)n, kper, (double2*)work);
-  }
+  const int edge = engine == EFN_GRAM_VALU_F64 ? efn::kGT : efn::kMWaveTile;
+  const int ti = (ma + edge - 1) / edge, tj = (mb + edge - 1) / edge;
+  const int ntile = upper ? ti * (ti + 1) / 2 : ti * tj;
+  const int tiles_j = upper ? -ti : tj;
+  const dim3 grid((unsigned)ns, (unsigned)ntile, (unsigned)nprob);
+  if (engine == EFN_GRAM_VALU_F64)
+    hipLaunchKernelGGL(efn::gram_partial_kernel, grid, dim3(256), 0, s, pp, ma, mb, tiles_j, (long long)n,
+                       kper, (double2*)work);
+  else if (engine == EFN_GRAM_MFMA_F64)
+    hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF64>, grid, dim3(256), 0, s, pp, ma, mb, tiles_j,
+                       (long long)n, kper, (double2*)work);
+  else
+    hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF32>, grid, dim3(256), 0, s, pp, ma, mb, tiles_j,
+                       (long long)n, kper, (double2*)work);
   EFN_LAUNCH_CHECK();
   const long long count = (long long)ma * mb;
   hipLaunchKernelGGL(efn::sum_partials_kernel, dim3((unsigned)((count + 255) / 256), (unsigned)nprob),
-                     dim3(256), 0, s, (const double2*)work, ns, count, (double2*)out);
+                     dim3(256), 0, s, (const double2*)work, ns, count, mb, upper ? 1 : 0, (double2*)out);
   EFN_LAUNCH_CHECK();
   return EFN_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int efn_gram_multi(int nprob, const void* const* A, const void* const* B, int ma, int mb, int64_t n,
+                   void* out, void* work, size_t work_bytes, void* stream) {
+  return gram_multi_impl(nprob, A, B, ma, mb, n, out, work, work_bytes, false, stream);
+}
+
+int efn_gram_multi_herm(int nprob, const void* const* A, const void* const* B, int m, int64_t n,
+                        void* out, void* work, size_t work_bytes, void* stream) {
+  return gram_multi_impl(nprob, A, B, m, m, n, out, work, work_bytes, true, stream);
 }
 
 int efn_gram(const void* A, int ma, const void* B, int mb, int64_t n, void* out, void* work,
              size_t work_bytes, void* stream) {
-  return efn_gram_multi(1, &A, &B, ma, mb, n, out, work, work_bytes, stream);
+  return gram_multi_impl(1, &A, &B, ma, mb, n, out, work, work_bytes, false, stream);
 }
 
 int efn_combine_multi(int nprob, const void* C, const void* const* S, int m_in, int m_out, int64_t n,
@@ -722,11 +738,15 @@ int efn_combine_multi(int nprob, const void* C, const void* const* S, int m_in, 
   }
   hipStream_t s = (hipStream_t)stream;
   const unsigned gx = (unsigned)((n + 255) / 256);
+  // one row tile per 32 outputs: S is read ceil(m_out / 32) times
   if (m_out <= 8) {
-    hipLaunchKernelGGL((efn::combine_kernel<8>), dim3(gx, (m_out + 7) / 8, nprob), dim3(256), 0, s,
+    hipLaunchKernelGGL((efn::combine_kernel<8>), dim3(gx, 1, nprob), dim3(256), 0, s,
+                       (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
+  } else if (m_out <= 16) {
+    hipLaunchKernelGGL((efn::combine_kernel<16>), dim3(gx, 1, nprob), dim3(256), 0, s,
                        (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
   } else {
-    hipLaunchKernelGGL((efn::combine_kernel<16>), dim3(gx, (m_out + 15) / 16, nprob), dim3(256), 0, s,
+    hipLaunchKernelGGL((efn::combine_kernel<32>), dim3(gx, (m_out + 31) / 32, nprob), dim3(256), 0, s,
                        (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
   }
   EFN_LAUNCH_CHECK();

@@ -16,7 +16,7 @@ import time
 import numpy as np
 import torch
 
-from .linalg import (SVQB_DROP, combine, combine_into, eigh, gram, normalize_rows,
+from .linalg import (SVQB_DROP, combine, combine_into, eigh, gram, gram_herm, normalize_rows,
                      ortho_block, row_axpby, rowdot, rownorm, rownorm2, axpby)
 
 
@@ -93,7 +93,7 @@ def rr_extract(op, X, theta_chunk: int = 8):
         X = X.pop()
     X = ortho_block(X)
     HX = op.theta(X)
-    A = gram(X, HX)
+    A = gram_herm(X, HX)
     dead = rownorm(X) < 0.5
     A = (A + A.conj().T) / 2 + torch.diag(torch.where(dead, 1e8, 0.0).to(A.dtype))
     wA, VA = eigh(A)

@@ -71,6 +71,18 @@ def gram(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
     return out
 
 
+def gram_herm(A: torch.Tensor, B: torch.Tensor | None = None) -> torch.Tensor:
+    """AᴴB (B = A if None) for a product known to be Hermitian: the upper block
+    triangle of tiles is computed and mirrored (solver.py:119, :275-294)."""
+    A = as_block(A)
+    B = A if B is None else as_block(B)
+    if int(A.shape[0]) != int(B.shape[0]):
+        raise ValueError("gram_herm: blocks must have the same number of rows")
+    if int(A.shape[0]) == 0:
+        return torch.empty((0, 0), dtype=torch.complex128, device=A.device)
+    return gram_multi([A.contiguous()], [B.contiguous()], hermitian=True)[0]
+
+
 def combine_into(C: torch.Tensor, S: torch.Tensor, out: torch.Tensor, X=None,
                  alpha: float = 0.0) -> torch.Tensor:
     """out_i = alpha·X_i + Σ_j C[j, i]·S_j with S one contiguous block (m_in rows)."""
@@ -215,7 +227,7 @@ def ortho_block(X, passes: int = 2, drop: float = SVQB_DROP):
     Rank-dropped directions come back as zero rows."""
     X = as_block(X)
     for _ in range(passes):
-        Cw = svqb_coeffs(gram(X, X), drop)
+        Cw = svqb_coeffs(gram_herm(X), drop)
         Y = torch.empty_like(X)
         combine_into(Cw, X, Y)
         X = Y
@@ -238,8 +250,12 @@ def _ptrs(ts):
     return (C.c_void_p * len(ts))(*[None if t is None else t.data_ptr() for t in ts])
 
 
-def gram_multi(As, Bs) -> torch.Tensor:
-    """(nprob, m_a, m_b) complex128: A_pᴴ·B_p for each pair (FP64-accumulated)."""
+def gram_multi(As, Bs, hermitian: bool = False) -> torch.Tensor:
+    """(nprob, m_a, m_b) complex128: A_pᴴ·B_p for each pair (FP64-accumulated).
+
+    hermitian=True (square products known to be Hermitian, e.g. AᴴA or the
+    Rayleigh–Ritz AᴴΘA that is symmetrised afterwards): only the upper block
+    triangle of tiles is computed and mirrored."""
     nprob = len(As)
     if nprob == 0:
         raise ValueError("gram_multi: no problems")
@@ -255,8 +271,14 @@ def gram_multi(As, Bs) -> torch.Tensor:
     lib = _lib.load()
     need = lib.efn_gram_multi_work_bytes(nprob, ma, mb, n)
     work = SCRATCH.get(need, "gram")
-    _lib.check(lib.efn_gram_multi(nprob, _ptrs(As), _ptrs(Bs), ma, mb, n, out.data_ptr(),
-                                  work.data_ptr(), work.numel(), stream_ptr()), "gram_multi")
+    if hermitian:
+        if ma != mb:
+            raise ValueError("gram_multi: a Hermitian product needs square blocks")
+        _lib.check(lib.efn_gram_multi_herm(nprob, _ptrs(As), _ptrs(Bs), ma, n, out.data_ptr(),
+                                           work.data_ptr(), work.numel(), stream_ptr()), "gram_multi")
+    else:
+        _lib.check(lib.efn_gram_multi(nprob, _ptrs(As), _ptrs(Bs), ma, mb, n, out.data_ptr(),
+                                      work.data_ptr(), work.numel(), stream_ptr()), "gram_multi")
     return out
 
 

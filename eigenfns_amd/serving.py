@@ -78,21 +78,100 @@ def _stats(lat):
             "mean_s": float(lat.mean()), "max_s": float(lat.max())}
 
 
+def _slots_for(sp, m, guard, max_nev, max_slots, per_engine):
+    slot_bytes = (2 * 3 * m + 4 * m + (int(np.ceil(max_nev / (m - guard))) * (m - guard) + m)) \
+        * 2 * sp.grid ** 3 * 8
+    return int(max(1, min(max_slots, per_engine // slot_bytes)))
+
+
+def _summary(done, wall, turns, engines):
+    lat = [r.t_done - r.t_arrive for r in done]
+    timing = {}
+    for e in engines.values():
+        for k, v in e.timing.items():
+            timing[k] = round(timing.get(k, 0.0) + v, 3)
+    return {"requests": len(done), "wall_s": wall, "solves_per_s": len(done) / wall,
+            "turns": turns, "slots": {g: e.nslots for g, e in engines.items()},
+            "engine_steps": {g: e.steps for g, e in engines.items()},
+            "host_phase_s": timing, **_stats(lat)}
+
+
 def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slots=16,
-                  hbm_budget=160e9, verbose=False):
-    """Run the stream through per-grid batched engines.  Returns (results, summary)."""
+                  hbm_budget=160e9, verbose=False, threads=True):
+    """Run the stream through per-grid batched engines.  Returns (results, summary).
+
+    threads=True: each grid's engine runs in its own host thread on its own HIP
+    stream (default priority), so one engine's host work (small eigensolves,
+    admission) overlaps the others' kernels, and small kernels of different
+    engines can share the GPU.  Requests are admitted FIFO per grid."""
     cache = cache or StructureCache()
     grids = sorted({sp.grid for sp in specs})
     max_nev = max(sp.nev for sp in specs)
-    engines = {}
     per_engine = hbm_budget / max(len(grids), 1)
+    if not threads:
+        return _serve_sequential(specs, cache, m, guard, tol, maxit, max_slots, per_engine,
+                                 max_nev, verbose)
+    import threading
+    import torch
+    queues = {g: sorted([sp for sp in specs if sp.grid == g], key=lambda s: s.t_arrive)
+              for g in grids}
+    engines, done, errors, turns = {}, [], [], [0]
+    lock = threading.Lock()
+    t0 = time.perf_counter()
+
+    def worker(g):
+        try:
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                pending = queues[g]
+                eng = BatchedLOBPCG(g, m, guard,
+                                    _slots_for(pending[0], m, guard, max_nev, max_slots, per_engine),
+                                    max_nev, tol, maxit)
+                with lock:
+                    engines[g] = eng
+                while pending or eng.active():
+                    now = time.perf_counter() - t0
+                    while pending and pending[0].t_arrive <= now and eng.free_slots():
+                        sp = pending.pop(0)
+                        req = Request(_make_op(cache, sp, workspace=False), sp.nev, seed=sp.seed,
+                                      rid=sp.rid, meta={"grid": sp.grid, "n_points": sp.n_points})
+                        req.t_arrive = t0 + sp.t_arrive
+                        eng.admit(req)
+                    if not eng.active():
+                        time.sleep(max(0.0, min(0.01, pending[0].t_arrive - now)))
+                        continue
+                    finished = eng.step()
+                    for req in finished:
+                        req.op = None   # frees the request's context (device-synchronising)
+                    with lock:
+                        turns[0] += 1
+                        done.extend(finished)
+                        if verbose and turns[0] % 50 == 0:
+                            print(f"  turn {turns[0]}: done {len(done)}/{len(specs)}", flush=True)
+                stream.synchronize()
+        except BaseException as exc:  # re-raised in the caller
+            errors.append(exc)
+
+    workers = [threading.Thread(target=worker, args=(g,), name=f"engine-{g}") for g in grids]
+    for w in workers:
+        w.start()
+    for w in workers:
+        w.join()
+    if errors:
+        raise errors[0]
+    return done, _summary(done, time.perf_counter() - t0, turns[0], engines)
+
+
+def _serve_sequential(specs, cache, m, guard, tol, maxit, max_slots, per_engine, max_nev,
+                      verbose):
+    """One host thread, one stream: every loop turn steps each engine in turn."""
+    engines = {}
     pending = sorted(specs, key=lambda s: s.t_arrive)
     done = []
     t0 = time.perf_counter()
     turns = 0
     while pending or any(e.active() for e in engines.values()):
         now = time.perf_counter() - t0
-        # admission: arrived requests, FIFO, into a free slot of their grid's engine
         still = []
         for sp in pending:
             if sp.t_arrive > now:
@@ -100,11 +179,9 @@ def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slo
                 continue
             eng = engines.get(sp.grid)
             if eng is None:
-                slot_bytes = (2 * 3 * m + 4 * m + (int(np.ceil(max_nev / (m - guard))) * (m - guard)
-                                                   + m)) * 2 * sp.grid ** 3 * 8
-                nslots = int(max(1, min(max_slots, per_engine // slot_bytes)))
-                eng = engines[sp.grid] = BatchedLOBPCG(sp.grid, m, guard, nslots, max_nev, tol,
-                                                       maxit)
+                eng = engines[sp.grid] = BatchedLOBPCG(
+                    sp.grid, m, guard, _slots_for(sp, m, guard, max_nev, max_slots, per_engine),
+                    max_nev, tol, maxit)
             if eng.free_slots():
                 req = Request(_make_op(cache, sp, workspace=False), sp.nev, seed=sp.seed,
                               rid=sp.rid, meta={"grid": sp.grid, "n_points": sp.n_points})
@@ -126,17 +203,7 @@ def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slo
         if verbose and turns % 50 == 0:
             print(f"  turn {turns}: done {len(done)}/{len(specs)} "
                   f"active {[e.active() for e in engines.values()]}", flush=True)
-    wall = time.perf_counter() - t0
-    lat = [r.t_done - r.t_arrive for r in done]
-    timing = {}
-    for e in engines.values():
-        for k, v in e.timing.items():
-            timing[k] = round(timing.get(k, 0.0) + v, 3)
-    summary = {"requests": len(done), "wall_s": wall, "solves_per_s": len(done) / wall,
-               "turns": turns, "slots": {g: e.nslots for g, e in engines.items()},
-               "engine_steps": {g: e.steps for g, e in engines.items()},
-               "host_phase_s": timing, **_stats(lat)}
-    return done, summary
+    return done, _summary(done, time.perf_counter() - t0, turns, engines)
 
 
 def serve_plain(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400):
@@ -172,11 +239,13 @@ if __name__ == "__main__":
     ap.add_argument("--nev-max", type=int, default=64)
     ap.add_argument("--plain", action="store_true")
     ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--sequential", action="store_true", help="one host thread, one stream")
     args = ap.parse_args()
     grids = tuple(int(g) for g in args.grids.split(","))
     specs = request_stream(args.n, args.seed, grids=grids, nev_range=(8, args.nev_max))
     cache = StructureCache()
-    res, summ = serve_batched(specs, cache, max_slots=args.slots, verbose=True)
+    res, summ = serve_batched(specs, cache, max_slots=args.slots, verbose=True,
+                              threads=not args.sequential)
     print(json.dumps({"batched": summ}), flush=True)
     if args.plain:
         pres, psumm = serve_plain(specs, cache)

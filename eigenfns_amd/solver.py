@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from .linalg import (DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_DROP, apply_chunked, combine,
-                     combine_into, deflate, deflate_chunk_rows, eigh, gram, normalize_rows,
+                     combine_into, deflate, deflate_chunk_rows, eigh, gram, gram_herm, normalize_rows,
                      ortho_block, row_axpby, rowdot, rownorm, rownorm2, svqb_coeffs)
 
 __all__ = ["gram", "combine", "deflate", "ortho_block", "apply_chunked", "deflate_chunk_rows",
@@ -59,7 +59,7 @@ def _deflate_inplace(Y, L, nl):
 def _svqb_into(Y, tmp, drop=SVQB_DROP, passes=2):
     """SVQB of Y (two sweeps), result written back into Y; tmp has Y's shape."""
     for _ in range(passes):
-        Cw = svqb_coeffs(gram(Y, Y), drop)
+        Cw = svqb_coeffs(gram_herm(Y), drop)
         combine_into(Cw, Y, tmp)
         Y.copy_(tmp)
     return Y
@@ -169,7 +169,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                 XW = S[:2 * m]
                 for _ in range(2):
                     combine_into(-gram(XW, P), XW, P, X=P, alpha=1.0)
-                wp, Vp = eigh(gram(P, P))
+                wp, Vp = eigh(gram_herm(P))
                 keepp = wp > SVQB_DROP * torch.clamp(wp.max(), min=1e-300)
                 if not bool(keepp.any()):
                     have_p = False
@@ -183,7 +183,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             Sb, HSb = S[:nb], HS[:nb]
             op.theta_into(Sb[m:], HSb[m:])
             stats.theta_applications += nb - m
-            A = gram(Sb, HSb)
+            A = gram_herm(Sb, HSb)
             A = (A + A.conj().T) / 2
             dead = rownorm(Sb) < 0.5
             penalty = max(RR_DEAD_PENALTY, 100.0 * float(torch.diagonal(A).abs().max().item()))

@@ -141,6 +141,16 @@ size_t efn_gram_multi_work_bytes(int nprob, int ma, int mb, int64_t n);
 int efn_gram_multi(int nprob, const void* const* A_c64, const void* const* B_c64, int ma, int mb,
                    int64_t n, void* out_c128, void* work, size_t work_bytes, void* stream);
 
+/* Hermitian form of efn_gram_multi for products known to be Hermitian
+ * (A_pᴴA_p, or A_pᴴ·Θ·A_p that the caller symmetrises anyway): only the upper
+ * block triangle of output tiles is computed; out[i][j] for i > j is the
+ * conjugate of out[j][i] and the diagonal is real, so the result is exactly
+ * Hermitian.  m×m outputs; same work buffer size as
+ * efn_gram_multi_work_bytes(nprob, m, m, n).  The Rayleigh–Ritz matrix
+ * A = Sᴴ·Θ·S of solver.py:275-294 (symmetrised at :294) per problem. */
+int efn_gram_multi_herm(int nprob, const void* const* A_c64, const void* const* B_c64, int m,
+                        int64_t n, void* out_c128, void* work, size_t work_bytes, void* stream);
+
 /* Linear combination out_i = alpha·X_i + Σ_j C[j][i]·S_j (solver.py:49-57,
  * :106, :125): C is m_in×m_out complex64 on the device, S m_in rows, X
  * (optional, NULL = none) and out m_out rows of n complex64.  out must not

@@ -96,3 +96,28 @@ def test_gram_unaligned_rows_use_vector_alu(engine_guard):
         engine_guard(engine)
         g = gram(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()).cpu().numpy()
         assert np.abs(g - gr).max() < 1e-12 * np.abs(gr).max()
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_gram_hermitian_upper_tiles(engine, engine_guard):
+    """The Hermitian form is exactly Hermitian: its strict upper triangle equals
+    the full product bitwise, the lower triangle is its conjugate transpose and
+    the diagonal is the full product's real part."""
+    import torch
+    from eigenfns_amd.linalg import gram, gram_multi
+    engine_guard(engine)
+    rng = np.random.default_rng(13)
+    for m, n in [(40, 6000), (96, 2 * 16 ** 3), (7, 1000), (130, 4096)]:
+        A = torch.from_numpy(_cblock(rng, m, n)).cuda()
+        B = torch.from_numpy(_cblock(rng, m, n)).cuda()
+        for X, Y in ((A, A), (A, B)):
+            h = gram_multi([X], [Y], hermitian=True)[0].cpu().numpy()
+            full = gram(X, Y).cpu().numpy()
+            iu = np.triu_indices(m, 1)
+            assert np.array_equal(h[iu], full[iu])
+            assert np.array_equal(h, h.conj().T)
+            assert np.array_equal(np.diag(h), np.diag(full).real.astype(h.dtype))
+        hA = gram_multi([A], [A], hermitian=True)[0].cpu().numpy()
+        fA = gram(A, A).cpu().numpy()
+        tol = 2e-6 if engine == "mfma_f32" else 1e-12   # bars of the full-product test
+        assert np.abs(hA - fA).max() < tol * np.abs(fA).max()
