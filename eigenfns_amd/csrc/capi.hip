@@ -12,6 +12,7 @@
 #include "../../include/eigenfns_hip.h"
 #include "blas_kernels.hpp"
 #include "operator_kernels.hpp"
+#include "raster_kernels.hpp"
 
 namespace {
 
@@ -872,6 +873,27 @@ int efn_random_block(const efn_ctx* c, int kind, uint64_t seed, int64_t row0, in
   }(seed);
   hipLaunchKernelGGL(efn::random_block_kernel, dim3(grid_for(count)), dim3(256), 0, (hipStream_t)stream,
                      kind, key, (long long)row0, G3, count, c->sing_voxel, (float2*)out);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+
+int efn_rasterize(int grid_size, double box_size, double aspect, double bb, const double* rodf_dev,
+                  const int* rodi_dev, int nrods, int periodic, float eps_rod, float eps_bg,
+                  float* eps_dev, void* stream) {
+  if (grid_size <= 0 || grid_size > 4096) return fail(EFN_EINVAL, "grid_size must be in [1, 4096]");
+  if (!(box_size > 0.0) || !(aspect > 0.0) || !(bb >= 0.0))
+    return fail(EFN_EINVAL, "box_size and aspect must be positive, bb non-negative");
+  if (nrods < 0) return fail(EFN_EINVAL, "nrods must be non-negative");
+  if (!eps_dev || (nrods > 0 && (!rodf_dev || !rodi_dev))) return fail(EFN_EINVAL, "NULL pointer");
+  hipStream_t s = (hipStream_t)stream;
+  const long long count = (long long)grid_size * grid_size * grid_size;
+  hipLaunchKernelGGL(efn::fill_f32_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s,
+                     eps_dev, count, eps_bg);
+  EFN_LAUNCH_CHECK();
+  if (nrods == 0) return EFN_OK;
+  hipLaunchKernelGGL(efn::raster_kernel, dim3((unsigned)nrods), dim3(256), 0, s, rodf_dev, rodi_dev,
+                     grid_size, box_size, aspect, bb, periodic ? 1 : 0, eps_rod, eps_dev);
   EFN_LAUNCH_CHECK();
   return EFN_OK;
 }

@@ -16,42 +16,12 @@ arrival order, with `solver.lobpcg_blocks` on the same kernels.
 from __future__ import annotations
 
 import time
-from dataclasses import dataclass
 
 import numpy as np
 
 from .batched import BatchedLOBPCG, Request
 from .operator import MaxwellOperator
-from .structure import network_eps
-
-NET_SEEDS = {64: 0, 216: 2, 1000: 1}   # the seeded networks of the benchmark configurations
-
-
-@dataclass
-class Spec:
-    rid: int
-    n_points: int
-    grid: int
-    k_frac: tuple
-    nev: int
-    seed: int
-    t_arrive: float = 0.0
-
-
-def request_stream(n: int, seed: int = 0, rate: float | None = None,
-                   nets=(64, 216, 1000), grids=(32, 48, 64, 96, 128), nev_range=(8, 64)):
-    """`n` requests; Poisson arrivals at `rate` per second (None: all at t = 0)."""
-    rng = np.random.default_rng(seed)
-    t = 0.0
-    out = []
-    for i in range(n):
-        if rate:
-            t += rng.exponential(1.0 / rate)
-        out.append(Spec(rid=i, n_points=int(rng.choice(nets)), grid=int(rng.choice(grids)),
-                        k_frac=tuple(float(x) for x in rng.uniform(-0.5, 0.5, 3)),
-                        nev=int(rng.integers(nev_range[0], nev_range[1] + 1)),
-                        seed=int(rng.integers(0, 2 ** 31)), t_arrive=t))
-    return out
+from .workloads import NET_SEEDS, Spec, network, request_stream  # noqa: F401 (re-exported)
 
 
 class StructureCache:
@@ -63,7 +33,7 @@ class StructureCache:
     def get(self, n_points: int, grid: int):
         key = (n_points, grid)
         if key not in self._eps:
-            self._eps[key] = network_eps(n_points, NET_SEEDS[n_points], grid)
+            self._eps[key] = network(n_points, grid)
         return self._eps[key]
 
 

@@ -111,6 +111,75 @@ def rasterize_penlike(rods, grid_size: int, box_size: float, minor_radius: float
     return grid
 
 
+def rod_parameters(rods, grid_size: int, box_size: float, minor_radius: float = 0.2252,
+                   aspect_ratio: float = 2.5, periodic: bool = False):
+    """Per-rod FP64 parameters of the membership test, computed with the same
+    NumPy expressions as `rasterize_penlike` (structure.py:93-127): returns
+    (rodf (R,8) float64: p1u, unit axis n, L_u, 0; rodi (R,6) int32: first and
+    last voxel index per axis).  Zero-length rods are dropped."""
+    rods = np.asarray(rods, dtype=np.float64).reshape(-1, 6)
+    G = int(grid_size)
+    s = float(aspect_ratio)
+    b = float(minor_radius)
+    dx = box_size / G
+    half = box_size / 2.0
+    coords = (np.arange(G, dtype=np.float64) + 0.5) * dx - half
+    pad = b * max(1.0, s) + dx
+    p1u = rods[:, :3].copy()
+    p1u[:, 2] /= s
+    p2u = rods[:, 3:].copy()
+    p2u[:, 2] /= s
+    vu = p2u - p1u
+    # the reference takes the 1-D norm of each rod's axis (a BLAS dot); the
+    # same call per rod keeps L_u, and with it the mask, bit-identical
+    Lu = np.array([float(np.linalg.norm(v)) for v in vu], dtype=np.float64)
+    keep = Lu > 0.0
+    p1u, vu, Lu = p1u[keep], vu[keep], Lu[keep]
+    nu = vu / Lu[:, None]
+    lo = np.minimum(rods[keep, :3], rods[keep, 3:]) - pad
+    hi = np.maximum(rods[keep, :3], rods[keep, 3:]) + pad
+    if periodic:
+        r0 = np.floor((lo + half) / dx - 0.5).astype(np.int64)
+        r1 = np.ceil((hi + half) / dx - 0.5).astype(np.int64)
+    else:
+        r0 = np.empty(lo.shape, np.int64)
+        r1 = np.empty(lo.shape, np.int64)
+        for i in range(lo.shape[0]):
+            for a in range(3):
+                r0[i, a], r1[i, a] = _index_range_clipped(coords, lo[i, a], hi[i, a], G)
+    rodf = np.zeros((len(Lu), 8), np.float64)
+    rodf[:, :3] = p1u
+    rodf[:, 3:6] = nu
+    rodf[:, 6] = Lu
+    rodi = np.stack([r0[:, 0], r1[:, 0], r0[:, 1], r1[:, 1], r0[:, 2], r1[:, 2]], axis=1)
+    if np.abs(rodi).max(initial=0) >= 2 ** 31 - 1:
+        raise ValueError("rod index box exceeds the int32 range")
+    return rodf, rodi.astype(np.int32)
+
+
+def rasterize_penlike_device(rods, grid_size: int, box_size: float, minor_radius: float = 0.2252,
+                             aspect_ratio: float = 2.5, eps_rod: float = 2.9275 ** 2,
+                             eps_bg: float = 1.0, periodic: bool = False):
+    """`rasterize_penlike` with the voxel test on the GPU; returns the ε grid
+    as a (G, G, G) float32 device tensor, bit-identical to the host result."""
+    import torch
+    from . import _lib
+    from ._device import device, stream_ptr
+    G = int(grid_size)
+    rodf, rodi = rod_parameters(rods, G, box_size, minor_radius, aspect_ratio, periodic)
+    dev = device()
+    eps = torch.empty((G, G, G), dtype=torch.float32, device=dev)
+    f = torch.from_numpy(rodf).to(dev)
+    i = torch.from_numpy(np.ascontiguousarray(rodi)).to(dev)
+    b = float(minor_radius)
+    _lib.check(_lib.load().efn_rasterize(G, float(box_size), float(aspect_ratio), b * b,
+                                          f.data_ptr(), i.data_ptr(), int(rodf.shape[0]),
+                                          1 if periodic else 0, float(np.float32(eps_rod)),
+                                          float(np.float32(eps_bg)), eps.data_ptr(), stream_ptr()),
+               "rasterize")
+    return eps
+
+
 def filling_fraction(eps: np.ndarray, eps_bg: float = 1.0) -> float:
     return float((np.asarray(eps) != eps_bg).mean())
 
@@ -145,9 +214,12 @@ def synthetic_network(n_points: int, seed: int, k: int = 4, radius_factor: float
 
 
 def network_eps(n_points: int, seed: int, grid_size: int, radius_factor: float = 0.33,
-                eps_rod: float = 8.41):
-    """ε grid of a synthetic network (binary mask, scalar ε per voxel).  Returns (eps, L)."""
+                eps_rod: float = 8.41, on_device: bool = False):
+    """ε grid of a synthetic network (binary mask, scalar ε per voxel).  Returns
+    (eps, L): a NumPy array, or with on_device=True a float32 device tensor
+    rasterised on the GPU (bit-identical)."""
     rods, L, radius = synthetic_network(n_points, seed, radius_factor=radius_factor)
-    eps = rasterize_penlike(rods, grid_size, L, minor_radius=radius, aspect_ratio=1.0,
-                            eps_rod=eps_rod, eps_bg=1.0, periodic=True)
+    raster = rasterize_penlike_device if on_device else rasterize_penlike
+    eps = raster(rods, grid_size, L, minor_radius=radius, aspect_ratio=1.0, eps_rod=eps_rod,
+                 eps_bg=1.0, periodic=True)
     return eps, L

@@ -1,0 +1,114 @@
+"""The five benchmark workloads: structures, k-points and request streams.
+
+Each configuration is a seeded synthetic rod network (`structure.network_eps`)
+on a cubic grid:
+
+  C1  64 points (seed 0), 32³, Γ, plain LOBPCG, block 16, lowest 24 bands,
+      residual tol 1e-6 (the reference's CPU-runnable case).
+  C2  1000 points (seed 1), 128³, Γ, block 32, bands 1–128 (guard 12, the
+      run_modes.py:59 default), eigenvalues checked against a dense FP64 solve
+      of the same network rasterised at 16³.
+  C3  216 points (seed 2), 64³, 64 Bloch k-points uniformly spaced by arc
+      length along Γ–X–M–Γ–R (X = (½,0,0), M = (½,½,0), R = (½,½,½) in units
+      of 2π/L), block widths drawn uniformly from 8–48, nev = width − 4,
+      guard 4 (one block per k-point).
+  C4  10,000 points (seed 3), rod radius 0.331836 × mean bond length: KPM at
+      256³ (degree 12000, 16 Rademacher probes, FP64 moment sums), then a
+      two-stage ChebSI at 192³ on λ ∈ [1.757, 1.930] (normalised units),
+      m = 104, build degree 3000 × 2 outer steps, polish 12000 × 4.
+  C5  a Poisson stream of 2000 requests over N ∈ {64, 216, 1000},
+      G ∈ {32, 48, 64, 96, 128}, k ~ U[−½, ½)³, nev ∈ 8–64
+      (`serving.request_stream`).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+NET_SEEDS = {64: 0, 216: 2, 1000: 1, 10000: 3}
+RADIUS = {64: 0.33, 216: 0.33, 1000: 0.33, 10000: 0.331836}
+
+C1 = dict(n_points=64, grid=32, nev=24, m=16, guard=8, tol=1e-6)
+C2 = dict(n_points=1000, grid=128, nev=128, m=32, guard=12, tol=1e-6, dense_grid=16)
+C3 = dict(n_points=216, grid=64, n_k=64, width=(8, 48), guard=4, tol=1e-6, width_seed=0)
+C4 = dict(n_points=10000, kpm_grid=256, kpm_degree=12000, kpm_probes=16, kpm_seed=1,
+          cheb_grid=192, window=(1.757, 1.930), cheb_m=104, build=(3000, 2), polish=(12000, 4))
+C5 = dict(n=2000, nets=(64, 216, 1000), grids=(32, 48, 64, 96, 128), nev_range=(8, 64))
+
+# high-symmetry points of the simple cubic Brillouin zone (units of 2π/L)
+KPATH = {"G": (0.0, 0.0, 0.0), "X": (0.5, 0.0, 0.0), "M": (0.5, 0.5, 0.0), "R": (0.5, 0.5, 0.5)}
+
+
+def network(n_points: int, grid: int):
+    """(ε grid, box size L) of the seeded network with `n_points` points."""
+    from .structure import network_eps
+    return network_eps(n_points, NET_SEEDS[n_points], grid, radius_factor=RADIUS[n_points])
+
+
+def kpath(n: int, corners=("G", "X", "M", "G", "R")) -> np.ndarray:
+    """`n` k-points spaced uniformly by arc length along the corner path, both
+    ends included; (n, 3) in units of 2π/L."""
+    pts = np.array([KPATH[c] for c in corners], dtype=np.float64)
+    seg = np.linalg.norm(np.diff(pts, axis=0), axis=1)
+    cum = np.concatenate([[0.0], np.cumsum(seg)])
+    s = np.linspace(0.0, cum[-1], n)
+    out = np.empty((n, 3))
+    for i, si in enumerate(s):
+        j = min(int(np.searchsorted(cum, si, side="right")) - 1, len(seg) - 1)
+        t = (si - cum[j]) / seg[j]
+        out[i] = pts[j] + t * (pts[j + 1] - pts[j])
+    return out
+
+
+def c3_problems(n_k: int | None = None):
+    """[(k_frac, width, nev)] of the batched k-point configuration."""
+    n_k = C3["n_k"] if n_k is None else n_k
+    ks = kpath(C3["n_k"])[:n_k]
+    lo, hi = C3["width"]
+    widths = np.random.default_rng(C3["width_seed"]).integers(lo, hi + 1, C3["n_k"])[:n_k]
+    return [(tuple(float(x) for x in k), int(w), int(w) - 4) for k, w in zip(ks, widths)]
+
+
+@dataclass
+class Spec:
+    """One eigen-solve request of a stream."""
+    rid: int
+    n_points: int
+    grid: int
+    k_frac: tuple
+    nev: int
+    seed: int
+    t_arrive: float = 0.0
+
+
+def request_stream(n: int, seed: int = 0, rate: float | None = None, nets=C5["nets"],
+                   grids=C5["grids"], nev_range=C5["nev_range"]):
+    """`n` requests; Poisson arrivals at `rate` per second (None: all at t = 0)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        if rate:
+            t += rng.exponential(1.0 / rate)
+        out.append(Spec(rid=i, n_points=int(rng.choice(nets)), grid=int(rng.choice(grids)),
+                        k_frac=tuple(float(x) for x in rng.uniform(-0.5, 0.5, 3)),
+                        nev=int(rng.integers(nev_range[0], nev_range[1] + 1)),
+                        seed=int(rng.integers(0, 2 ** 31)), t_arrive=t))
+    return out
+
+
+# The serving sample timed by the benchmark: 24 requests over the three
+# smaller grids, all arriving at t = 0; the requests at 32³ with nev ≤ 32 are
+# checked against FP64 oracle eigenvalues.
+SERVING_SAMPLE = dict(n=24, seed=11, grids=(32, 48, 64))
+SERVING_SOLVER = dict(m=32, guard=8, tol=1e-4, maxit=400)
+
+
+def serving_sample():
+    return request_stream(SERVING_SAMPLE["n"], SERVING_SAMPLE["seed"],
+                          grids=SERVING_SAMPLE["grids"])
+
+
+def serving_oracle_subset(specs):
+    return [sp for sp in specs if sp.grid == 32 and sp.nev <= 32]

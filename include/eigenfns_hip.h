@@ -205,6 +205,21 @@ int efn_copy_multi(int nprob, const void* const* src_c64, void* const* dst_c64, 
 int efn_random_block(const efn_ctx* ctx, int kind, uint64_t seed, int64_t row0, int m,
                      void* out_c64, void* stream);
 
+/* Binary "pen-like" rasterisation of a rod network into ε (G³ floats on the
+ * device, C order), rasterize_penlike of structure.py:51-151.  The caller
+ * computes the per-rod FP64 parameters with the reference's expressions
+ * (eigenfns_amd.structure does): rodf_dev = nrods×8 doubles (p1u[3] — the
+ * first endpoint with z divided by the aspect ratio —, the unit axis n[3] in
+ * the unwarped space, its length L_u, unused), rodi_dev = nrods×6 ints (first
+ * and last voxel index along x, y, z; unwrapped indices when periodic,
+ * clipped to [0, G) otherwise).  bb = radius².  Every voxel centre in a rod's
+ * box is tested in FP64 with the reference's operation order and no fused
+ * multiply-add, so the mask is bit-identical to the host rasteriser's.
+ * Voxels inside a rod get eps_rod, all others eps_bg. */
+int efn_rasterize(int grid_size, double box_size, double aspect, double bb, const double* rodf_dev,
+                  const int* rodi_dev, int nrods, int periodic, float eps_rod, float eps_bg,
+                  float* eps_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,100 @@
+"""GPU parity of the solve workloads against FP64 oracle eigenvalues.
+
+The fixture tests/golden/solve_golden.npz (written on the CPU by
+tests/golden/make_solve_golden.py) holds: C1's 24 bands from the oracle's FP64
+LOBPCG at 32³; the dense FP64 spectrum (lowest 128) of C2's network at 16³;
+C3's two narrowest k-points at 64³ from the oracle LOBPCG; the serving
+sample's 32³ requests with nev ≤ 32 from the oracle LOBPCG.  Bar: 1e-6
+relative eigenvalue error, the accuracy the serving metric is quoted at.
+"""
+import numpy as np
+import pytest
+
+from eigenfns_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = "tests/golden/solve_golden.npz"
+TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def golden():
+    import json
+    import os
+    if not os.path.exists(GOLDEN):
+        pytest.skip("solve_golden.npz not generated")
+    z = np.load(GOLDEN, allow_pickle=False)
+    meta = {k[:-6]: json.loads(str(z[k])) for k in z.files if k.endswith("__meta")}
+    return z, meta
+
+
+def _need(z, key):
+    if key not in z.files:
+        pytest.skip(f"{key} not in solve_golden.npz")
+
+
+def _rel(vals, ref):
+    return float(np.max(np.abs(np.asarray(vals[:len(ref)]) - ref) / ref))
+
+
+def test_c1_plain_lobpcg_matches_fp64_oracle(golden):
+    from eigenfns_amd.solve_bench import run_c1
+    z, _ = golden
+    _need(z, "c1")
+    res = run_c1()
+    err = _rel(res["vals"], z["c1"])
+    print(f"C1: {res['iterations']} iterations, {res['gpu_seconds']} s, max rel err {err:.2e}")
+    assert len(res["vals"]) == wl.C1["nev"]
+    assert err < TOL
+
+
+def test_c2_network_at_16_matches_dense_fp64(golden):
+    from eigenfns_amd.operator import MaxwellOperator
+    from eigenfns_amd.solver import lobpcg_blocks
+    z, _ = golden
+    _need(z, "c2dense")
+    c = wl.C2
+    eps, L = wl.network(c["n_points"], c["dense_grid"])
+    vals, _, st = lobpcg_blocks(MaxwellOperator(eps, L), c["nev"], m=c["m"], guard=c["guard"],
+                                tol=c["tol"], verbose=False)
+    err = _rel(vals, z["c2dense"])
+    print(f"C2 at 16³: {st.iterations} iterations, max rel err vs dense FP64 {err:.2e}")
+    assert err < TOL
+
+
+def test_c3_kpoints_batched_match_fp64_oracle(golden):
+    from eigenfns_amd.batched import Request, solve_batched
+    from eigenfns_amd.operator import MaxwellOperator
+    z, meta = golden
+    keys = sorted(k for k in meta if k.startswith("c3_k"))
+    if not keys:
+        pytest.skip("no C3 k-points in solve_golden.npz")
+    eps, L = wl.network(wl.C3["n_points"], wl.C3["grid"])
+    probs = wl.c3_problems()
+    reqs = []
+    for key in keys:
+        j = meta[key]["index"]
+        k, width, nev = probs[j]
+        assert tuple(meta[key]["k"]) == pytest.approx(k) and meta[key]["nev"] == nev
+        reqs.append(Request(MaxwellOperator(eps, L, k, max_chunk=-1), nev, seed=j, rid=j))
+    done = solve_batched(reqs, wl.C3["grid"], m=16, guard=16 - max(r.nev for r in reqs),
+                         nslots=len(reqs), tol=wl.C3["tol"])
+    for req in done:
+        err = _rel(req.vals, z[f"c3_k{req.rid}"])
+        print(f"C3 k-point {req.rid}: nev {req.nev}, max rel err {err:.2e}")
+        assert err < TOL
+
+
+def test_serving_sample_matches_fp64_oracle(golden):
+    from eigenfns_amd.serving import StructureCache, serve_batched
+    z, meta = golden
+    subset = wl.serving_oracle_subset(wl.serving_sample())
+    if not all(f"srv_{sp.rid}" in meta for sp in subset):
+        pytest.skip("serving sample not in solve_golden.npz")
+    done, summ = serve_batched(subset, StructureCache(), **wl.SERVING_SOLVER)
+    assert len(done) == len(subset)
+    for req in done:
+        err = _rel(req.vals, z[f"srv_{req.rid}"])
+        print(f"request {req.rid}: nev {req.nev}, max rel err {err:.2e}")
+        assert err < TOL

@@ -84,3 +84,32 @@ def test_synthetic_network_geometry():
     inside = np.all(np.abs(rods[:, :3]) <= L / 2, axis=1) | np.all(np.abs(rods[:, 3:]) <= L / 2, axis=1)
     assert inside.all()
     assert 0 < radius < L / 10
+
+
+def test_workload_definitions():
+    """k-path corners and spacing, C3 widths, and the fixture metadata agree
+    with the workload definitions they were generated from."""
+    import json
+    import os
+    from eigenfns_amd import workloads as wl
+    ks = wl.kpath(64)
+    assert np.allclose(ks[0], 0) and np.allclose(ks[-1], 0.5)
+    steps = np.linalg.norm(np.diff(ks, axis=0), axis=1)
+    # uniform arc length except where a step turns a corner (shorter chord)
+    assert np.all(steps <= steps.max() + 1e-12) and np.median(steps) == pytest.approx(steps.max())
+    probs = wl.c3_problems()
+    assert len(probs) == 64
+    assert all(8 <= w <= 48 and nev == w - 4 for _, w, nev in probs)
+    path = os.path.join(os.path.dirname(__file__), "golden", "solve_golden.npz")
+    if not os.path.exists(path):
+        pytest.skip("solve_golden.npz not generated")
+    z = np.load(path, allow_pickle=False)
+    meta = {k[:-6]: json.loads(str(z[k])) for k in z.files if k.endswith("__meta")}
+    if "c1" in meta:
+        assert meta["c1"]["nev"] == wl.C1["nev"] == len(z["c1"])
+    for key, md in meta.items():
+        if key.startswith("srv_"):
+            sp = wl.serving_sample()[md["rid"]]
+            assert (sp.grid, sp.nev, sp.n_points) == (md["grid"], md["nev"], md["n_points"])
+            assert np.allclose(sp.k_frac, md["k"])
+        assert np.all(np.diff(z[key]) >= -1e-12) and np.all(z[key] > 0)
