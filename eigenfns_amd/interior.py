@@ -233,8 +233,14 @@ def two_stage(op, lam_lo: float, lam_hi: float, lam_max: float, m: int = 104,
             t0 = time.perf_counter()
             if X is None:
                 X = _random_start(op, m, seed)
-            X = torch.cat([_apply_cheb_poly(op, X[s:s + chunk], coef, lam_max)
-                           for s in range(0, m, chunk)], dim=0)
+            pieces = []
+            for s in range(0, m, chunk):
+                pieces.append(_apply_cheb_poly(op, X[s:s + chunk], coef, lam_max))
+                if verbose:
+                    print(f"  {stage} outer {outer}: filtered rows {s}..{min(s + chunk, m) - 1} "
+                          f"[{time.perf_counter() - t0:.1f}s]", flush=True)
+            X = torch.cat(pieces, dim=0)
+            del pieces
             lam, X, rn = rr_extract(op, X)
             n_theta += degree * m + 2 * m
             inwin = np.isfinite(lam) & (lam >= lam_lo) & (lam <= lam_hi)

@@ -13,7 +13,9 @@ tests/golden/solve_golden.npz where that file covers the case.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
+import sys
 import time
 from pathlib import Path
 
@@ -37,6 +39,10 @@ def _rel_err(vals, ref):
         return None
     n = min(len(vals), len(ref))
     return float(np.max(np.abs(np.asarray(vals[:n]) - ref[:n]) / np.abs(ref[:n])))
+
+
+def _progress(msg):
+    print(f"[solve_bench] {msg}", file=sys.stderr, flush=True)
 
 
 def _sync_time(fn):
@@ -77,19 +83,21 @@ def run_c2():
     del op16
     eps, L = wl.network(c["n_points"], c["grid"])
     op = MaxwellOperator(eps, L)
-    (vals, _, st), t = _sync_time(lambda: lobpcg_blocks(op, c["nev"], m=c["m"], guard=c["guard"],
-                                                        tol=c["tol"], seed=0, verbose=False))
+    with contextlib.redirect_stdout(sys.stderr):    # per-block progress lines
+        (vals, _, st), t = _sync_time(lambda: lobpcg_blocks(
+            op, c["nev"], m=c["m"], guard=c["guard"], tol=c["tol"], seed=0, verbose=True))
     res.update({"gpu_seconds": round(t, 2), "iterations": st.iterations,
                 "theta_applications": st.theta_applications,
                 "lambda_1_128": [float(vals[0]), float(vals[-1])]})
     return res
 
 
-def run_c3(n_k: int | None = None, plain: bool = True):
-    """Every k-point is a request; widths are bucketed into engines of block
-    16, 32 and 48 whose guard is the bucket minus its largest nev (≥ 4), so
-    each request locks its nev = width − 4 bands in one block.  The plain
-    baseline solves each k-point alone with block = width, guard 4."""
+def run_c3(n_k: int | None = None, plain: bool = True, bucket_step: int = 4):
+    """Every k-point is a request; widths are rounded up to a multiple of
+    `bucket_step` and each bucket is one engine (own host thread and stream)
+    whose guard is the bucket minus its largest nev (≥ 4), so each request
+    locks its nev = width − 4 bands in one block.  The plain baseline solves
+    each k-point alone with block = width, guard 4."""
     from .batched import BatchedLOBPCG, Request
     from .operator import MaxwellOperator
     from .solver import lobpcg_blocks
@@ -98,7 +106,7 @@ def run_c3(n_k: int | None = None, plain: bool = True):
     eps, L = wl.network(c["n_points"], c["grid"])
     buckets = {}
     for j, (k, w, nev) in enumerate(probs):
-        b = 16 if w <= 16 else (32 if w <= 32 else 48)
+        b = bucket_step * ((w + bucket_step - 1) // bucket_step)
         buckets.setdefault(b, []).append((j, k, w, nev))
 
     def batched():
@@ -121,6 +129,8 @@ def run_c3(n_k: int | None = None, plain: bool = True):
                         for req in eng.step():
                             done[req.rid] = req.vals
                             req.op = None
+                            _progress(f"C3 batched: k-point {req.rid} done "
+                                      f"({len(done)}/{len(probs)})")
                     torch.cuda.current_stream().synchronize()
             except BaseException as exc:
                 errors.append(exc)
@@ -154,6 +164,7 @@ def run_c3(n_k: int | None = None, plain: bool = True):
                 out[j] = lobpcg_blocks(op, nev, m=w, guard=c["guard"], tol=c["tol"], seed=j,
                                        verbose=False)[0]
                 del op
+                _progress(f"C3 plain: k-point {j} done ({len(out)}/{len(probs)})")
             return out
         pl, tp = _sync_time(one_at_a_time)
         res.update({"plain_seconds": round(tp, 2), "plain_k_per_s": round(len(probs) / tp, 3),
@@ -173,8 +184,9 @@ def run_c4kpm(degree: int | None = None):
     op = MaxwellOperator(eps, L)
     lam_max = 1.05 * lanczos_lambda_max(op)
     kpm_moments(op, 20, 2, seed=c["kpm_seed"], lam_max=lam_max)  # warm
-    mom, t = _sync_time(lambda: kpm_moments(op, degree, c["kpm_probes"], seed=c["kpm_seed"],
-                                            lam_max=lam_max))
+    with contextlib.redirect_stdout(sys.stderr):    # a line every 1000 degrees
+        mom, t = _sync_time(lambda: kpm_moments(op, degree, c["kpm_probes"], seed=c["kpm_seed"],
+                                                lam_max=lam_max, verbose=True))
     n_theta = c["kpm_probes"] * (degree // 2)
     per_probe_degree = (48 * G ** 3 + 4 * G ** 3 / c["kpm_probes"])
     return {"case": "C4-KPM", "grid": G, "degree": degree, "probes": c["kpm_probes"],
@@ -196,9 +208,10 @@ def run_c4cheb(outers: int | None = None, degree: int | None = None):
     op = MaxwellOperator(eps, L)
     lam_max = 1.05 * lanczos_lambda_max(op)
     lo, hi = c["window"]
-    (vals, _, rn, info), t = _sync_time(lambda: two_stage(
-        op, lo, hi, lam_max, m=c["cheb_m"], build_degree=degree, build_outers=outers,
-        polish_degree=c["polish"][0], polish_outers=0, verbose=True))
+    with contextlib.redirect_stdout(sys.stderr):
+        (vals, _, rn, info), t = _sync_time(lambda: two_stage(
+            op, lo, hi, lam_max, m=c["cheb_m"], build_degree=degree, build_outers=outers,
+            polish_degree=c["polish"][0], polish_outers=0, verbose=True))
     hist = info["history"]
     vdeg = c["cheb_m"] * degree * outers
     filt_s = sum(h["outer_seconds"] for h in hist)
@@ -233,6 +246,7 @@ def main(argv=None):
     ap.add_argument("case", choices=["c1", "c2", "c3", "c4kpm", "c4cheb", "c5"])
     ap.add_argument("--n-k", type=int, default=None)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--bucket-step", type=int, default=4)
     ap.add_argument("--degree", type=int, default=None)
     ap.add_argument("--outers", type=int, default=None)
     ap.add_argument("--n", type=int, default=200)
@@ -244,7 +258,7 @@ def main(argv=None):
     elif args.case == "c2":
         out = run_c2()
     elif args.case == "c3":
-        out = run_c3(args.n_k, plain=not args.no_plain)
+        out = run_c3(args.n_k, plain=not args.no_plain, bucket_step=args.bucket_step)
     elif args.case == "c4kpm":
         out = run_c4kpm(args.degree)
     elif args.case == "c4cheb":

@@ -30,8 +30,10 @@ NET_SEEDS = {64: 0, 216: 2, 1000: 1, 10000: 3}
 RADIUS = {64: 0.33, 216: 0.33, 1000: 0.33, 10000: 0.331836}
 
 C1 = dict(n_points=64, grid=32, nev=24, m=16, guard=8, tol=1e-6)
-C2 = dict(n_points=1000, grid=128, nev=128, m=32, guard=12, tol=1e-6, dense_grid=16)
-C3 = dict(n_points=216, grid=64, n_k=64, width=(8, 48), guard=4, tol=1e-6, width_seed=0)
+# C2 and C3 use the reference's production residual tolerance (run_modes.py:61);
+# their eigenvalues are checked against FP64 values at 1e-6.
+C2 = dict(n_points=1000, grid=128, nev=128, m=32, guard=12, tol=1e-4, dense_grid=16)
+C3 = dict(n_points=216, grid=64, n_k=64, width=(8, 48), guard=4, tol=1e-4, width_seed=0)
 C4 = dict(n_points=10000, kpm_grid=256, kpm_degree=12000, kpm_probes=16, kpm_seed=1,
           cheb_grid=192, window=(1.757, 1.930), cheb_m=104, build=(3000, 2), polish=(12000, 4))
 C5 = dict(n=2000, nets=(64, 216, 1000), grids=(32, 48, 64, 96, 128), nev_range=(8, 64))
