@@ -6,10 +6,13 @@ every entry point raises — there is no CPU or framework fallback.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import threading
 from pathlib import Path
 
-_LIB_PATH = Path(__file__).resolve().parent / "libeigenfns_hip.so"
+# EIGENFNS_HIP_LIB names another build of the same library (same C ABI)
+_LIB_PATH = Path(os.environ.get("EIGENFNS_HIP_LIB") or
+                 Path(__file__).resolve().parent / "libeigenfns_hip.so")
 _lock = threading.Lock()
 _lib = None
 

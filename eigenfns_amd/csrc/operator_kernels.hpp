@@ -116,9 +116,18 @@ __device__ __forceinline__ Frame point_frame(const FrameTables& ft, const RowFra
 __host__ __device__ constexpr int ztile_for(int G) { return G >= 16 ? 16 : G; }
 
 // Rows per zin/zout block when all three components are in LDS at once: a
-// power of two dividing G² (a block never spans two vectors), 8–32 rows.
+// power of two dividing G² (a block never spans two vectors).  Measured on
+// MI355X (Θ, 32-vector blocks): 4 rows at G ≥ 128 and 8 at 48 ≤ G < 128 beat
+// 8/16 by 7 % at 192³ and 14 % at 96³ — more, smaller blocks per CU hide the
+// HBM latency of the direct stage-1/stage-3 accesses better than fuller ones.
+#ifndef EFN_ZROWS_BIG
+#define EFN_ZROWS_BIG 4
+#endif
+#ifndef EFN_ZROWS_MID
+#define EFN_ZROWS_MID 8
+#endif
 __host__ __device__ constexpr int zrows3_for(int G) {
-  int target = G >= 128 ? 8 : (G >= 48 ? 16 : 32);
+  int target = G >= 128 ? EFN_ZROWS_BIG : (G >= 48 ? EFN_ZROWS_MID : 32);
   int p = 1;
   while (p * 2 <= target && (G * G) % (p * 2) == 0) p *= 2;
   return p;
@@ -127,6 +136,10 @@ template <int G>
 __host__ __device__ constexpr int zrows3() { return zrows3_for(G); }
 
 constexpr int kThreads = 256;
+
+#ifndef EFN_ZIN_NATURAL
+#define EFN_ZIN_NATURAL 0
+#endif
 
 
 // ---------------------------------------------------------------------------
@@ -185,6 +198,16 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
   }
   __syncthreads();
   float2* w = ws + (long long)v * 3 * G3;
+#if EFN_ZIN_NATURAL
+  // step 3 back to LDS in natural order, then each component's R rows (one
+  // contiguous run of R·G points) stored with consecutive lanes on consecutive z
+  lds_step3_natural<G, +1, 3 * R, kThreads>(buf);
+  for (int e = threadIdx.x; e < 3 * R * G; e += kThreads) {
+    const int c = e / (R * G), rz = e - c * (R * G);
+    const int r = rz / G, z = rz - r * G;
+    w[c * G3 + xy0 * G + rz] = buf[(c * R + r) * P::kStride + P::pos(z)];
+  }
+#else
   for (int t = threadIdx.x; t < 3 * R * N1; t += kThreads) {
     const int p = t / N1, k1 = t - p * N1;
     const int c = p / R, r = p - c * R;
@@ -194,6 +217,7 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) dst[k1 + N1 * k2] = o[k2];
   }
+#endif
 }
 
 // ---------------------------------------------------------------------------

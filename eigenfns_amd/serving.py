@@ -15,6 +15,7 @@ arrival order, with `solver.lobpcg_blocks` on the same kernels.
 """
 from __future__ import annotations
 
+import sys
 import time
 
 import numpy as np
@@ -48,10 +49,28 @@ def _stats(lat):
             "mean_s": float(lat.mean()), "max_s": float(lat.max())}
 
 
-def _slots_for(sp, m, guard, max_nev, max_slots, per_engine):
-    slot_bytes = (2 * 3 * m + 4 * m + (int(np.ceil(max_nev / (m - guard))) * (m - guard) + m)) \
-        * 2 * sp.grid ** 3 * 8
-    return int(max(1, min(max_slots, per_engine // slot_bytes)))
+def _slot_bytes(grid, m, guard, max_nev):
+    """Device bytes one engine slot holds: [X; W; P] and Θ of it, R, two
+    temporaries, the guard carry and the locked buffer (BatchedLOBPCG)."""
+    cap = int(np.ceil(max_nev / (m - guard))) * (m - guard) + m
+    return (2 * 3 * m + 4 * m + cap) * 2 * grid ** 3 * 8
+
+
+def plan_slots(grids, m, guard, max_nev, max_slots, hbm_budget):
+    """Slots per grid under one HBM budget: every grid gets the same count
+    first (slot bytes grow as G³, so equal counts spend the budget in
+    proportion to each grid's work per request), then what is left goes to the
+    cheapest grids, up to `max_slots` each."""
+    grids = sorted(set(grids))
+    cost = {g: _slot_bytes(g, m, guard, max_nev) for g in grids}
+    base = int(min(max_slots, max(1, hbm_budget // sum(cost.values()))))
+    slots = {g: base for g in grids}
+    left = hbm_budget - sum(cost[g] * base for g in grids)
+    for g in grids:                                    # smallest grid first
+        extra = int(min(max_slots - slots[g], max(0, left) // cost[g]))
+        slots[g] += extra
+        left -= extra * cost[g]
+    return slots
 
 
 def _summary(done, wall, turns, engines):
@@ -66,8 +85,8 @@ def _summary(done, wall, turns, engines):
             "host_phase_s": timing, **_stats(lat)}
 
 
-def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slots=16,
-                  hbm_budget=160e9, verbose=False, threads=True):
+def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slots=32,
+                  hbm_budget=200e9, verbose=False, threads=True, progress=False):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
 
     threads=True: each grid's engine runs in its own host thread on its own HIP
@@ -77,10 +96,9 @@ def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slo
     cache = cache or StructureCache()
     grids = sorted({sp.grid for sp in specs})
     max_nev = max(sp.nev for sp in specs)
-    per_engine = hbm_budget / max(len(grids), 1)
+    plan = plan_slots(grids, m, guard, max_nev, max_slots, hbm_budget)
     if not threads:
-        return _serve_sequential(specs, cache, m, guard, tol, maxit, max_slots, per_engine,
-                                 max_nev, verbose)
+        return _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose)
     import threading
     import torch
     queues = {g: sorted([sp for sp in specs if sp.grid == g], key=lambda s: s.t_arrive)
@@ -94,9 +112,7 @@ def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slo
             stream = torch.cuda.Stream()
             with torch.cuda.stream(stream):
                 pending = queues[g]
-                eng = BatchedLOBPCG(g, m, guard,
-                                    _slots_for(pending[0], m, guard, max_nev, max_slots, per_engine),
-                                    max_nev, tol, maxit)
+                eng = BatchedLOBPCG(g, m, guard, min(plan[g], len(pending)), max_nev, tol, maxit)
                 with lock:
                     engines[g] = eng
                 while pending or eng.active():
@@ -116,6 +132,9 @@ def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slo
                     with lock:
                         turns[0] += 1
                         done.extend(finished)
+                        if finished and progress:
+                            print(f"[serve] batched: {len(done)}/{len(specs)} done "
+                                  f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
                         if verbose and turns[0] % 50 == 0:
                             print(f"  turn {turns[0]}: done {len(done)}/{len(specs)}", flush=True)
                 stream.synchronize()
@@ -132,8 +151,7 @@ def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slo
     return done, _summary(done, time.perf_counter() - t0, turns[0], engines)
 
 
-def _serve_sequential(specs, cache, m, guard, tol, maxit, max_slots, per_engine, max_nev,
-                      verbose):
+def _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose):
     """One host thread, one stream: every loop turn steps each engine in turn."""
     engines = {}
     pending = sorted(specs, key=lambda s: s.t_arrive)
@@ -149,9 +167,8 @@ def _serve_sequential(specs, cache, m, guard, tol, maxit, max_slots, per_engine,
                 continue
             eng = engines.get(sp.grid)
             if eng is None:
-                eng = engines[sp.grid] = BatchedLOBPCG(
-                    sp.grid, m, guard, _slots_for(sp, m, guard, max_nev, max_slots, per_engine),
-                    max_nev, tol, maxit)
+                eng = engines[sp.grid] = BatchedLOBPCG(sp.grid, m, guard, plan[sp.grid], max_nev,
+                                                       tol, maxit)
             if eng.free_slots():
                 req = Request(_make_op(cache, sp, workspace=False), sp.nev, seed=sp.seed,
                               rid=sp.rid, meta={"grid": sp.grid, "n_points": sp.n_points})
@@ -176,7 +193,7 @@ def _serve_sequential(specs, cache, m, guard, tol, maxit, max_slots, per_engine,
     return done, _summary(done, time.perf_counter() - t0, turns, engines)
 
 
-def serve_plain(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400):
+def serve_plain(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, progress=False):
     """The same stream solved one request at a time with lobpcg_blocks (FIFO)."""
     from .solver import lobpcg_blocks
     cache = cache or StructureCache()
@@ -192,6 +209,9 @@ def serve_plain(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400):
         t_done = time.perf_counter() - t0
         out.append({"rid": sp.rid, "vals": vals, "latency": t_done - sp.t_arrive,
                     "iterations": st.iterations})
+        if progress:
+            print(f"[serve] plain: {len(out)}/{len(specs)} done ({t_done:.1f}s)", file=sys.stderr,
+                  flush=True)
         del op
     wall = time.perf_counter() - t0
     summary = {"requests": len(out), "wall_s": wall, "solves_per_s": len(out) / wall,
@@ -208,7 +228,7 @@ if __name__ == "__main__":
     ap.add_argument("--grids", default="32,48,64,96,128")
     ap.add_argument("--nev-max", type=int, default=64)
     ap.add_argument("--plain", action="store_true")
-    ap.add_argument("--slots", type=int, default=16)
+    ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--sequential", action="store_true", help="one host thread, one stream")
     args = ap.parse_args()
     grids = tuple(int(g) for g in args.grids.split(","))

@@ -229,11 +229,11 @@ def run_c5(n: int, rate: float | None, seed: int = 0, plain: bool = True, grids=
     cache = StructureCache()
     for sp in specs:
         cache.get(sp.n_points, sp.grid)
-    res, summ = serve_batched(specs, cache, max_slots=32)
+    res, summ = serve_batched(specs, cache, max_slots=32, progress=True)
     out = {"case": "C5", "requests": n, "rate": rate, "grids": list(grids or wl.C5["grids"]),
            "batched": summ}
     if plain:
-        pres, psumm = serve_plain(specs, cache)
+        pres, psumm = serve_plain(specs, cache, progress=True)
         by = {r.rid: r.vals for r in res}
         out["plain"] = psumm
         out["speedup"] = summ["solves_per_s"] / psumm["solves_per_s"]
