@@ -62,6 +62,7 @@ _SIGNATURES = {
     "efn_copy_multi": (_i, [_i, _vp, _vp, _i64, _vp]),
     "efn_random_block": (_i, [_vp, _i, C.c_uint64, _i64, _i, _vp, _vp]),
     "efn_rasterize": (_i, [_i, _d, _d, _d, _vp, _vp, _i, _i, _f, _f, _vp, _vp]),
+    "efn_realspace": (_i, [_vp, _i, _vp, _vp, _vp, _i, _vp]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

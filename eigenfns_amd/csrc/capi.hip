@@ -77,6 +77,7 @@ struct efn_ctx {
   efn::OpDesc* d_desc = nullptr; // this operator's descriptor for the pass kernels
   size_t ws_bytes = 0;
   int (*run)(efn_ctx*, const OpRequest&, hipStream_t) = nullptr;
+  int (*real)(efn_ctx*, const OpRequest&, int, const float*, void*, hipStream_t) = nullptr;
 
   FrameTables tables() const {
     FrameTables ft;
@@ -182,10 +183,45 @@ int run_op(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
 struct GridEntry {
   int G;
   int (*run)(efn_ctx*, const OpRequest&, hipStream_t);
+  int (*real)(efn_ctx*, const OpRequest&, int, const float*, void*, hipStream_t);
   int zrows;  // z-pencils per zin/zout block
 };
 
-#define EFN_GRID(g) {g, &run_op<g>, efn::zrows3<g>()}
+// Real-space fields (modes: efn::RealMode): zin forms the Cartesian field
+// (H, or the curl for E), then inverse y and x passes and the epilogue.
+template <int G>
+int run_real(efn_ctx* c, const OpRequest& rq, int mode, const float* lam, void* out, hipStream_t s) {
+  using namespace efn;
+  constexpr int R = zrows3<G>();
+  constexpr int nzt = (G + ztile_for(G) - 1) / ztile_for(G);
+  constexpr int nb = G * G / R;
+  constexpr long long G3 = (long long)G * G * G;
+  for (int v0 = 0; v0 < rq.m; v0 += c->chunk) {
+    const int nv = rq.m - v0 < c->chunk ? rq.m - v0 : c->chunk;
+    const dim3 gz((unsigned)(nv * nb)), gs((unsigned)(nv * 3 * G * nzt)), blk(kThreads);
+    if (mode == kRealH)
+      hipLaunchKernelGGL((zin_kernel<G, kCartH>), gz, blk, 0, s, rq.in, c->d_ws, rq.vm, (long long)v0, c->d_twid);
+    else
+      hipLaunchKernelGGL((zin_kernel<G, kTheta>), gz, blk, 0, s, rq.in, c->d_ws, rq.vm, (long long)v0, c->d_twid);
+    EFN_LAUNCH_CHECK();
+    hipLaunchKernelGGL((strided_kernel<G, 1, +1>), gs, blk, 0, s, c->d_ws, c->d_twid);
+    hipLaunchKernelGGL((strided_kernel<G, 0, +1>), gs, blk, 0, s, c->d_ws, c->d_twid);
+    EFN_LAUNCH_CHECK();
+    const long long count = (long long)nv * G3;
+    const dim3 ge((unsigned)((count + kThreads - 1) / kThreads));
+    const float inv_n = 1.0f / (float)G3, n_f = (float)G3;
+    if (mode == kRealH)
+      hipLaunchKernelGGL((real_epilogue_kernel<kRealH>), ge, blk, 0, s, c->d_ws, rq.vm, (long long)v0, count, G3, inv_n, n_f, lam, out);
+    else if (mode == kRealE)
+      hipLaunchKernelGGL((real_epilogue_kernel<kRealE>), ge, blk, 0, s, c->d_ws, rq.vm, (long long)v0, count, G3, inv_n, n_f, lam, out);
+    else
+      hipLaunchKernelGGL((real_epilogue_kernel<kRealEd>), ge, blk, 0, s, c->d_ws, rq.vm, (long long)v0, count, G3, inv_n, n_f, lam, out);
+    EFN_LAUNCH_CHECK();
+  }
+  return EFN_OK;
+}
+
+#define EFN_GRID(g) {g, &run_op<g>, &run_real<g>, efn::zrows3<g>()}
 const GridEntry kGrids[] = {EFN_GRID(4),   EFN_GRID(6),   EFN_GRID(8),   EFN_GRID(12),
                             EFN_GRID(16),  EFN_GRID(24),  EFN_GRID(32),  EFN_GRID(48),
                             EFN_GRID(64),  EFN_GRID(96),  EFN_GRID(128), EFN_GRID(192),
@@ -259,6 +295,7 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   c->L = L;
   c->k[0] = kx; c->k[1] = ky; c->k[2] = kz;
   c->run = ge->run;
+  c->real = ge->real;
   const long long G3 = (long long)G * G * G;
   // --- FP64 axis values exactly as operator.py:47-49 (2π·fftfreq(G, L/G) + 2π·k/L)
   std::vector<double> ax[3];
@@ -471,6 +508,22 @@ int efn_theta(efn_ctx* c, const void* in, void* out, int m, float alpha, float b
   rq.ea.beta = beta;
   rq.ea.out = (float2*)out;
   return c->run(c, rq, (hipStream_t)stream);
+}
+
+int efn_realspace(efn_ctx* c, int mode, const void* in, const float* lam_dev, void* out, int m,
+                  void* stream) {
+  int rc;
+  if ((rc = check_block(c, in, m, "in")) || (rc = check_block(c, out, m, "out")) || (rc = check_ws(c)))
+    return rc;
+  if (mode < efn::kRealH || mode > efn::kRealEd) return fail(EFN_EINVAL, "mode must be 0 (H), 1 (E) or 2 (eps|E|^2)");
+  if (m == 0) return EFN_OK;
+  if (in == out) return fail(EFN_EINVAL, "in and out must not alias");
+  OpRequest rq{};
+  rq.kind = mode == efn::kRealH ? efn::kCartH : efn::kTheta;
+  rq.in = (const float2*)in;
+  rq.m = m;
+  rq.vm = single_map(c, m);
+  return c->real(c, rq, mode, lam_dev, out, (hipStream_t)stream);
 }
 
 int efn_precondition(efn_ctx* c, const void* in, void* out, int m, void* stream) {

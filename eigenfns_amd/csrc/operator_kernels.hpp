@@ -22,7 +22,7 @@
 
 namespace efn {
 
-enum OpKind { kTheta = 0, kPrecond = 1 };
+enum OpKind { kTheta = 0, kPrecond = 1, kCartH = 2 };
 enum Epi { kStore = 0, kCheb = 1, kKpm = 2, kDot = 3 };
 
 struct FrameTables {
@@ -177,6 +177,9 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
     if (KIND == kTheta) {  // c1 = −i·kn·b, c2 = i·kn·a
       c1 = make_float2(f.kn * b.y, -f.kn * b.x);
       c2 = make_float2(-f.kn * a.y, f.kn * a.x);
+    } else if (KIND == kCartH) {  // Cartesian H = a·t̂₁ + b·t̂₂ (operator.py:169-172)
+      c1 = a;
+      c2 = b;
     } else {               // c1 = i·b/knr, c2 = −i·a/knr
       const float inv = 1.0f / fmaxf(f.kn, ft.kn_floor);
       c1 = make_float2(-b.y * inv, b.x * inv);
@@ -495,6 +498,51 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
       ea.partial[2 * (long long)blockIdx.x] = t0;
       ea.partial[2 * (long long)blockIdx.x + 1] = t1;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Real-space fields from the inverse-transformed workspace W (3 Cartesian
+// components, unnormalised inverse FFT) of the vectors v0 .. v0+nv−1:
+//   kRealH:  H(r) = W/G³                                  (operator.py:174-176)
+//   kRealE:  E(r) = (ε⁻¹/G³)·W/√λ                         (operator.py:178-184)
+//   kRealEd: ε|E|² summed over the components, FP32        (run_modes.py:120-122)
+// out: (m,3,G³) complex64 for H and E, (m,G³) float32 for ε|E|²; rows by the
+// vector map's output offsets; lam: eigenvalue per input vector (NULL: 1).
+enum RealMode { kRealH = 0, kRealE = 1, kRealEd = 2 };
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void real_epilogue_kernel(const float2* __restrict__ ws, VecMap vm,
+                                                                 long long v0, long long count,
+                                                                 long long G3, float inv_n,
+                                                                 float n_f, const float* __restrict__ lam,
+                                                                 void* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= count) return;
+  const long long v = i / G3, vox = i - v * G3;
+  const VecRef vr = map_vector(vm, v0 + v);
+  const float2* w = ws + v * 3 * G3 + vox;
+  const float2 d0 = w[0], d1 = w[G3], d2 = w[2 * G3];
+  if (MODE == kRealH) {
+    float2* o = static_cast<float2*>(out) + vr.out_vec * 3 * G3 + vox;
+    o[0] = cscale(d0, inv_n);
+    o[G3] = cscale(d1, inv_n);
+    o[2 * G3] = cscale(d2, inv_n);
+    return;
+  }
+  const float rs = lam ? 1.0f / sqrtf(lam[vr.in_vec]) : 1.0f;
+  const float sc = vr.desc->inv_eps[vox] * rs;
+  const float2 e0 = cscale(d0, sc), e1 = cscale(d1, sc), e2 = cscale(d2, sc);
+  if (MODE == kRealE) {
+    float2* o = static_cast<float2*>(out) + vr.out_vec * 3 * G3 + vox;
+    o[0] = e0;
+    o[G3] = e1;
+    o[2 * G3] = e2;
+  } else {
+    const float eps = vr.desc->eps[vox] * n_f;  // ε (the table holds ε/G³)
+    const float s2 = (e0.x * e0.x + e0.y * e0.y) + (e1.x * e1.x + e1.y * e1.y) +
+                     (e2.x * e2.x + e2.y * e2.y);
+    static_cast<float*>(out)[vr.out_vec * G3 + vox] = eps * s2;
   }
 }
 

@@ -245,19 +245,39 @@ class MaxwellOperator:
         t = self.basis.t
         return X[:, 0:1] * t[0][None] + X[:, 1:2] * t[1][None]
 
+    def _realspace(self, Hs, mode: int, eigvals, out_shape, dtype):
+        X = self._check_block(Hs)
+        m = X.shape[0]
+        out = torch.empty((m,) + out_shape, dtype=dtype, device=self._dev)
+        lam = None
+        if eigvals is not None:
+            lam = torch.as_tensor(np.asarray(eigvals, dtype=np.float32).reshape(-1),
+                                  device=self._dev)
+            if lam.numel() != m:
+                raise ValueError(f"{m} vectors but {lam.numel()} eigenvalues")
+        _lib.check(_lib.load().efn_realspace(self._ctx, mode, X.data_ptr(),
+                                             None if lam is None else lam.data_ptr(),
+                                             out.data_ptr(), m, stream_ptr()), "realspace")
+        return out
+
     def h_realspace(self, Hs) -> torch.Tensor:
-        """Real-space Cartesian H(r) for a block (post-solve; not on the hot path)."""
-        return torch.fft.ifftn(self.to_cartesian_spectral(Hs), dim=(2, 3, 4))
+        """Real-space Cartesian H(r) = ifftn(a·t̂₁ + b·t̂₂) for a block (operator.py:174-176),
+        (m,3,G,G,G) complex64."""
+        G = self.grid_size
+        return self._realspace(Hs, 0, None, (3, G, G, G), torch.complex64)
 
     def e_realspace(self, Hs, eigvals) -> torch.Tensor:
-        """Real-space E(r) ∝ ε⁻¹ ∇×H / λ^{1/2} for a block (post-solve; not on the hot path)."""
-        X = self._check_block(Hs)
-        kn, t = self.basis.kn, self.basis.t
-        a, b = X[:, 0], X[:, 1]
-        Ec = (-1j * kn * b)[:, None] * t[0][None] + (1j * kn * a)[:, None] * t[1][None]
-        E = self.inv_eps * torch.fft.ifftn(Ec, dim=(2, 3, 4))
-        lam = torch.as_tensor(np.asarray(eigvals), dtype=torch.float32, device=self._dev)
-        return E / torch.sqrt(lam)[:, None, None, None, None]
+        """Real-space E(r) = ε⁻¹·ifftn(∇×H)/λ^{1/2} for a block (operator.py:178-184),
+        (m,3,G,G,G) complex64."""
+        G = self.grid_size
+        return self._realspace(Hs, 1, eigvals, (3, G, G, G), torch.complex64)
+
+    def energy_density(self, Hs, eigvals) -> torch.Tensor:
+        """ε·|E(r)|² per voxel for a block, (m,G,G,G) float32 — the per-band
+        energy density of run_modes.py:116-124 and run_interior.py:252-262,
+        fused with the field transform (E is never stored)."""
+        G = self.grid_size
+        return self._realspace(Hs, 2, eigvals, (G, G, G), torch.float32)
 
 
 def _multi(kind, ws_op, ops, in_base, in_off, out_base, out_off, nrow, alpha=1.0, beta=0.0):

@@ -83,6 +83,16 @@ int efn_precondition_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const v
                            const int64_t* in_off, void* out_base, const int64_t* out_off, int nrow,
                            void* stream);
 
+/* Real-space fields of m vectors (post-solve; operator.py:169-184 and the
+ * energy density of run_modes.py:116-124 / run_interior.py:252-262):
+ *   mode 0: H(r) = ifftn(a·t̂₁ + b·t̂₂)                      out (m,3,G,G,G) c64
+ *   mode 1: E(r) = ε⁻¹·ifftn(−i|k+G|b·t̂₁ + i|k+G|a·t̂₂)/√λ   out (m,3,G,G,G) c64
+ *   mode 2: ε·Σ_c |E_c(r)|²                                  out (m,G,G,G) float32
+ * lam_dev: m eigenvalues (float32, device) for modes 1–2; NULL means λ = 1.
+ * Uses the context's workspace (same passes as Θ up to the x pass). */
+int efn_realspace(efn_ctx* ctx, int mode, const void* in_c64, const float* lam_dev, void* out, int m,
+                  void* stream);
+
 /* MPB transverse-projection preconditioner, operator.py:104-127 / :157-161. */
 int efn_precondition(efn_ctx* ctx, const void* in_c64, void* out_c64, int m, void* stream);
 

@@ -6,6 +6,7 @@ Restates eigenfns/operator.py:
   precondition_simple_block  operator.py:98-101
   precondition_block         operator.py:104-127 (MPB transverse-projection preconditioner)
   MaxwellOperatorNP          operator.py:130-184
+  energy_density             scripts/run_modes.py:116-122 (ε|E|² per band)
 
 Vectors are (m, 2, G, G, G) complex, C order, z fastest; frequencies in
 np.fft.fftfreq order; ``ifftn`` carries 1/G³ and ``fftn`` is unnormalised.
@@ -17,7 +18,10 @@ and the exact integer swap rule at Γ; the homogeneous known answer
 Θ·H = (|k+G|²/ε₀)·H per coefficient; the analytic homogeneous and Bloch
 spectra of the dense matrix (tests/test_operator.py:27-65 of the reference);
 Hermiticity/PSD on a disordered ε; a brute-force DFT at G=4; for the
-preconditioner the homogeneous known answer (ε₀/knr²)·H.
+preconditioner the homogeneous known answer (ε₀/knr²)·H; for the real-space
+fields the plane-wave closed forms (|H(r)|² = 1/G⁶, ε|E(r)|² = 1/G⁶ for a unit
+coefficient) and, on a disordered ε, Parseval: Σ_r|H(r)|² = ‖H‖²/G³ and
+Σ_r ε|E(r)|² = Re⟨H, ΘH⟩/(G³λ) for any H and λ.
 """
 from __future__ import annotations
 
@@ -180,3 +184,8 @@ class MaxwellOperatorNP:
         Ec = (-1j * kn * b)[:, None] * t[0][None] + (1j * kn * a)[:, None] * t[1][None]
         E = self.inv_eps * _ifftn(Ec, self.fft_dtype)
         return E / np.sqrt(np.asarray(eigvals))[:, None, None, None, None]
+
+    def energy_density(self, H, eigvals):
+        """ε·Σ_c|E_c(r)|² per band (scripts/run_modes.py:116-122)."""
+        E = self.e_realspace(H, eigvals)
+        return self.eps_grid[None] * (np.abs(E) ** 2).sum(1)

@@ -181,3 +181,42 @@ def test_ortho_and_deflate():
     D = linalg.deflate(Y, L)
     leak = linalg.gram(L, D).abs().max().item()
     assert leak < 1e-4 * float(np.abs(X).max() * np.sqrt(n))
+
+
+@pytest.mark.parametrize("G,k", [(12, (0.0, 0.0, 0.0)), (16, (0.23, -0.4, 0.1)), (32, (0.0, 0.0, 0.0)),
+                                 (48, (0.5, 0.5, 0.0))])
+def test_realspace_fields_match_fp64_oracle(G, k):
+    """H(r), E(r) and ε|E|² (operator.py:174-184, run_modes.py:116-122) from the
+    fused passes against the FP64 oracle: rel L2 < 1e-5."""
+    import torch
+    from eigenfns_amd.operator import MaxwellOperator
+    from oracle.operator_np import MaxwellOperatorNP
+    rng = np.random.default_rng(G)
+    eps = np.where(rng.random((G, G, G)) < 0.35, 8.41, 1.0).astype(np.float32)
+    op = MaxwellOperator(eps, 1.7, k)
+    ref = MaxwellOperatorNP(eps, 1.7, k)
+    X = op.random_block(3, seed=G)
+    Xn = X.cpu().numpy().astype(np.complex128)
+    lam = np.array([0.5, 2.0, 7.5])
+    for got, want in ((op.h_realspace(X), ref.h_realspace(Xn)),
+                      (op.e_realspace(X, lam), ref.e_realspace(Xn, lam)),
+                      (op.energy_density(X, lam), ref.energy_density(Xn, lam))):
+        g = got.cpu().numpy()
+        assert g.shape == want.shape
+        rel = np.linalg.norm((g - want).ravel()) / np.linalg.norm(want.ravel())
+        assert rel < 1e-5, rel
+
+
+def test_energy_density_parseval_at_scale():
+    """At 96³ on a rod network: Σ_r ε|E(r)|² = Re⟨H, ΘH⟩/(G³λ) per vector
+    (size-independent identity; both sides from the device)."""
+    from eigenfns_amd.linalg import rowdot
+    from eigenfns_amd.operator import MaxwellOperator
+    from eigenfns_amd.structure import network_eps
+    eps, L = network_eps(216, 2, 96)
+    op = MaxwellOperator(eps, L, (0.1, 0.2, 0.0))
+    X = op.random_block(4, seed=3)
+    lam = np.array([0.3, 1.0, 2.0, 4.0])
+    ed = op.energy_density(X, lam).double().sum(dim=(1, 2, 3)).cpu().numpy()
+    quad = rowdot(X, op.theta(X)).real.cpu().numpy()
+    assert np.allclose(ed, quad / (96 ** 3 * lam), rtol=2e-5), (ed, quad / (96 ** 3 * lam))

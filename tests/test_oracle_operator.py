@@ -139,3 +139,52 @@ def test_fp32_restatement_tracks_fp64():
 def test_non_cubic_eps_raises():
     with pytest.raises(ValueError):
         MaxwellOperatorNP(np.ones((4, 4, 5), np.float32), 1.0)
+
+
+# ---- real-space fields and energy density (operator.py:169-184, run_modes.py:116-122)
+
+def _plane_wave(G, L, k, idx, pol):
+    from oracle.operator_np import MaxwellOperatorNP
+    op = MaxwellOperatorNP(np.full((G, G, G), 2.5), L, k)
+    H = np.zeros((1, 2, G, G, G), np.complex128)
+    H[(0, pol) + idx] = 1.0
+    lam = float(op.kn[idx] ** 2 / 2.5)
+    return op, H, lam
+
+
+@pytest.mark.parametrize("k,idx,pol", [((0, 0, 0), (1, 2, 3), 0), ((0.2, -0.1, 0.3), (5, 0, 7), 1),
+                                       ((0, 0, 0), (0, 0, 2), 1)])
+def test_realspace_plane_wave_closed_forms(k, idx, pol):
+    """A unit transverse coefficient in homogeneous ε₀: |H(r)|² = 1/G⁶ and
+    ε₀|E(r)|² = 1/G⁶ at every voxel, E ⊥ H along the frame vectors."""
+    G = 8
+    op, H, lam = _plane_wave(G, 1.3, k, idx, pol)
+    Hr = op.h_realspace(H)[0]
+    assert np.allclose((np.abs(Hr) ** 2).sum(0), 1.0 / G ** 6, rtol=1e-12, atol=0)
+    ed = op.energy_density(H, [lam])[0]
+    assert np.allclose(ed, 1.0 / G ** 6, rtol=1e-12, atol=0)
+    E = op.e_realspace(H, [lam])[0]
+    # E·H* = 0 pointwise (transverse pair t̂₁, t̂₂ of the same wave)
+    assert np.abs((E * Hr.conj()).sum(0)).max() < 1e-12 * np.abs(E).max() * np.abs(Hr).max()
+
+
+@pytest.mark.parametrize("k", [(0.0, 0.0, 0.0), (0.31, 0.0, -0.17)])
+def test_realspace_parseval_identities(k):
+    """Disordered ε, random H, arbitrary λ: Σ_r|H(r)|² = ‖H‖²/G³ and
+    Σ_r ε|E(r)|² = Re⟨H, ΘH⟩/(G³λ)."""
+    from oracle.operator_np import MaxwellOperatorNP
+    G = 10
+    rng = np.random.default_rng(4)
+    eps = np.where(rng.random((G, G, G)) < 0.4, 8.41, 1.0)
+    op = MaxwellOperatorNP(eps, 2.0, k)
+    H = (rng.standard_normal((2, 2, G, G, G)) + 1j * rng.standard_normal((2, 2, G, G, G))) * op.mask
+    lam = np.array([0.7, 3.1])
+    Hr = op.h_realspace(H)
+    n2 = (np.abs(H) ** 2).reshape(2, -1).sum(1)
+    assert np.allclose((np.abs(Hr) ** 2).reshape(2, -1).sum(1), n2 / G ** 3, rtol=1e-12)
+    ed = op.energy_density(H, lam)
+    TH = op.theta(H)
+    quad = np.real(np.sum(H.conj() * TH, axis=(1, 2, 3, 4)))
+    assert np.allclose(ed.reshape(2, -1).sum(1), quad / (G ** 3 * lam), rtol=1e-11)
+    # a dropped ε⁻¹ or a missing 1/λ fails: the identity is specific
+    assert not np.allclose(ed.reshape(2, -1).sum(1), quad / G ** 3, rtol=1e-3)
