@@ -132,11 +132,12 @@ int run_op(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
   using namespace efn;
   constexpr int R = zrows3<G>();
   constexpr int nzt = (G + ztile_for(G) - 1) / ztile_for(G);
+  constexpr int nxt = (G + xtile_for(G) - 1) / xtile_for(G);
   constexpr int nb = G * G / R;  // zin/zout blocks per vector
   const int use_eps = rq.kind == kTheta ? 0 : 1;
   for (int v0 = 0; v0 < rq.m; v0 += c->chunk) {
     const int nv = rq.m - v0 < c->chunk ? rq.m - v0 : c->chunk;
-    const long long tiles = (long long)nv * G * nzt;  // x-pass tiles per component
+    const long long tiles = (long long)nv * G * nxt;  // x-pass tiles per component
     const dim3 gz((unsigned)(nv * nb)), gs((unsigned)(nv * 3 * G * nzt)), gx((unsigned)(3 * tiles)),
         blk(kThreads);
     if (rq.kind == kTheta)

@@ -115,6 +115,12 @@ __device__ __forceinline__ Frame point_frame(const FrameTables& ft, const RowFra
 // z-tile width of the strided (y, x) passes: 16 complex = 128 B runs.
 __host__ __device__ constexpr int ztile_for(int G) { return G >= 16 ? 16 : G; }
 
+// z-tile width of the x pass with the real-space multiply (xmid).
+#ifndef EFN_XTILE
+#define EFN_XTILE 16
+#endif
+__host__ __device__ constexpr int xtile_for(int G) { return G >= EFN_XTILE ? EFN_XTILE : (G >= 16 ? 16 : G); }
+
 // Rows per zin/zout block when all three components are in LDS at once: a
 // power of two dividing G² (a block never spans two vectors).  Measured on
 // MI355X (Θ, 32-vector blocks): 4 rows at G ≥ 128 and 8 at 48 ≤ G < 128 beat
@@ -278,7 +284,7 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
                                                         long long tiles) {
   using P = Plan<G>;
   constexpr int N1 = P::N1, N2 = P::N2;
-  constexpr int ZT = ztile_for(G);
+  constexpr int ZT = xtile_for(G);
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
   constexpr int nzt = (G + ZT - 1) / ZT;
   __shared__ float2 buf[ZT * P::kStride];

@@ -215,21 +215,38 @@ def bandpass_subspace_hosted(op, lam_lo: float, lam_hi: float, lam_max: float, m
 def two_stage(op, lam_lo: float, lam_hi: float, lam_max: float, m: int = 104,
               build_degree: int = 3000, build_outers: int = 2, polish_degree: int = 12000,
               polish_outers: int = 4, tol: float = 1e-4, chunk: int = 16, seed: int = 0,
-              verbose: bool = True):
+              verbose: bool = True, state_path=None, resume: bool = False):
     """Two-stage bandpass ChebSI protocol of scripts/run_interior.py:154-227.
 
     Build stage: `build_outers` filter+RR outers at `build_degree` from a random
     start; polish stage: up to `polish_outers` at `polish_degree` on the same
-    basis, stopping once every in-window pair has rn < tol.  Returns
-    (vals, vecs, rn, history) for the converged in-window pairs, sorted by λ."""
+    basis, stopping once every in-window pair has rn < tol.  With `state_path`
+    the basis and protocol position are checkpointed after every outer step
+    (run_interior.py:116-121); `resume=True` continues from that file, with
+    its λ_max.  Returns (vals, vecs, rn, info) for the converged in-window
+    pairs, sorted by λ."""
+    from .io import load_interior_state, save_interior_state
     history = []
     X = None
     lam = rn = None
     n_theta = 0
-    for stage, degree, outers in (("build", build_degree, build_outers),
-                                  ("polish", polish_degree, polish_outers)):
+    start_stage, outer0 = "build", 0
+    if resume and state_path is not None:
+        st = load_interior_state(state_path)
+        if st is not None:
+            X = torch.as_tensor(st["X"]).to(op._dev)
+            start_stage, outer0 = st["stage"], st["outer"] + 1
+            n_theta, lam_max, history = st["n_theta"], st["lam_max"], st["history"]
+            if verbose:
+                print(f"  resumed: stage={start_stage} next outer={outer0} n_theta={n_theta}",
+                      flush=True)
+    stages = (("build", build_degree, build_outers), ("polish", polish_degree, polish_outers))
+    for stage, degree, outers in stages:
+        if stage == "build" and start_stage == "polish":
+            continue
+        first = outer0 if stage == start_stage else 0
         coef = bandpass_coeffs(lam_lo, lam_hi, lam_max, degree)
-        for outer in range(outers):
+        for outer in range(first, outers):
             t0 = time.perf_counter()
             if X is None:
                 X = _random_start(op, m, seed)
@@ -251,10 +268,17 @@ def two_stage(op, lam_lo: float, lam_hi: float, lam_max: float, m: int = 104,
             if verbose:
                 print(f"  {stage} outer {outer}: in-window {rec['in_window']:3d} "
                       f"converged {rec['converged']:3d} [{rec['outer_seconds']:.1f}s]", flush=True)
+            if state_path is not None:
+                save_interior_state(state_path, X, stage, outer, n_theta, lam_max, history)
             if stage == "polish" and inwin.any() and conv.sum() == inwin.sum():
                 break
+    if X is None:
+        X = _random_start(op, m, seed)
+    if lam is None:  # resumed past the last outer step: extract from the saved basis
+        lam, X, rn = rr_extract(op, X)
+        n_theta += 2 * m
     inwin = np.isfinite(lam) & (lam >= lam_lo) & (lam <= lam_hi)
     conv = inwin & (rn < tol)
     idx = np.where(conv)[0][np.argsort(lam[conv], kind="stable")]
     return lam[idx], X[torch.as_tensor(idx, device=X.device)], rn[idx], {
-        "history": history, "theta_applications": n_theta}
+        "history": history, "theta_applications": n_theta, "lam_max": lam_max}
