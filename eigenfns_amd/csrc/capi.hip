@@ -14,6 +14,7 @@
 #include "operator_kernels.hpp"
 #include "raster_kernels.hpp"
 
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -76,6 +77,7 @@ struct efn_ctx {
   double2* d_red = nullptr;      // chunk·2 doubles of reduced sums
   efn::OpDesc* d_desc = nullptr; // this operator's descriptor for the pass kernels
   size_t ws_bytes = 0;
+
   int (*run)(efn_ctx*, const OpRequest&, hipStream_t) = nullptr;
   int (*real)(efn_ctx*, const OpRequest&, int, const float*, void*, hipStream_t) = nullptr;
 
@@ -137,42 +139,44 @@ int run_op(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
   const int use_eps = rq.kind == kTheta ? 0 : 1;
   for (int v0 = 0; v0 < rq.m; v0 += c->chunk) {
     const int nv = rq.m - v0 < c->chunk ? rq.m - v0 : c->chunk;
+    float2* const ws = c->d_ws;
+    double* const partial = c->d_partial;
     const long long tiles = (long long)nv * G * nxt;  // x-pass tiles per component
     const dim3 gz((unsigned)(nv * nb)), gs((unsigned)(nv * 3 * G * nzt)), gx((unsigned)(3 * tiles)),
         blk(kThreads);
     if (rq.kind == kTheta)
-      hipLaunchKernelGGL((zin_kernel<G, kTheta>), gz, blk, 0, s, rq.in, c->d_ws, rq.vm, (long long)v0, c->d_twid);
+      hipLaunchKernelGGL((zin_kernel<G, kTheta>), gz, blk, 0, s, rq.in, ws, rq.vm, (long long)v0, c->d_twid);
     else
-      hipLaunchKernelGGL((zin_kernel<G, kPrecond>), gz, blk, 0, s, rq.in, c->d_ws, rq.vm, (long long)v0, c->d_twid);
+      hipLaunchKernelGGL((zin_kernel<G, kPrecond>), gz, blk, 0, s, rq.in, ws, rq.vm, (long long)v0, c->d_twid);
     EFN_LAUNCH_CHECK();
-    hipLaunchKernelGGL((strided_kernel<G, 1, +1>), gs, blk, 0, s, c->d_ws, c->d_twid);
-    hipLaunchKernelGGL((xmid_kernel<G>), gx, blk, 0, s, c->d_ws, rq.vm, (long long)v0, use_eps, c->d_twid, tiles);
-    hipLaunchKernelGGL((strided_kernel<G, 1, -1>), gs, blk, 0, s, c->d_ws, c->d_twid);
+    hipLaunchKernelGGL((strided_kernel<G, 1, +1>), gs, blk, 0, s, ws, c->d_twid);
+    hipLaunchKernelGGL((xmid_kernel<G>), gx, blk, 0, s, ws, rq.vm, (long long)v0, use_eps, c->d_twid, tiles);
+    hipLaunchKernelGGL((strided_kernel<G, 1, -1>), gs, blk, 0, s, ws, c->d_twid);
     EFN_LAUNCH_CHECK();
     EpiArgs ea = rq.ea;
     ea.x = rq.in;
-    ea.partial = c->d_partial;
+    ea.partial = partial;
     if (rq.kind == kPrecond) {
-      hipLaunchKernelGGL((zout_kernel<G, kPrecond, kStore>), gz, blk, 0, s, c->d_ws, rq.vm, (long long)v0, c->d_twid, ea);
+      hipLaunchKernelGGL((zout_kernel<G, kPrecond, kStore>), gz, blk, 0, s, ws, rq.vm, (long long)v0, c->d_twid, ea);
     } else {
       switch (rq.epi) {
         case kStore:
-          hipLaunchKernelGGL((zout_kernel<G, kTheta, kStore>), gz, blk, 0, s, c->d_ws, rq.vm, (long long)v0, c->d_twid, ea);
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kStore>), gz, blk, 0, s, ws, rq.vm, (long long)v0, c->d_twid, ea);
           break;
         case kCheb:
-          hipLaunchKernelGGL((zout_kernel<G, kTheta, kCheb>), gz, blk, 0, s, c->d_ws, rq.vm, (long long)v0, c->d_twid, ea);
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kCheb>), gz, blk, 0, s, ws, rq.vm, (long long)v0, c->d_twid, ea);
           break;
         case kKpm:
-          hipLaunchKernelGGL((zout_kernel<G, kTheta, kKpm>), gz, blk, 0, s, c->d_ws, rq.vm, (long long)v0, c->d_twid, ea);
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kKpm>), gz, blk, 0, s, ws, rq.vm, (long long)v0, c->d_twid, ea);
           break;
         default:
-          hipLaunchKernelGGL((zout_kernel<G, kTheta, kDot>), gz, blk, 0, s, c->d_ws, rq.vm, (long long)v0, c->d_twid, ea);
+          hipLaunchKernelGGL((zout_kernel<G, kTheta, kDot>), gz, blk, 0, s, ws, rq.vm, (long long)v0, c->d_twid, ea);
           break;
       }
     }
     EFN_LAUNCH_CHECK();
     if (rq.reduce_mode) {
-      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nv), dim3(256), 0, s, c->d_partial, nb, nv,
+      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nv), dim3(256), 0, s, partial, nb, nv,
                          rq.reduce_mode, rq.mom ? rq.mom + (long long)v0 * rq.ld : nullptr, rq.ld,
                          rq.k, rq.est ? rq.est + v0 : nullptr, rq.ck64);
       EFN_LAUNCH_CHECK();

@@ -119,6 +119,7 @@ __host__ __device__ constexpr int ztile_for(int G) { return G >= 16 ? 16 : G; }
 #ifndef EFN_XTILE
 #define EFN_XTILE 16
 #endif
+
 __host__ __device__ constexpr int xtile_for(int G) { return G >= EFN_XTILE ? EFN_XTILE : (G >= 16 ? 16 : G); }
 
 // Rows per zin/zout block when all three components are in LDS at once: a
@@ -287,7 +288,11 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   constexpr int ZT = xtile_for(G);
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
   constexpr int nzt = (G + ZT - 1) / ZT;
-  __shared__ float2 buf[ZT * P::kStride];
+  // forward-transform rows (register-resident variant): (N1+1)·N2 slots, odd stride
+  constexpr int kFStrideRaw = (N1 + 1) * N2;
+  constexpr int kFStride = (kFStrideRaw % 2 == 0) ? kFStrideRaw + 1 : kFStrideRaw;
+  constexpr int kBufStride = kFStride > P::kStride ? kFStride : P::kStride;
+  __shared__ float2 buf[ZT * kBufStride];
   __shared__ float2 twid[G];
   for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
   __syncthreads();
@@ -338,36 +343,34 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
     }
   }
   __syncthreads();
+  // Forward x-FFT straight from registers: a task holds x = k1 + N1·k2
+  // (k2 = 0..N2−1), the input of the transposed four-step x = n2' + N1·n1'
+  // (n2' = k1, n1' = k2): DFT-N2 over k2 → f1', twiddle ω^{k1·f1'}, then
+  // DFT-N1 over k1 per f1' → f = f1' + N2·f2'.  LDS rows hold element
+  // (n2', f1') at n2' + (N1+1)·f1' (odd stride for the strided writes).
+  float2* fbuf = buf;  // reused: every inverse step-3 read has completed (barrier above)
 #pragma unroll
   for (int j = 0; j < TPT; ++j) {
     const int t = threadIdx.x + j * kThreads;
     const int z = t % ZT, k1 = t / ZT;
     if (t < T3 && z0 + z < G) {
+      DFT<N2, -1>::run(r[j]);
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) buf[z * P::kStride + P::pos(k1 + N1 * k2)] = r[j][k2];
+      for (int f1 = 1; f1 < N2; ++f1) r[j][f1] = cmul(r[j][f1], twid[k1 * f1]);
+#pragma unroll
+      for (int f1 = 0; f1 < N2; ++f1) fbuf[z * kFStride + k1 + (N1 + 1) * f1] = r[j][f1];
     }
   }
   __syncthreads();
-  // forward step 1 in LDS (each task rewrites its own slots)
   for (int t = threadIdx.x; t < ZT * N2; t += kThreads) {
-    const int z = t % ZT, n2 = t / ZT;
+    const int z = t % ZT, f1 = t / ZT;
     if (z0 + z < G) {
-      float2* pen = buf + z * P::kStride;
-      float2 vv[N1];
+      float2 o2[N1];
 #pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) vv[n1] = pen[P::pos(N2 * n1 + n2)];
-      step1_store<G, -1>(vv, n2, pen, twid);
-    }
-  }
-  __syncthreads();
-  // forward step 3: LDS → registers → HBM
-  for (int t = threadIdx.x; t < T3; t += kThreads) {
-    const int z = t % ZT, k1 = t / ZT;
-    if (z0 + z < G) {
-      float2 o2[N2];
-      step3_load<G, -1>(o2, k1, buf + z * P::kStride);
+      for (int n2 = 0; n2 < N1; ++n2) o2[n2] = fbuf[z * kFStride + n2 + (N1 + 1) * f1];
+      DFT<N1, -1>::run(o2);
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) base[(k1 + N1 * k2) * G2 + z] = o2[k2];
+      for (int f2 = 0; f2 < N1; ++f2) base[(long long)(f1 + N2 * f2) * G2 + z] = o2[f2];
     }
   }
 }

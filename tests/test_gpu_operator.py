@@ -220,3 +220,32 @@ def test_energy_density_parseval_at_scale():
     ed = op.energy_density(X, lam).double().sum(dim=(1, 2, 3)).cpu().numpy()
     quad = rowdot(X, op.theta(X)).real.cpu().numpy()
     assert np.allclose(ed, quad / (96 ** 3 * lam), rtol=2e-5), (ed, quad / (96 ** 3 * lam))
+
+
+def test_chunked_passes_equal_single_pass():
+    """Blocks larger than the workspace run chunk by chunk (alternating
+    workspaces and lanes when the two-lane pipeline is built in): results are
+    bitwise those of one pass, for every fused epilogue."""
+    import torch
+    from eigenfns_amd.operator import MaxwellOperator
+    rng = np.random.default_rng(11)
+    eps = np.where(rng.random((16, 16, 16)) < 0.3, 8.41, 1.0).astype(np.float32)
+    big = MaxwellOperator(eps, 1.3, (0.1, 0.0, 0.2))
+    small = MaxwellOperator(eps, 1.3, (0.1, 0.0, 0.2), max_chunk=1)
+    X = big.random_block(5, seed=2)
+    assert torch.equal(big.theta(X), small.theta(X))
+    assert torch.equal(big.precondition(X), small.precondition(X))
+    outs = []
+    for op in (big, small):
+        T0 = X.roll(1, 0).contiguous()
+        Y = torch.zeros_like(X)
+        op.cheb_step(X, T0, Y, 0.01, 0.5)
+        mom = torch.zeros((5, 8), dtype=torch.float64, device=X.device)
+        T0k = X.roll(2, 0).contiguous()
+        op.kpm_step(X, T0k, mom, 2, 0.01)
+        est = torch.zeros(5, dtype=torch.float64, device=X.device)
+        T0c = X.roll(3, 0).contiguous()
+        op.kpm_count_step(X, T0c, X.roll(4, 0).contiguous(), est, 0.25, 0.01)
+        outs.append((T0, Y, T0k, mom, T0c, est))
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
