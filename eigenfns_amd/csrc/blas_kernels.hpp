@@ -384,6 +384,82 @@ __global__ __launch_bounds__(256) void combine_kernel(const float2* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// combine on the matrix cores (v_mfma_f32_16x16x4f32): out = Cᵀ·S (+ alpha·X)
+// as real products of the complex operands, FP32 accumulation as in the
+// vector-ALU kernel:
+//   Re += Cr·Sr − Ci·Si,  Im += Cr·Si + Ci·Sr      (A = Cᵀ 16 rows × 4 j, B = S 4 j × 16 columns)
+// A wave owns 16·NA output rows × 64 columns (NA × 4 tiles of 16×16); a
+// block's four waves take consecutive 64-column ranges.  B operands: lane l
+// reads S[j0 + l/16][c0 + l%16] (16 lanes = 128 B per row); D[i][c] sits in
+// lane c + 16·(i/4), element i%4, so stores are 128-B runs as well.
+template <int NA>
+__global__ __launch_bounds__(256) void combine_mfma_kernel(const float2* __restrict__ Call, int m_in,
+                                                           int m_out, CombinePtrs cp, long long n,
+                                                           float alpha) {
+  const float2* __restrict__ C = Call + (long long)blockIdx.z * m_in * m_out;
+  const float2* __restrict__ S = cp.s[blockIdx.z];
+  const float2* __restrict__ X = cp.x[blockIdx.z];
+  float2* __restrict__ out = cp.out[blockIdx.z];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const long long c0 = ((long long)blockIdx.x * 4 + wave) * 64;
+  const int i0 = blockIdx.y * 16 * NA;
+  floatx4 re[NA][4], im[NA][4];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      re[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+      im[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+  for (int j0 = 0; j0 < m_in; j0 += 4) {
+    const int j = j0 + g;
+    float cr[NA], ci[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      const int i = i0 + 16 * a + r;
+      const float2 cv = (j < m_in && i < m_out) ? C[(long long)j * m_out + i] : make_float2(0.f, 0.f);
+      cr[a] = cv.x;
+      ci[a] = cv.y;
+    }
+    float2 sv[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const long long col = c0 + 16 * b + r;
+      sv[b] = (j < m_in && col < n) ? S[(long long)j * n + col] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        re[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(cr[a], sv[b].x, re[a][b], 0, 0, 0);
+        re[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ci[a], sv[b].y, re[a][b], 0, 0, 0);
+        im[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(cr[a], sv[b].y, im[a][b], 0, 0, 0);
+        im[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ci[a], sv[b].x, im[a][b], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const long long col = c0 + 16 * b + r;
+      if (col >= n) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + 16 * a + 4 * g + q;
+        if (i >= m_out) continue;
+        float2 v = make_float2(re[a][b][q], im[a][b][q]);
+        if (X != nullptr) {
+          const float2 xv = X[(long long)i * n + col];
+          v.x = fmaf(alpha, xv.x, v.x);
+          v.y = fmaf(alpha, xv.y, v.y);
+        }
+        out[(long long)i * n + col] = v;
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Row reductions in FP64 over splits: part[i][s] = Σ_k f(A[i][k], B[i][k])
 // op 0: |A|²  (re only)   op 1: conj(A)·B (re, im)
 __device__ __forceinline__ double2 wave_sum2(double2 v) {

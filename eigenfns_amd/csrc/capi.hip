@@ -271,6 +271,7 @@ int gram_nsplit(int, int, long long n) {
 }
 
 std::atomic<int> g_gram_engine{EFN_GRAM_MFMA_F64};
+std::atomic<int> g_combine_mfma{1};
 
 int rowreduce_nsplit(long long n) {
   long long ns = (n + 65535) / 65536;
@@ -693,6 +694,12 @@ int efn_precondition_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const v
   return ws->run(ws, rq, (hipStream_t)stream);
 }
 
+int efn_combine_engine(int engine) {
+  if (engine < 0) return g_combine_mfma.load();
+  if (engine > 1) return fail(EFN_EINVAL, "unknown combine engine");
+  return g_combine_mfma.exchange(engine);
+}
+
 int efn_gram_engine(int engine) {
   if (engine < 0) return g_gram_engine.load();
   if (engine > EFN_GRAM_MFMA_F32) return fail(EFN_EINVAL, "unknown Gram engine");
@@ -797,6 +804,18 @@ int efn_combine_multi(int nprob, const void* C, const void* const* S, int m_in, 
   }
   hipStream_t s = (hipStream_t)stream;
   const unsigned gx = (unsigned)((n + 255) / 256);
+  // matrix cores: 64 columns per wave, 16·NA output rows per block row
+  if (g_combine_mfma.load() && m_out > 8) {
+    const unsigned gxm = (unsigned)((n + 255) / 256);
+    if (m_out <= 16)
+      hipLaunchKernelGGL((efn::combine_mfma_kernel<1>), dim3(gxm, 1, nprob), dim3(256), 0, s,
+                         (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
+    else
+      hipLaunchKernelGGL((efn::combine_mfma_kernel<2>), dim3(gxm, (m_out + 31) / 32, nprob), dim3(256),
+                         0, s, (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
+    EFN_LAUNCH_CHECK();
+    return EFN_OK;
+  }
   // one row tile per 32 outputs: S is read ceil(m_out / 32) times
   if (m_out <= 8) {
     hipLaunchKernelGGL((efn::combine_kernel<8>), dim3(gx, 1, nprob), dim3(256), 0, s,

@@ -50,6 +50,18 @@ def gram_engine(name: str | None = None) -> str:
     return inv[rc]
 
 
+def combine_engine(name: str | None = None) -> str:
+    """Select the process-wide combine kernel ("mfma": FP32 matrix cores, the
+    default for more than 8 output rows; "valu": vector ALU); returns the
+    previous one."""
+    names = {"valu": 0, "mfma": 1}
+    inv = {v: k for k, v in names.items()}
+    rc = _lib.load().efn_combine_engine(-1 if name is None else names[name])
+    if rc < 0:
+        _lib.check(rc, "combine_engine")
+    return inv[rc]
+
+
 def gram(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
     """⟨A_i, B_j⟩ as an (m_a, m_b) complex128 device tensor (solver.py:35-43).
 

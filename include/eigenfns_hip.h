@@ -168,6 +168,12 @@ int efn_gram_multi_herm(int nprob, const void* const* A_c64, const void* const* 
 int efn_combine(const void* C_c64, int m_in, int m_out, const void* S_c64, int64_t n,
                 const void* X_c64, float alpha, void* out_c64, void* stream);
 
+/* Process-wide combine engine: 1 = matrix cores (v_mfma_f32_16x16x4f32,
+ * default, for m_out > 8), 0 = vector ALU; FP32 accumulation either way.
+ * Returns the previous engine (engine < 0: query only).  No reference
+ * counterpart. */
+int efn_combine_engine(int engine);
+
 /* combine for `nprob` (≤ 32) problems in one launch: C is nprob×m_in×m_out
  * complex64 on the device; S, X (optional, NULL = none) and out are host
  * arrays of per-problem device pointers.  solver.py:49-57 per problem. */

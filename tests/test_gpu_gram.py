@@ -121,3 +121,34 @@ def test_gram_hermitian_upper_tiles(engine, engine_guard):
         fA = gram(A, A).cpu().numpy()
         tol = 2e-6 if engine == "mfma_f32" else 1e-12   # bars of the full-product test
         assert np.abs(hA - fA).max() < tol * np.abs(fA).max()
+
+
+@pytest.mark.parametrize("engine", ["valu", "mfma"])
+def test_combine_engines_match_complex128(engine):
+    """out = alpha·X + Cᵀ·S on both engines: ragged sizes (m_in not a multiple
+    of 4, m_out across the 8/16/32 tile edges, n not a multiple of 256), X
+    aliasing out, and the multi-problem form; FP32 accumulation bar 2e-5."""
+    import torch
+    from eigenfns_amd.linalg import combine_engine, combine_into, combine_multi
+    prev = combine_engine(engine)
+    try:
+        rng = np.random.default_rng(17)
+        for m_in, m_out, n in [(3, 5, 700), (96, 12, 5000), (97, 32, 8193), (40, 40, 4096),
+                               (96, 96, 2 * 16 ** 3)]:
+            S = _cblock(rng, m_in, n)
+            Cm = (rng.standard_normal((m_in, m_out)) + 1j * rng.standard_normal((m_in, m_out))).astype(np.complex64)
+            X = _cblock(rng, m_out, n)
+            ref = 0.5 * X.astype(np.complex128) + Cm.astype(np.complex128).T @ S.astype(np.complex128)
+            scale = np.abs(ref).max()
+            Sd, Cd = torch.from_numpy(S).cuda(), torch.from_numpy(Cm).cuda()
+            out = torch.from_numpy(X).cuda()
+            combine_into(Cd, Sd, out, X=out, alpha=0.5)          # X aliases out
+            assert np.abs(out.cpu().numpy() - ref).max() < 2e-5 * scale, (m_in, m_out, n)
+            outs = [torch.empty((m_out, n), dtype=torch.complex64, device="cuda") for _ in range(2)]
+            combine_multi(torch.stack([Cd, Cd]), [Sd, Sd], outs)
+            plain = Cm.astype(np.complex128).T @ S.astype(np.complex128)
+            for o in outs:
+                assert np.abs(o.cpu().numpy() - plain).max() < 2e-5 * np.abs(plain).max()
+            assert torch.equal(outs[0], outs[1])
+    finally:
+        combine_engine(prev)
