@@ -140,6 +140,10 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 constexpr int kMWaveTile = 32;
 constexpr int kMFlush = 4;
 constexpr int kMStep = 8;  // complex entries per wave step
+#ifndef EFN_GRAM_PREFETCH
+#define EFN_GRAM_PREFETCH 3
+#endif
+constexpr int kMPrefetch = EFN_GRAM_PREFETCH;  // steps in flight per wave
 
 struct GramF64 {
   typedef double in_t;
@@ -210,19 +214,9 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
     }
   int steps = 0;
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  // register double buffer: the next step's loads are in flight during this step's MFMAs
-  float4 an[2], bn[2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a) an[a] = ra[a] ? load_c2(ap[a], kb + 2 * g, ke) : z4;
-#pragma unroll
-  for (int b = 0; b < 2; ++b) bn[b] = rb[b] ? load_c2(bp[b], kb + 2 * g, ke) : z4;
-  for (long long k = kb; k < ke; k += kMStep) {
-    float4 av[2] = {an[0], an[1]}, bv[2] = {bn[0], bn[1]};
-    const long long kn = k + kMStep + 2 * g;
-#pragma unroll
-    for (int a = 0; a < 2; ++a) an[a] = ra[a] ? load_c2(ap[a], kn, ke) : z4;
-#pragma unroll
-    for (int b = 0; b < 2; ++b) bn[b] = rb[b] ? load_c2(bp[b], kn, ke) : z4;
+  // register ring of kMPrefetch steps: each step's loads are issued
+  // kMPrefetch − 1 steps (of 32 MFMAs each) before they are consumed
+  auto step = [&](const float4 (&av)[2], const float4 (&bv)[2]) {
     T x[2][4], y[2][4], w[2][4];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
@@ -255,6 +249,31 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
             cre[a][b][q] = 0;
             cim[a][b][q] = 0;
           }
+    }
+  };
+  constexpr int D = kMPrefetch;
+  float4 ra_buf[D][2], rb_buf[D][2];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const long long kk = kb + d * kMStep + 2 * g;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) ra_buf[d][a] = ra[a] ? load_c2(ap[a], kk, ke) : z4;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) rb_buf[d][b] = rb[b] ? load_c2(bp[b], kk, ke) : z4;
+  }
+  for (long long k = kb; k < ke; k += D * kMStep) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (k + d * kMStep < ke) {  // wave-uniform
+        const float4 av[2] = {ra_buf[d][0], ra_buf[d][1]};
+        const float4 bv[2] = {rb_buf[d][0], rb_buf[d][1]};
+        const long long kn = k + (d + D) * kMStep + 2 * g;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) ra_buf[d][a] = ra[a] ? load_c2(ap[a], kn, ke) : z4;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) rb_buf[d][b] = rb[b] ? load_c2(bp[b], kn, ke) : z4;
+        step(av, bv);
+      }
     }
   }
   // this wave's tile: element (P::row(g, q), r) of sub-tile (a, b)
