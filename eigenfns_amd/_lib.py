@@ -58,6 +58,10 @@ _SIGNATURES = {
     "efn_rowreduce": (_i, [_i, _vp, _vp, _i, _i64, _vp, _vp, _sz, _vp]),
     "efn_row_axpby": (_i, [_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp]),
     "efn_rowreduce_strided": (_i, [_i, _vp, _i64, _vp, _i64, _i, _i, _i64, _vp, _vp, _sz, _vp]),
+    "efn_rowstats_work_bytes": (_sz, [_i, _i64]),
+    "efn_rowstats_strided": (_i, [_vp, _i64, _vp, _i64, _i, _i, _i64, _vp, _vp, _sz, _vp]),
+    "efn_row_axpby_norm_strided": (_i, [_vp, _i64, _vp, _vp, _i64, _vp, _i, _i, _i64, _vp, _i64, _vp,
+                                        _vp, _sz, _vp]),
     "efn_row_axpby_strided": (_i, [_vp, _i64, _vp, _vp, _i64, _vp, _i, _i, _i64, _vp, _i64, _vp]),
     "efn_axpby": (_i, [_vp, _f, _vp, _f, _i64, _vp, _vp]),
     "efn_copy_multi": (_i, [_i, _vp, _vp, _i64, _vp]),

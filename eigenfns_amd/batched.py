@@ -32,7 +32,8 @@ import numpy as np
 import torch
 
 from .linalg import (RR_DEAD_PENALTY, SVQB_DROP, combine_multi, copy_multi, eigh_batched_host,
-                     gram_multi, rowreduce_strided, row_axpby_strided, MAX_PROBLEMS)
+                     gram_multi, rowreduce_strided, row_axpby_strided, row_axpby_norm_strided,
+                     rowstats_strided, MAX_PROBLEMS)
 from .operator import MaxwellOperator, precondition_multi, theta_multi
 
 
@@ -83,7 +84,8 @@ class BatchedLOBPCG:
 
     def __init__(self, grid_size: int, m: int = 24, guard: int = 8, nslots: int = 8,
                  max_nev: int = 64, tol: float = 1e-4, maxit: int = 400,
-                 keep_vectors: bool = False, normalized_rq: bool = True):
+                 keep_vectors: bool = False, normalized_rq: bool = True,
+                 verify_dead_rows: bool = False):
         if not 0 < guard < m:
             raise ValueError("need 0 < guard < m")
         if not 1 <= nslots <= MAX_PROBLEMS:
@@ -96,6 +98,7 @@ class BatchedLOBPCG:
         self.maxit = int(maxit)
         self.keep_vectors = keep_vectors
         self.normalized_rq = normalized_rq
+        self.verify_dead_rows = verify_dead_rows  # also measure the row norms and compare (tests)
         per = self.m - self.guard
         self.cap = int(np.ceil(max_nev / per)) * per + self.m
         self.max_nev = int(max_nev)
@@ -183,7 +186,10 @@ class BatchedLOBPCG:
         combine_multi(-C, Ls, Ts, Xs=Ts, alpha=1.0)
 
     def _svqb(self, idx, block, passes=2):
-        """SVQB of block(s) for slots idx, two sweeps, written back in place."""
+        """SVQB of block(s) for slots idx, two sweeps, written back in place.
+        Returns the last sweep's keep mask (len(idx), rows): a dropped direction
+        is an exactly zero row of the result."""
+        keep = None
         for _ in range(passes):
             Bs = [block(s) for s in idx]
             w, V = eigh_batched_host(gram_multi(Bs, Bs, hermitian=True))
@@ -194,6 +200,7 @@ class BatchedLOBPCG:
             Ts = [self._rows(self.T1, s, self.m) for s in idx]
             combine_multi(Cw, Bs, Ts)
             copy_multi(Ts, Bs)
+        return keep
 
     def _theta(self, idx, src, row0_src, dst, row0_dst, stride_src, stride_dst, nrow):
         for c in range(0, len(idx), MAX_PROBLEMS):
@@ -242,7 +249,7 @@ class BatchedLOBPCG:
             Ws = [self._W(s) for s in idx]
             combine_multi(-gram_multi(Xs, Ws), Xs, Ws, Xs=Ws, alpha=1.0)
         t = self._tick("precond+deflate+proj", t)
-        self._svqb(idx, self._W)
+        keep_w = self._svqb(idx, self._W)
         t = self._tick("svqb_W", t)
         # P projected out of [X, W] twice, fixed-shape whitening (solver.py:258-274)
         for _ in range(2):
@@ -263,11 +270,20 @@ class BatchedLOBPCG:
         t = self._tick("theta_WP", t)
         Sbs = [self._Sb(s) for s in idx]
         A = gram_multi(Sbs, [self._HSb(s) for s in idx], hermitian=True).cpu().numpy()
-        rn2 = rowreduce_strided(0, self.S, None, 3 * m, 3 * m, 3 * m, self.nslots * 3 * m,
-                                self.n).real.cpu().numpy().reshape(self.nslots, 3 * m)
+        # dead rows (solver.py:296-297, norm < 0.5) without another pass over S:
+        # X norms come from this step's residual statistics; W and P rows are
+        # unit or exactly zero after SVQB / whitening (their keep masks)
+        dead_rows = {s: ~np.concatenate([self._xn[s] >= 0.5, keep_w[j], keepp[j]])
+                     for j, s in enumerate(idx)}
+        if self.verify_dead_rows:
+            rn2 = rowreduce_strided(0, self.S, None, 3 * m, 3 * m, 3 * m, self.nslots * 3 * m,
+                                    self.n).real.cpu().numpy().reshape(self.nslots, 3 * m)
+            for s in idx:
+                if not np.array_equal(dead_rows[s], np.sqrt(rn2[s]) < 0.5):
+                    raise AssertionError(f"slot {s}: dead-row bookkeeping disagrees with row norms")
         for j, s in enumerate(idx):
             A[j] = (A[j] + A[j].conj().T) / 2
-            dead = np.sqrt(rn2[s]) < 0.5
+            dead = dead_rows[s]
             penalty = max(RR_DEAD_PENALTY, 100.0 * float(np.abs(np.diag(A[j])).max()))
             A[j][np.diag_indices(3 * m)] += np.where(dead, penalty, 0.0)
         from .linalg import eigh_stack_host
@@ -351,18 +367,18 @@ class BatchedLOBPCG:
             t = self._tick("init", t)
         # Rayleigh quotients, residuals, dead rows for the X rows of every slot
         M = self.nslots * m
-        lam = rowreduce_strided(1, self.S, self.HS, m, 3 * m, 3 * m, M, self.n).real
-        xn2 = rowreduce_strided(0, self.S, None, m, 3 * m, 3 * m, M, self.n).real
+        # one read of X and ΘX for ⟨x,Θx⟩, |x|², |Θx|²; R = ΘX − λX with |R|² in the same pass
+        st = rowstats_strided(self.S, self.HS, m, 3 * m, 3 * m, M, self.n)
+        lam, xn2, hx2 = st[:, 0], st[:, 2], st[:, 3]
         if self.normalized_rq:
             lam = lam / torch.clamp(xn2, min=1e-30)
-        row_axpby_strided(self.HS, 3 * m, torch.ones_like(lam), self.S, 3 * m, -lam, m, M,
-                          self.n, self.R, m)
-        rr2 = rowreduce_strided(0, self.R, None, m, m, m, M, self.n).real
-        hx2 = rowreduce_strided(0, self.HS, None, m, 3 * m, 3 * m, M, self.n).real
+        rr2 = row_axpby_norm_strided(self.HS, 3 * m, torch.ones_like(lam), self.S, 3 * m, -lam, m,
+                                     M, self.n, self.R, m)
         stats = torch.stack([lam, xn2, rr2, hx2]).cpu().numpy()
         t = self._tick("residuals+sync", t)
         lam_h = stats[0].reshape(self.nslots, m)
         xn = np.sqrt(stats[1]).reshape(self.nslots, m)
+        self._xn = xn
         rn_h = (np.sqrt(stats[2]) / np.maximum(np.sqrt(stats[3]), 1e-30)).reshape(self.nslots, m)
         cont = []
         for s in act:
@@ -384,10 +400,11 @@ class BatchedLOBPCG:
 
 
 def solve_batched(requests, grid_size, m=24, guard=8, nslots=8, max_nev=None, tol=1e-4,
-                  maxit=400, keep_vectors=False):
+                  maxit=400, keep_vectors=False, verify_dead_rows=False):
     """Solve a list of same-grid requests through one engine (admission as slots free)."""
     max_nev = max_nev or max(r.nev for r in requests)
-    eng = BatchedLOBPCG(grid_size, m, guard, nslots, max_nev, tol, maxit, keep_vectors)
+    eng = BatchedLOBPCG(grid_size, m, guard, nslots, max_nev, tol, maxit, keep_vectors,
+                        verify_dead_rows=verify_dead_rows)
     pending = list(requests)
     done = []
     while pending or eng.active():

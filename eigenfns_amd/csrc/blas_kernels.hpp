@@ -540,6 +540,100 @@ __global__ void rowreduce_final_kernel(const double2* __restrict__ part, int nsp
   out[i] = make_double2(re, im);
 }
 
+// Fused row statistics of a pair of strided row sets in one read of each:
+// part[i][s] = (Re⟨A,B⟩, Im⟨A,B⟩, |A|², |B|²) over split s, FP64 sums of exact
+// FP32 products (same split layout as rowreduce_partial_kernel).
+__global__ __launch_bounds__(256) void rowstats_partial_kernel(const float2* __restrict__ A, long long sa,
+                                                               const float2* __restrict__ B, long long sb,
+                                                               int nrow, long long n, long long kper,
+                                                               double2* __restrict__ part) {
+  __shared__ double2 red[8];
+  const int i = blockIdx.y, s = blockIdx.x, nsplit = gridDim.x;
+  const long long kb = (long long)s * kper;
+  const long long ke = kb + kper < n ? kb + kper : n;
+  const float2* a = A + phys_row(i, nrow, sa) * n;
+  const float2* b = B + phys_row(i, nrow, sb) * n;
+  double dre = 0.0, dim_ = 0.0, aa = 0.0, bb = 0.0;
+  for (long long k = kb + threadIdx.x; k < ke; k += 256) {
+    const float2 x = a[k], y = b[k];
+    dre += (double)x.x * y.x + (double)x.y * y.y;
+    dim_ += (double)x.x * y.y - (double)x.y * y.x;
+    aa += (double)x.x * x.x + (double)x.y * x.y;
+    bb += (double)y.x * y.x + (double)y.y * y.y;
+  }
+  const double2 v0 = wave_sum2(make_double2(dre, dim_)), v1 = wave_sum2(make_double2(aa, bb));
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = v0;
+    red[4 + (threadIdx.x >> 6)] = v1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 t0 = red[0], t1 = red[4];
+    for (int w = 1; w < 4; ++w) {
+      t0.x += red[w].x; t0.y += red[w].y;
+      t1.x += red[4 + w].x; t1.y += red[4 + w].y;
+    }
+    part[((long long)i * nsplit + s) * 2] = t0;
+    part[((long long)i * nsplit + s) * 2 + 1] = t1;
+  }
+}
+
+// out[i] = the four sums of row i over the splits, fixed order
+__global__ void rowstats_final_kernel(const double2* __restrict__ part, int nsplit, int m,
+                                      double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < nsplit; ++s) {
+    const double2 p0 = part[((long long)i * nsplit + s) * 2], p1 = part[((long long)i * nsplit + s) * 2 + 1];
+    v[0] += p0.x;
+    v[1] += p0.y;
+    v[2] += p1.x;
+    v[3] += p1.y;
+  }
+  for (int c = 0; c < 4; ++c) out[(long long)i * 4 + c] = v[c];
+}
+
+// out[i] = a[i]·X[i] + b[i]·Y[i] and per-split partials of |out[i]|² (FP64 of
+// the stored FP32 values), split layout as rowreduce_partial_kernel.
+__global__ __launch_bounds__(256) void row_axpby_norm_kernel(const float2* __restrict__ X, long long sx,
+                                                             const double* __restrict__ a,
+                                                             const float2* __restrict__ Y, long long sy,
+                                                             const double* __restrict__ b, int nrow,
+                                                             long long n, long long kper,
+                                                             float2* __restrict__ out, long long so,
+                                                             double2* __restrict__ part) {
+  __shared__ double2 red[4];
+  const int i = blockIdx.y, s = blockIdx.x, nsplit = gridDim.x;
+  const long long kb = (long long)s * kper;
+  const long long ke = kb + kper < n ? kb + kper : n;
+  const float ai = (float)a[i];
+  const float bi = Y ? (float)b[i] : 0.f;
+  const float2* xr = X + phys_row(i, nrow, sx) * n;
+  const float2* yr = Y ? Y + phys_row(i, nrow, sy) * n : nullptr;
+  float2* orow = out + phys_row(i, nrow, so) * n;
+  double acc = 0.0;
+  for (long long k = kb + threadIdx.x; k < ke; k += 256) {
+    const float2 x = xr[k];
+    float2 r = make_float2(ai * x.x, ai * x.y);
+    if (yr) {
+      const float2 y = yr[k];
+      r.x = fmaf(bi, y.x, r.x);
+      r.y = fmaf(bi, y.y, r.y);
+    }
+    orow[k] = r;
+    acc += (double)r.x * r.x + (double)r.y * r.y;
+  }
+  const double2 v = wave_sum2(make_double2(acc, 0.0));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0].x;
+    for (int w = 1; w < 4; ++w) t += red[w].x;
+    part[(long long)i * nsplit + s] = make_double2(t, 0.0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // out[i][k] = a[i]·X[i][k] + b[i]·Y[i][k] (a, b real per row, FP64 on device,
 // applied in FP32).  Y may be null.

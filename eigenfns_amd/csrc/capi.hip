@@ -875,6 +875,54 @@ int efn_rowreduce_strided(int op, const void* A, int64_t stride_a, const void* B
   return EFN_OK;
 }
 
+size_t efn_rowstats_work_bytes(int m, int64_t n) {
+  return 2 * efn_rowreduce_work_bytes(m, n);
+}
+
+int efn_rowstats_strided(const void* A, int64_t stride_a, const void* B, int64_t stride_b, int nrow,
+                         int m, int64_t n, double* out4, void* work, size_t work_bytes, void* stream) {
+  if (m < 0 || n <= 0) return fail(EFN_EINVAL, "need m >= 0 and n > 0");
+  if (m == 0) return EFN_OK;
+  if (nrow <= 0 || stride_a < nrow || stride_b < nrow) return fail(EFN_EINVAL, "row map needs 0 < nrow <= stride");
+  if (!A || !B || !out4) return fail(EFN_EINVAL, "NULL pointer");
+  if (!work || work_bytes < efn_rowstats_work_bytes(m, n)) return fail(EFN_EINVAL, "rowstats work buffer too small");
+  hipStream_t s = (hipStream_t)stream;
+  const int ns = rowreduce_nsplit(n);
+  const long long kper = (n + ns - 1) / ns;
+  hipLaunchKernelGGL(efn::rowstats_partial_kernel, dim3(ns, m), dim3(256), 0, s, (const float2*)A,
+                     (long long)stride_a, (const float2*)B, (long long)stride_b, nrow, (long long)n, kper,
+                     (double2*)work);
+  EFN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(efn::rowstats_final_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
+                     (const double2*)work, ns, m, out4);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+int efn_row_axpby_norm_strided(const void* X, int64_t stride_x, const double* a, const void* Y,
+                               int64_t stride_y, const double* b, int nrow, int m, int64_t n, void* out,
+                               int64_t stride_out, double* norm2_c128, void* work, size_t work_bytes,
+                               void* stream) {
+  if (m < 0 || n <= 0) return fail(EFN_EINVAL, "need m >= 0 and n > 0");
+  if (m == 0) return EFN_OK;
+  if (nrow <= 0 || stride_x < nrow || stride_out < nrow || (Y && stride_y < nrow))
+    return fail(EFN_EINVAL, "row map needs 0 < nrow <= stride");
+  if (!X || !a || !out || !norm2_c128 || (Y && !b)) return fail(EFN_EINVAL, "NULL pointer");
+  if (out == X || (Y && out == Y)) return fail(EFN_EINVAL, "out must not alias X or Y");
+  if (!work || work_bytes < efn_rowreduce_work_bytes(m, n)) return fail(EFN_EINVAL, "work buffer too small");
+  hipStream_t s = (hipStream_t)stream;
+  const int ns = rowreduce_nsplit(n);
+  const long long kper = (n + ns - 1) / ns;
+  hipLaunchKernelGGL(efn::row_axpby_norm_kernel, dim3(ns, m), dim3(256), 0, s, (const float2*)X,
+                     (long long)stride_x, a, (const float2*)Y, (long long)stride_y, b, nrow, (long long)n,
+                     kper, (float2*)out, (long long)stride_out, (double2*)work);
+  EFN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(efn::rowreduce_final_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
+                     (const double2*)work, ns, m, (double2*)norm2_c128);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
 int efn_rowreduce(int op, const void* A, const void* B, int m, int64_t n, void* out, void* work,
                   size_t work_bytes, void* stream) {
   const int nrow = m > 0 ? m : 1;

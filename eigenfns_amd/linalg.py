@@ -341,6 +341,36 @@ def row_axpby_strided(X, stride_x, a, Y, stride_y, b, nrow, m, n, out, stride_ou
     return out
 
 
+def rowstats_strided(A, B, nrow: int, stride_a: int, stride_b: int, m: int, n: int):
+    """(m, 4) FP64 per logical row: Re⟨A_i,B_i⟩, Im⟨A_i,B_i⟩, |A_i|², |B_i|² in one
+    read of each block (Rayleigh quotient and residual inputs, solver.py:228-238)."""
+    out = torch.empty((m, 4), dtype=torch.float64, device=A.device)
+    lib = _lib.load()
+    work = SCRATCH.get(lib.efn_rowstats_work_bytes(m, n), "rowstats")
+    _lib.check(lib.efn_rowstats_strided(A.data_ptr(), stride_a, B.data_ptr(), stride_b, nrow, m, n,
+                                        out.data_ptr(), work.data_ptr(), work.numel(),
+                                        stream_ptr()), "rowstats_strided")
+    return out
+
+
+def row_axpby_norm_strided(X, stride_x, a, Y, stride_y, b, nrow, m, n, out, stride_out):
+    """out_i = a_i·X_i + b_i·Y_i over strided rows; returns |out_i|² (FP64, m) from
+    the same pass."""
+    a = torch.as_tensor(a, dtype=torch.float64, device=X.device).reshape(-1).contiguous()
+    if Y is not None:
+        b = torch.as_tensor(b, dtype=torch.float64, device=X.device).reshape(-1).contiguous()
+    nrm = torch.empty(m, dtype=torch.complex128, device=X.device)
+    lib = _lib.load()
+    work = SCRATCH.get(lib.efn_rowreduce_work_bytes(m, n), "rowred")
+    _lib.check(lib.efn_row_axpby_norm_strided(X.data_ptr(), stride_x, a.data_ptr(),
+                                              None if Y is None else Y.data_ptr(), stride_y,
+                                              None if Y is None else b.data_ptr(), nrow, m, n,
+                                              out.data_ptr(), stride_out, nrm.data_ptr(),
+                                              work.data_ptr(), work.numel(), stream_ptr()),
+               "row_axpby_norm_strided")
+    return nrm.real
+
+
 _POOL = None
 
 

@@ -199,7 +199,23 @@ int efn_row_axpby_strided(const void* X_c64, int64_t stride_x, const double* a_d
                           const void* Y_c64, int64_t stride_y, const double* b_dev, int nrow, int m,
                           int64_t n, void* out_c64, int64_t stride_out, void* stream);
 
-/* out_i = a[i]·X_i + b[i]·Y_i with per-row FP64 coefficients on the device
+/* Fused residual statistics (solver.py:228-238 per row, one read of each
+ * block): out4[i] = (Re⟨A_i,B_i⟩, Im⟨A_i,B_i⟩, |A_i|², |B_i|²) in FP64, strided
+ * rows as above.  work ≥ efn_rowstats_work_bytes(m, n). */
+size_t efn_rowstats_work_bytes(int m, int64_t n);
+int efn_rowstats_strided(const void* A_c64, int64_t stride_a, const void* B_c64, int64_t stride_b,
+                         int nrow, int m, int64_t n, double* out4, void* work, size_t work_bytes,
+                         void* stream);
+/* out_i = a[i]·X_i + b[i]·Y_i (strided rows) and norm2[i] = (|out_i|², 0) of the
+ * stored FP32 values in FP64, in one pass (the residual R = HX − λX and its
+ * norm, solver.py:229-231).  out must not alias X or Y;
+ * work ≥ efn_rowreduce_work_bytes(m, n). */
+int efn_row_axpby_norm_strided(const void* X_c64, int64_t stride_x, const double* a_dev,
+                               const void* Y_c64, int64_t stride_y, const double* b_dev, int nrow,
+                               int m, int64_t n, void* out_c64, int64_t stride_out,
+                               double* norm2_c128, void* work, size_t work_bytes, void* stream);
+
+/* out_i = a[i]·X_i + b[i]·Y_i with per-row FP64 coefficients on the device/* out_i = a[i]·X_i + b[i]·Y_i with per-row FP64 coefficients on the device
  * (Y, b may be NULL).  R = HX − λX (solver.py:229), row scaling (:306-307). */
 int efn_row_axpby(const void* X_c64, const double* a_dev, const void* Y_c64, const double* b_dev,
                   int m, int64_t n, void* out_c64, void* stream);

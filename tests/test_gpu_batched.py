@@ -69,7 +69,8 @@ def test_theta_multi_equals_per_operator():
 
 def test_batched_mix_matches_dense(dense_refs):
     from eigenfns_amd.batched import solve_batched
-    done = solve_batched(_requests(), 8, m=16, guard=6, nslots=3, tol=1e-6, maxit=300)
+    done = solve_batched(_requests(), 8, m=16, guard=6, nslots=3, tol=1e-6, maxit=300,
+                         verify_dead_rows=True)
     assert len(done) == len(PROBLEMS)
     for req in done:
         w = dense_refs[req.rid]
@@ -109,3 +110,30 @@ def test_batched_keeps_vectors_orthonormal():
         lam = np.real(np.sum(V.conj() * HV, axis=1))
         r = np.linalg.norm(HV - lam[:, None] * V, axis=1) / np.linalg.norm(HV, axis=1)
         assert r.max() < 1e-4
+
+
+def test_fused_row_statistics_match_separate_reductions():
+    """rowstats / row_axpby_norm (one pass each) against the separate row
+    reductions and axpby over the same strided rows."""
+    import torch
+    from eigenfns_amd.linalg import (row_axpby_norm_strided, row_axpby_strided, rowreduce_strided,
+                                     rowstats_strided)
+    rng = np.random.default_rng(9)
+    m, nslot, n = 5, 3, 2 * 12 ** 3
+    S = torch.from_numpy((rng.standard_normal((nslot * 3 * m, n))
+                          + 1j * rng.standard_normal((nslot * 3 * m, n))).astype(np.complex64)).cuda()
+    H = torch.from_numpy((rng.standard_normal((nslot * 3 * m, n))
+                          + 1j * rng.standard_normal((nslot * 3 * m, n))).astype(np.complex64)).cuda()
+    M = nslot * m
+    st = rowstats_strided(S, H, m, 3 * m, 3 * m, M, n)
+    dot = rowreduce_strided(1, S, H, m, 3 * m, 3 * m, M, n)
+    assert torch.allclose(st[:, 0], dot.real, rtol=1e-13) and torch.allclose(st[:, 1], dot.imag, rtol=1e-12)
+    assert torch.allclose(st[:, 2], rowreduce_strided(0, S, None, m, 3 * m, 3 * m, M, n).real, rtol=1e-13)
+    assert torch.allclose(st[:, 3], rowreduce_strided(0, H, None, m, 3 * m, 3 * m, M, n).real, rtol=1e-13)
+    lam = st[:, 0] / st[:, 2]
+    R1 = torch.empty((M, n), dtype=torch.complex64, device="cuda")
+    R2 = torch.empty_like(R1)
+    nr = row_axpby_norm_strided(H, 3 * m, torch.ones_like(lam), S, 3 * m, -lam, m, M, n, R1, m)
+    row_axpby_strided(H, 3 * m, torch.ones_like(lam), S, 3 * m, -lam, m, M, n, R2, m)
+    assert torch.equal(R1, R2)
+    assert torch.allclose(nr, rowreduce_strided(0, R2, None, m, m, m, M, n).real, rtol=1e-13)
