@@ -26,16 +26,19 @@ from .workloads import NET_SEEDS, Spec, network, request_stream  # noqa: F401 (r
 
 
 class StructureCache:
-    """ε grids rasterised once per (network, grid)."""
+    """ε grids rasterised once per (network, grid); safe to share between threads."""
 
     def __init__(self):
+        import threading
         self._eps = {}
+        self._lock = threading.Lock()
 
     def get(self, n_points: int, grid: int):
         key = (n_points, grid)
-        if key not in self._eps:
-            self._eps[key] = network(n_points, grid)
-        return self._eps[key]
+        with self._lock:
+            if key not in self._eps:
+                self._eps[key] = network(n_points, grid)
+            return self._eps[key]
 
 
 def _make_op(cache, sp, workspace: bool):
@@ -86,13 +89,17 @@ def _summary(done, wall, turns, engines):
 
 
 def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slots=32,
-                  hbm_budget=200e9, verbose=False, threads=True, progress=False):
+                  hbm_budget=200e9, verbose=False, threads=True, progress=False,
+                  engines_per_grid=1):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
 
     threads=True: each grid's engine runs in its own host thread on its own HIP
     stream (default priority), so one engine's host work (small eigensolves,
     admission) overlaps the others' kernels, and small kernels of different
-    engines can share the GPU.  Requests are admitted FIFO per grid."""
+    engines can share the GPU.  Requests are admitted FIFO per grid.
+    engines_per_grid > 1 splits each grid's slots over that many engines
+    (threads, streams) pulling from the same queue, so one half's host work
+    overlaps the other half's kernels."""
     cache = cache or StructureCache()
     grids = sorted({sp.grid for sp in specs})
     max_nev = max(sp.nev for sp in specs)
@@ -107,24 +114,34 @@ def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slo
     lock = threading.Lock()
     t0 = time.perf_counter()
 
-    def worker(g):
+    k = max(1, int(engines_per_grid))
+
+    def worker(g, e):
         try:
             stream = torch.cuda.Stream()
             with torch.cuda.stream(stream):
                 pending = queues[g]
-                eng = BatchedLOBPCG(g, m, guard, min(plan[g], len(pending)), max_nev, tol, maxit)
+                nslots = max(1, min(-(-plan[g] // k), -(-len(pending) // k)))
+                eng = BatchedLOBPCG(g, m, guard, nslots, max_nev, tol, maxit)
                 with lock:
-                    engines[g] = eng
-                while pending or eng.active():
+                    engines[str(g) if k == 1 else f"{g}.{e}"] = eng
+                while True:
                     now = time.perf_counter() - t0
-                    while pending and pending[0].t_arrive <= now and eng.free_slots():
-                        sp = pending.pop(0)
+                    admit = []
+                    with lock:  # the grid's queue is shared by its engines
+                        while pending and pending[0].t_arrive <= now and len(admit) < eng.free_slots():
+                            admit.append(pending.pop(0))
+                        left = len(pending)
+                        nxt = pending[0].t_arrive if pending else None
+                    for sp in admit:
                         req = Request(_make_op(cache, sp, workspace=False), sp.nev, seed=sp.seed,
                                       rid=sp.rid, meta={"grid": sp.grid, "n_points": sp.n_points})
                         req.t_arrive = t0 + sp.t_arrive
                         eng.admit(req)
                     if not eng.active():
-                        time.sleep(max(0.0, min(0.01, pending[0].t_arrive - now)))
+                        if not left:
+                            break
+                        time.sleep(max(0.0, min(0.01, nxt - now)))
                         continue
                     finished = eng.step()
                     for req in finished:
@@ -141,7 +158,8 @@ def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slo
         except BaseException as exc:  # re-raised in the caller
             errors.append(exc)
 
-    workers = [threading.Thread(target=worker, args=(g,), name=f"engine-{g}") for g in grids]
+    workers = [threading.Thread(target=worker, args=(g, e), name=f"engine-{g}.{e}")
+               for g in grids for e in range(k)]
     for w in workers:
         w.start()
     for w in workers:
@@ -230,12 +248,13 @@ if __name__ == "__main__":
     ap.add_argument("--plain", action="store_true")
     ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--sequential", action="store_true", help="one host thread, one stream")
+    ap.add_argument("--engines-per-grid", type=int, default=1)
     args = ap.parse_args()
     grids = tuple(int(g) for g in args.grids.split(","))
     specs = request_stream(args.n, args.seed, grids=grids, nev_range=(8, args.nev_max))
     cache = StructureCache()
     res, summ = serve_batched(specs, cache, max_slots=args.slots, verbose=True,
-                              threads=not args.sequential)
+                              threads=not args.sequential, engines_per_grid=args.engines_per_grid)
     print(json.dumps({"batched": summ}), flush=True)
     if args.plain:
         pres, psumm = serve_plain(specs, cache)
