@@ -155,3 +155,38 @@ def test_kpm_moments_deflated_count_and_dos(g8):
     assert np.abs(mom - mref).max() < 1e-5 * np.abs(mref).max()
     mean, se, _ = counting(mom, lam_max, [lam_max])
     assert abs(mean[0] - 1022) < 1e-3
+
+
+@pytest.mark.parametrize("k", [(0.0, 0.0, 0.0), (0.2, -0.1, 0.35)])
+def test_homogeneous_spectrum_matches_analytic_on_device(k):
+    """The reference's known answers (tests/test_operator.py:27-38, :53-65) on the
+    device path: in homogeneous ε the spectrum is |k+G|²/ε, each twice (two
+    transverse polarizations), with the two Γ zero modes excluded."""
+    from eigenfns_amd.operator import MaxwellOperator
+    from eigenfns_amd.solver import lobpcg_blocks
+    from oracle.operator_np import make_basis
+    G, L, e0, nev = 8, 1.3, 2.5, 20
+    op = MaxwellOperator(np.full((G, G, G), e0, np.float32), L, k)
+    vals, _, _ = lobpcg_blocks(op, nev, m=16, guard=8, tol=1e-6, verbose=False)
+    kn, _, _ = make_basis(G, L, k)
+    lam = np.sort(np.repeat((kn.ravel() ** 2) / e0, 2))
+    if np.allclose(k, 0):
+        lam = lam[2:]
+    assert np.abs(vals - lam[:nev]).max() / lam[nev - 1] < 1e-6
+
+
+def test_device_theta_is_hermitian_and_psd():
+    """⟨X, ΘY⟩ = ⟨ΘX, Y⟩ and ⟨X, ΘX⟩ ≥ 0 on a disordered ε (reference
+    tests/test_operator.py:41-50), evaluated with the device Θ and Gram."""
+    from eigenfns_amd.linalg import gram
+    from eigenfns_amd.operator import MaxwellOperator
+    rng = np.random.default_rng(8)
+    G = 16
+    eps = np.where(rng.random((G, G, G)) < 0.35, 8.41, 1.0).astype(np.float32)
+    op = MaxwellOperator(eps, 1.1, (0.1, 0.2, -0.3))
+    X, Y = op.random_block(6, seed=1), op.random_block(6, seed=2)
+    a = gram(X, op.theta(Y)).cpu().numpy()
+    b = gram(op.theta(X), Y).cpu().numpy()
+    assert np.abs(a - b).max() < 1e-5 * np.abs(a).max()
+    q = np.diag(gram(X, op.theta(X)).cpu().numpy())
+    assert np.all(q.real > 0) and np.abs(q.imag).max() < 1e-5 * q.real.max()
