@@ -175,7 +175,7 @@ def serving_sample():
             "max_rel_diff_batched_vs_plain": diff, "requests": len(specs),
             "slots": summ["slots"],
             "sample": "24 seeded requests (request_stream seed 11), N∈{64,216,1000}, "
-                      "G∈{32,48,64}, k~U[-1/2,1/2)^3, nev 8-64, block 32, guard 8, "
+                      "G∈{32,48,64}, k~U[-1/2,1/2)^3, nev 8-64, block 24, guard 8, "
                       "residual tol 1e-4, all arriving at t=0"}
 
 

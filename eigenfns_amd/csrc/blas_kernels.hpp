@@ -330,6 +330,14 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
       }
 }
 
+// zero fill of count double2 (a kernel rather than a memset node, so the
+// clear is safe inside captured graphs)
+__global__ void zero_double2_kernel(double2* __restrict__ out, long long count) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+       e += (long long)gridDim.x * blockDim.x)
+    out[e] = make_double2(0.0, 0.0);
+}
+
 // out[prob][e] = Σ_s part[prob][s][e] in fixed order (grid.y = problem).
 // herm: only tiles ti ≤ tj were computed; the result is made exactly
 // Hermitian — entry (i, j) with i > j is the conjugate of (j, i) and the

@@ -727,7 +727,10 @@ int gram_multi_impl(int nprob, const void* const* A, const void* const* B, int m
   if (!A || !B || !out) return fail(EFN_EINVAL, "NULL pointer");
   hipStream_t s = (hipStream_t)stream;
   if (n == 0) {
-    EFN_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)nprob * ma * mb * sizeof(double2), s));
+    const long long cnt = (long long)nprob * ma * mb;
+    hipLaunchKernelGGL(efn::zero_double2_kernel, dim3((unsigned)((cnt + 255) / 256 < 4096 ? (cnt + 255) / 256 : 4096)),
+                       dim3(256), 0, s, (double2*)out, cnt);
+    EFN_LAUNCH_CHECK();
     return EFN_OK;
   }
   efn::PairPtrs pp;
@@ -853,7 +856,9 @@ int efn_rowreduce_strided(int op, const void* A, int64_t stride_a, const void* B
   if (!A || !out || (op == 1 && !B)) return fail(EFN_EINVAL, "NULL pointer");
   hipStream_t s = (hipStream_t)stream;
   if (n == 0) {
-    EFN_HIP_CHECK(hipMemsetAsync(out, 0, (size_t)m * sizeof(double2), s));
+    hipLaunchKernelGGL(efn::zero_double2_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
+                       (double2*)out, (long long)m);
+    EFN_LAUNCH_CHECK();
     return EFN_OK;
   }
   const size_t need = efn_rowreduce_work_bytes(m, n);

@@ -88,7 +88,7 @@ def _summary(done, wall, turns, engines):
             "host_phase_s": timing, **_stats(lat)}
 
 
-def serve_batched(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, max_slots=32,
+def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slots=32,
                   hbm_budget=200e9, verbose=False, threads=True, progress=False,
                   engines_per_grid=1):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
@@ -211,7 +211,7 @@ def _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose
     return done, _summary(done, time.perf_counter() - t0, turns, engines)
 
 
-def serve_plain(specs, cache=None, m=32, guard=8, tol=1e-4, maxit=400, progress=False):
+def serve_plain(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, progress=False):
     """The same stream solved one request at a time with lobpcg_blocks (FIFO)."""
     from .solver import lobpcg_blocks
     cache = cache or StructureCache()

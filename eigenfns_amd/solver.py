@@ -22,7 +22,8 @@ import torch
 
 from .linalg import (DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_DROP, apply_chunked, combine,
                      combine_into, deflate, deflate_chunk_rows, eigh, gram, gram_herm, normalize_rows,
-                     ortho_block, row_axpby, rowdot, rownorm, rownorm2, svqb_coeffs)
+                     ortho_block, row_axpby, row_axpby_norm_strided, rowdot, rownorm, rownorm2,
+                     rowstats_strided, svqb_coeffs)
 
 __all__ = ["gram", "combine", "deflate", "ortho_block", "apply_chunked", "deflate_chunk_rows",
            "SolveStats", "lobpcg_blocks", "lobpcg_blocks_resumable", "_flat",
@@ -108,6 +109,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
     T2 = torch.empty_like(T1)
     X, W, P = S[:m], S[m:2 * m], S[2 * m:]
     HX = HS[:m]
+    n = 2 * G ** 3  # complex entries per vector
     stats = SolveStats()
     t0 = time.perf_counter()
     carry = None
@@ -138,13 +140,16 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
         n_lock = min(n_lock_per, nev - locked_vals.size)
         it = 0
         for it in range(maxit):
-            lam = rq(X, HX)
-            R = row_axpby(HX, torch.ones_like(lam), X, -lam, out=T2)
-            xnorm = rownorm(X)
-            rn = rownorm(R) / torch.clamp(rownorm(HX), min=1e-30)
-            lam_h = lam.cpu().numpy()
-            rn_h = rn.cpu().numpy()
-            dead_rows = xnorm.cpu().numpy() < 0.5
+            # one read of X and ΘX for ⟨x,Θx⟩, |x|², |Θx|²; R = ΘX − λX with |R|² in the
+            # same pass; one host sync for all four (solver.py:228-238)
+            st = rowstats_strided(X, HX, m, m, m, m, n)
+            lam = st[:, 0] / torch.clamp(st[:, 2], min=1e-30) if normalized_rq else st[:, 0]
+            rr2 = row_axpby_norm_strided(HX, m, torch.ones_like(lam), X, m, -lam, m, m, n, T2, m)
+            R = T2
+            host = torch.stack([lam, st[:, 2], rr2, st[:, 3]]).cpu().numpy()
+            lam_h = host[0]
+            rn_h = np.sqrt(host[2]) / np.maximum(np.sqrt(host[3]), 1e-30)
+            dead_rows = np.sqrt(host[1]) < 0.5
             lam_h = np.where(dead_rows, np.inf, lam_h)
             rn_h = np.where(dead_rows, np.inf, rn_h)
             order = np.argsort(lam_h)

@@ -104,7 +104,10 @@ def request_stream(n: int, seed: int = 0, rate: float | None = None, nets=C5["ne
 # smaller grids, all arriving at t = 0; the requests at 32³ with nev ≤ 32 are
 # checked against FP64 oracle eigenvalues.
 SERVING_SAMPLE = dict(n=24, seed=11, grids=(32, 48, 64))
-SERVING_SOLVER = dict(m=32, guard=8, tol=1e-4, maxit=400)
+# block 24 / guard 8: measured 5.5 solves/s on the sample against 5.0 for 32/8,
+# 4.7 for 32/12 and 3.4 for 48/16, every setting within 5e-7 of the FP64
+# oracle (the error is the FP32 floor, not the 1e-4 residual tolerance)
+SERVING_SOLVER = dict(m=24, guard=8, tol=1e-4, maxit=400)
 
 
 def serving_sample():
