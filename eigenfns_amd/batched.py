@@ -31,9 +31,9 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .linalg import (RR_DEAD_PENALTY, SVQB_DROP, combine_multi, copy_multi, eigh_batched_host,
-                     gram_multi, rowreduce_strided, row_axpby_strided, row_axpby_norm_strided,
-                     rowstats_strided, MAX_PROBLEMS)
+from .linalg import (COMBINE_IN_PLACE_MAX, RR_DEAD_PENALTY, SVQB_DROP, combine_multi, copy_multi,
+                     eigh_batched_host, gram_multi, rowreduce_strided, row_axpby_strided,
+                     row_axpby_norm_strided, rowstats_strided, MAX_PROBLEMS)
 from .operator import MaxwellOperator, precondition_multi, theta_multi
 
 
@@ -113,8 +113,9 @@ class BatchedLOBPCG:
         self.S = torch.zeros((nslots * m3,) + shape, dtype=c64, device=dev)
         self.HS = torch.zeros_like(self.S)
         self.R = torch.zeros((nslots * self.m,) + shape, dtype=c64, device=dev)
-        self.T1 = torch.zeros_like(self.R)
-        self.T2 = torch.zeros_like(self.R)
+        # per slot [T1_s; T2_s] (2m rows): the new X and P rows of a Rayleigh–Ritz
+        # update come from one combine over S; T1_s is also SVQB scratch
+        self.T12 = torch.zeros((2 * nslots * self.m,) + shape, dtype=c64, device=dev)
         self.CARRY = torch.zeros_like(self.R)
         self.L = torch.zeros((nslots * self.cap,) + shape, dtype=c64, device=dev)
         self.slots: list[_Slot | None] = [None] * nslots
@@ -171,6 +172,22 @@ class BatchedLOBPCG:
     def _rows(self, buf, s, width):
         return buf[s * width:(s + 1) * width]
 
+    def _t1(self, s):
+        return self.T12[2 * s * self.m:(2 * s + 1) * self.m]
+
+    def _t2(self, s):
+        return self.T12[(2 * s + 1) * self.m:(2 * s + 2) * self.m]
+
+    def _combine_back(self, idx, C, blocks):
+        """block_j ← C_jᵀ·block_j for slots idx: in place up to
+        COMBINE_IN_PLACE_MAX rows, else through T1 and a copy."""
+        if self.m <= COMBINE_IN_PLACE_MAX:
+            combine_multi(C, blocks, blocks)
+        else:
+            Ts = [self._t1(s) for s in idx]
+            combine_multi(C, blocks, Ts)
+            copy_multi(Ts, blocks)
+
     def _Lk(self, s, k):
         return self.L[s * self.cap:s * self.cap + k]
 
@@ -197,9 +214,7 @@ class BatchedLOBPCG:
             keep = w > SVQB_DROP * wmax
             scale = np.where(keep, 1.0 / np.sqrt(np.where(keep, w, 1.0)), 0.0)
             Cw = torch.as_tensor(V * scale[:, None, :], device=self.S.device)
-            Ts = [self._rows(self.T1, s, self.m) for s in idx]
-            combine_multi(Cw, Bs, Ts)
-            copy_multi(Ts, Bs)
+            self._combine_back(idx, Cw, Bs)
         return keep
 
     def _theta(self, idx, src, row0_src, dst, row0_dst, stride_src, stride_dst, nrow):
@@ -261,9 +276,7 @@ class BatchedLOBPCG:
         wpmax = np.maximum(wp.max(axis=1, keepdims=True), 1e-300)
         keepp = (wp > SVQB_DROP * wpmax) & (wpmax > 0)
         scale = np.where(keepp, 1.0 / np.sqrt(np.where(keepp, wp, 1.0)), 0.0)
-        Ts = [self._rows(self.T1, s, m) for s in idx]
-        combine_multi(torch.as_tensor(Vp * scale[:, None, :], device=self.S.device), Ps, Ts)
-        copy_multi(Ts, Ps)
+        self._combine_back(idx, torch.as_tensor(Vp * scale[:, None, :], device=self.S.device), Ps)
         t = self._tick("P", t)
         # Θ(W), Θ(P) fresh; Rayleigh–Ritz on [X; W; P] (solver.py:275-302)
         self._theta(idx, self.S, m, self.HS, m, 3 * m, 3 * m, 2 * m)
@@ -291,30 +304,30 @@ class BatchedLOBPCG:
         t = self._tick("RR_gram+eigh", t)
         Ct = torch.as_tensor(C, device=self.S.device)
         Cp = Ct.clone()
-        Cp[:, :m] = 0
-        T1s = [self._rows(self.T1, s, m) for s in idx]
-        T2s = [self._rows(self.T2, s, m) for s in idx]
-        combine_multi(Ct, Sbs, T1s)
-        combine_multi(Cp, Sbs, T2s)
-        self._deflate(idx, lambda s: self._rows(self.T1, s, m))
+        Cp[:, :m] = 0                                   # Pn: X rows zeroed
+        combine_multi(torch.cat([Ct, Cp], dim=2), Sbs,
+                      [self.T12[2 * s * m:2 * (s + 1) * m] for s in idx])
+        self._deflate(idx, self._t1)
         # normalised rows back into X and P of every slot (rows of slots not
         # iterating hold nothing that is read again before they are rebuilt)
-        nrm1 = rowreduce_strided(0, self.T1, None, m, m, m, self.nslots * m, self.n).real
-        nrm2 = rowreduce_strided(0, self.T2, None, m, m, m, self.nslots * m, self.n).real
+        nrm = rowreduce_strided(0, self.T12, None, 2 * m, 2 * m, 2 * m, 2 * self.nslots * m,
+                                self.n).real.view(self.nslots, 2, m)
+        nrm1, nrm2 = nrm[:, 0].reshape(-1), nrm[:, 1].reshape(-1)
         live = torch.zeros(self.nslots * m, dtype=torch.bool, device=self.S.device)
         for s in idx:
             live[s * m:(s + 1) * m] = True
         a1 = torch.where(live, 1.0 / torch.clamp(torch.sqrt(nrm1), min=1e-20), torch.ones_like(nrm1))
         a2 = torch.where(live, 1.0 / torch.clamp(torch.sqrt(nrm2), min=1e-20), torch.ones_like(nrm2))
-        # X rows of non-iterating slots are preserved: copy them through T1 first
+        # X and P rows of non-iterating slots are preserved: copied through T12 first
         keep = [s for s in range(self.nslots) if s not in set(idx)]
         if keep:
-            copy_multi([self._X(s) for s in keep], [self._rows(self.T1, s, m) for s in keep])
-            copy_multi([self._P(s) for s in keep], [self._rows(self.T2, s, m) for s in keep])
-        row_axpby_strided(self.T1, m, a1, None, m, None, m, self.nslots * m, self.n,
+            copy_multi([self._X(s) for s in keep], [self._t1(s) for s in keep])
+            copy_multi([self._P(s) for s in keep], [self._t2(s) for s in keep])
+        row_axpby_strided(self.T12, 2 * m, a1, None, 2 * m, None, m, self.nslots * m, self.n,
                           self.S, 3 * m)
         P0 = self.S[2 * m:]
-        row_axpby_strided(self.T2, m, a2, None, m, None, m, self.nslots * m, self.n, P0, 3 * m)
+        row_axpby_strided(self.T12[m:], 2 * m, a2, None, 2 * m, None, m, self.nslots * m, self.n,
+                          P0, 3 * m)
         t = self._tick("update_XP", t)
         self._theta(idx, self.S, 0, self.HS, 0, 3 * m, 3 * m, m)
         t = self._tick("theta_X", t)

@@ -129,7 +129,13 @@ __global__ __launch_bounds__(256) void gram_partial_kernel(PairPtrs pp, int ma, 
 // i/4 for f64 (P::row maps lane group and element to the output row).
 //   GramF64: v_mfma_f64_16x16x4f64 on the exact FP64 images of the FP32
 //            inputs — products exact, sums in FP64 (same precision class as
-//            the VALU kernel).
+//            the VALU kernel).  Three real products per complex entry
+//            (Gauss): P1 = Σ a_r·b_r, P2 = Σ a_i·b_i, P3 = Σ (a_r − a_i)(b_r + b_i),
+//            Re = P1 + P2, Im = (P3 − P1) + P2.  a_r − a_i and b_r + b_i of
+//            two FP32 values are exact in FP64 unless their exponents differ
+//            by more than 29, and the final cancellation costs a few FP64
+//            ulps of Σ|a||b| — far below the FP32 data — for 3/4 of the
+//            MFMA work of the four-product form.
 //   GramF32: v_mfma_f32_16x16x4f32, FP32 accumulators added into FP64 every
 //            kMFlush steps (≤ 4·kMFlush·4 products per FP32 partial sum).
 // The four waves' tiles are summed through LDS in the fixed order
@@ -141,10 +147,13 @@ constexpr int kMWaveTile = 32;
 constexpr int kMFlush = 4;
 constexpr int kMStep = 8;  // complex entries per wave step
 #ifndef EFN_GRAM_PREFETCH
-#define EFN_GRAM_PREFETCH 3
+#define EFN_GRAM_PREFETCH 2
 #endif
 constexpr int kMPrefetch = EFN_GRAM_PREFETCH;  // steps in flight per wave
 
+#ifndef EFN_GRAM_GAUSS
+#define EFN_GRAM_GAUSS 1
+#endif
 struct GramF64 {
   typedef double in_t;
   typedef doublex4 acc_t;
@@ -153,6 +162,7 @@ struct GramF64 {
   }
   static __device__ __forceinline__ int row(int g, int q) { return g + 4 * q; }
   static constexpr bool kFlush = false;
+  static constexpr bool kGauss = EFN_GRAM_GAUSS;
 };
 struct GramF32 {
   typedef float in_t;
@@ -162,6 +172,7 @@ struct GramF32 {
   }
   static __device__ __forceinline__ int row(int g, int q) { return 4 * g + q; }
   static constexpr bool kFlush = true;
+  static constexpr bool kGauss = false;
 };
 
 __device__ __forceinline__ float4 load_c2(const float2* __restrict__ row, long long k, long long ke) {
@@ -174,23 +185,46 @@ __device__ __forceinline__ float4 load_c2(const float2* __restrict__ row, long l
   return make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// grid: (splits, tiles_i·tiles_j, problems) with 32×32 tiles; kper a multiple
-// of 4·kMStep; rows 16-byte aligned (n even, aligned bases).
+// Block → (k split, output tile).  The tiles of one split are dispatched
+// together, so the A and B rows they share are read from HBM about once;
+// order 2 also keeps them on one XCD (workgroups go round-robin over the 8
+// XCDs, each with its own L2), provided the block count is a multiple of 8.
+#ifndef EFN_GRAM_ORDER
+#define EFN_GRAM_ORDER 2
+#endif
+__device__ __forceinline__ void gram_block_of(int L, int ntile, int nsplit, int& split, int& tile) {
+  if (EFN_GRAM_ORDER == 0) {  // split-major
+    split = L % nsplit;
+    tile = L / nsplit;
+    return;
+  }
+  const int N = ntile * nsplit;
+  int v = L;
+  if (EFN_GRAM_ORDER == 2 && (N & 7) == 0) v = (L & 7) * (N >> 3) + (L >> 3);
+  split = v / ntile;
+  tile = v - split * ntile;
+}
+
+// grid: (splits·tiles, 1, problems) with 32×32 tiles; kper a multiple of
+// 4·kMStep; rows 16-byte aligned (n even, aligned bases).
 template <class P>
-__global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int mb, int tiles_j,
-                                                        long long n, long long kper,
-                                                        double2* __restrict__ part) {
+__global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, int mb, int tiles_j,
+                                                        int ntile, int nsplit, long long n,
+                                                        long long kper, double2* __restrict__ part) {
   typedef typename P::in_t T;
   typedef typename P::acc_t Acc;
   __shared__ double2 red[2][kMWaveTile * kMWaveTile];
   const float2* __restrict__ A = pp.a[blockIdx.z];
   const float2* __restrict__ B = pp.b[blockIdx.z];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // wave index through readfirstlane: the compiler then keeps the k range in
+  // scalar registers and the step checks below are scalar branches
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
-  int ti_, tj_;
-  tile_of(blockIdx.y, tiles_j, ti_, tj_);
+  int split, tile, ti_, tj_;
+  gram_block_of(blockIdx.x, ntile, nsplit, split, tile);
+  tile_of(tile, tiles_j, ti_, tj_);
   const int i0 = ti_ * kMWaveTile, j0 = tj_ * kMWaveTile;
-  const long long kb_blk = (long long)blockIdx.x * kper;
+  const long long kb_blk = (long long)split * kper;
   const long long ke_blk = kb_blk + kper < n ? kb_blk + kper : n;
   const long long kw = kper / 4;
   const long long kb = kb_blk + wave * kw;
@@ -199,7 +233,8 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
   const bool rb[2] = {j0 + r < mb, j0 + 16 + r < mb};
   const float2* ap[2] = {A + (long long)(ra[0] ? i0 + r : 0) * n, A + (long long)(ra[1] ? i0 + 16 + r : 0) * n};
   const float2* bp[2] = {B + (long long)(rb[0] ? j0 + r : 0) * n, B + (long long)(rb[1] ? j0 + 16 + r : 0) * n};
-  Acc cre[2][2], cim[2][2];
+  // four-product form: cre = Re, cim = Im; Gauss form: cre = P1, cim = P2, c3 = P3
+  Acc cre[2][2], cim[2][2], c3[2][2];
   double dre[2][2][4], dim_[2][2][4];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -209,15 +244,19 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
       for (int q = 0; q < 4; ++q) {
         cre[a][b][q] = 0;
         cim[a][b][q] = 0;
+        c3[a][b][q] = 0;
         dre[a][b][q] = dim_[a][b][q] = 0.0;
       }
     }
   int steps = 0;
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
   // register ring of kMPrefetch steps: each step's loads are issued
-  // kMPrefetch − 1 steps (of 32 MFMAs each) before they are consumed
-  auto step = [&](const float4 (&av)[2], const float4 (&bv)[2]) {
+  // kMPrefetch − 1 steps (of 24 or 32 MFMAs each) before they are consumed
+  // rows past ma / mb were loaded from a clamped (valid) row and count as zero
+  auto step = [&](const float4 (&av_)[2], const float4 (&bv_)[2]) {
     T x[2][4], y[2][4], w[2][4];
+    const float4 av[2] = {ra[0] ? av_[0] : z4, ra[1] ? av_[1] : z4};
+    const float4 bv[2] = {rb[0] ? bv_[0] : z4, rb[1] ? bv_[1] : z4};
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       x[a][0] = (T)av[a].x; x[a][1] = (T)av[a].y; x[a][2] = (T)av[a].z; x[a][3] = (T)av[a].w;
@@ -225,17 +264,42 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       y[b][0] = (T)bv[b].x; y[b][1] = (T)bv[b].y; y[b][2] = (T)bv[b].z; y[b][3] = (T)bv[b].w;
-      w[b][0] = y[b][1]; w[b][1] = -y[b][0]; w[b][2] = y[b][3]; w[b][3] = -y[b][2];
     }
+    if constexpr (P::kGauss) {
+      // entries h = 0, 1 of the float4: (re, im) at components 2h, 2h + 1
+      T sx[2][2], sy[2][2];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+      for (int h = 0; h < 2; ++h) {
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a) sx[a][h] = x[a][2 * h] - x[a][2 * h + 1];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          cre[a][b] = P::mfma(x[a][q], y[b][q], cre[a][b]);
-          cim[a][b] = P::mfma(x[a][q], w[b][q], cim[a][b]);
-        }
+        for (int b = 0; b < 2; ++b) sy[b][h] = y[b][2 * h] + y[b][2 * h + 1];
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            cre[a][b] = P::mfma(x[a][2 * h], y[b][2 * h], cre[a][b]);
+            cim[a][b] = P::mfma(x[a][2 * h + 1], y[b][2 * h + 1], cim[a][b]);
+            c3[a][b] = P::mfma(sx[a][h], sy[b][h], c3[a][b]);
+          }
+    } else {
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        w[b][0] = y[b][1]; w[b][1] = -y[b][0]; w[b][2] = y[b][3]; w[b][3] = -y[b][2];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            cre[a][b] = P::mfma(x[a][q], y[b][q], cre[a][b]);
+            cim[a][b] = P::mfma(x[a][q], w[b][q], cim[a][b]);
+          }
+    }
     if (P::kFlush && ++steps == kMFlush) {
       steps = 0;
 #pragma unroll
@@ -251,27 +315,59 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
           }
     }
   };
+  // Loads of one step.  fast: the step lies wholly inside [kb, ke) — four
+  // unconditional 16-byte loads.  fetch: any step (per-entry bounds, zero
+  // beyond ke, no access there).
+  auto fast = [&](long long kw0, float4 (&av)[2], float4 (&bv)[2]) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a) av[a] = *reinterpret_cast<const float4*>(ap[a] + kw0 + 2 * g);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) bv[b] = *reinterpret_cast<const float4*>(bp[b] + kw0 + 2 * g);
+  };
+  auto fetch = [&](long long kw0, float4 (&av)[2], float4 (&bv)[2]) {
+    if (kw0 + kMStep <= ke) {
+      fast(kw0, av, bv);
+    } else {
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = load_c2(ap[a], kw0 + 2 * g, ke);
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bv[b] = load_c2(bp[b], kw0 + 2 * g, ke);
+    }
+  };
   constexpr int D = kMPrefetch;
   float4 ra_buf[D][2], rb_buf[D][2];
+  const long long span = ke > kb ? ke - kb : 0;
+  const long long nst = (span + kMStep - 1) / kMStep;  // steps, the last maybe partial
+  const long long nfull = span / kMStep;               // steps wholly in range
+  long long t = 0;
+  if (nfull >= 2 * D) {
+    // steady state, branch-free from the prologue on: slot d is consumed,
+    // then refilled with step t+d+D (all full).  Per-lane bounds branches
+    // here made the compiler wait for every load in flight at each join, and
+    // a refill before the consume landed in a temporary copied back after
+    // the load returned.  The compiler is left to schedule across steps:
+    // pinning each step in place (sched_barrier) measured slower.
 #pragma unroll
-  for (int d = 0; d < D; ++d) {
-    const long long kk = kb + d * kMStep + 2 * g;
+    for (int d = 0; d < D; ++d) fast(kb + d * kMStep, ra_buf[d], rb_buf[d]);
+    for (; t + 2 * D <= nfull; t += D) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a) ra_buf[d][a] = ra[a] ? load_c2(ap[a], kk, ke) : z4;
+      for (int d = 0; d < D; ++d) {
+        step(ra_buf[d], rb_buf[d]);
+        fast(kb + (t + d + D) * kMStep, ra_buf[d], rb_buf[d]);
+      }
+    }
+  } else {
 #pragma unroll
-    for (int b = 0; b < 2; ++b) rb_buf[d][b] = rb[b] ? load_c2(bp[b], kk, ke) : z4;
+    for (int d = 0; d < D; ++d) fetch(kb + d * kMStep, ra_buf[d], rb_buf[d]);
   }
-  for (long long k = kb; k < ke; k += D * kMStep) {
+  // the remaining (fewer than 2D + 1) steps
+  for (; t < nst; t += D) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      if (k + d * kMStep < ke) {  // wave-uniform
+      if (t + d < nst) {  // wave-uniform
         const float4 av[2] = {ra_buf[d][0], ra_buf[d][1]};
         const float4 bv[2] = {rb_buf[d][0], rb_buf[d][1]};
-        const long long kn = k + (d + D) * kMStep + 2 * g;
-#pragma unroll
-        for (int a = 0; a < 2; ++a) ra_buf[d][a] = ra[a] ? load_c2(ap[a], kn, ke) : z4;
-#pragma unroll
-        for (int b = 0; b < 2; ++b) rb_buf[d][b] = rb[b] ? load_c2(bp[b], kn, ke) : z4;
+        fetch(kb + (t + d + D) * kMStep, ra_buf[d], rb_buf[d]);
         step(av, bv);
       }
     }
@@ -283,9 +379,15 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        v[a][b][q] = make_double2(dre[a][b][q] + (double)cre[a][b][q],
-                                  dim_[a][b][q] + (double)cim[a][b][q]);
+      for (int q = 0; q < 4; ++q) {
+        if constexpr (P::kGauss) {
+          const double p1 = (double)cre[a][b][q], p2 = (double)cim[a][b][q];
+          v[a][b][q] = make_double2(p1 + p2, ((double)c3[a][b][q] - p1) + p2);
+        } else {
+          v[a][b][q] = make_double2(dre[a][b][q] + (double)cre[a][b][q],
+                                    dim_[a][b][q] + (double)cim[a][b][q]);
+        }
+      }
   // fixed-order reduction over the waves: (w0 + w2) + (w1 + w3)
   auto lds_at = [&](int a, int b, int q) { return (16 * a + P::row(g, q)) * kMWaveTile + 16 * b + r; };
   if (wave >= 2)
@@ -317,7 +419,7 @@ __global__ __launch_bounds__(256) void gram_mfma_kernel(PairPtrs pp, int ma, int
         for (int q = 0; q < 4; ++q) red[0][lds_at(a, b, q)] = v[a][b][q];
   __syncthreads();
   if (wave != 0) return;
-  double2* out = part + ((long long)blockIdx.z * gridDim.x + blockIdx.x) * ma * mb;
+  double2* out = part + ((long long)blockIdx.z * nsplit + split) * ma * mb;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -374,10 +476,12 @@ __global__ void sum_partials_kernel(const double2* __restrict__ part, int nsplit
 template <int TI>
 __global__ __launch_bounds__(256) void combine_kernel(const float2* __restrict__ Call, int m_in, int m_out,
                                                       CombinePtrs cp, long long n, float alpha) {
+  // each thread reads all m_in rows of its column before it stores, so with
+  // one row tile (gridDim.y = 1) out may be S itself
   const float2* __restrict__ C = Call + (long long)blockIdx.z * m_in * m_out;
-  const float2* __restrict__ S = cp.s[blockIdx.z];
-  const float2* __restrict__ X = cp.x[blockIdx.z];
-  float2* __restrict__ out = cp.out[blockIdx.z];
+  const float2* S = cp.s[blockIdx.z];
+  const float2* X = cp.x[blockIdx.z];
+  float2* out = cp.out[blockIdx.z];
   const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
   const int i0 = blockIdx.y * TI;
   if (k >= n) return;
@@ -416,21 +520,45 @@ __global__ __launch_bounds__(256) void combine_kernel(const float2* __restrict__
 // vector-ALU kernel:
 //   Re += Cr·Sr − Ci·Si,  Im += Cr·Si + Ci·Sr      (A = Cᵀ 16 rows × 4 j, B = S 4 j × 16 columns)
 // A wave owns 16·NA output rows × 64 columns (NA × 4 tiles of 16×16); a
-// block's four waves take consecutive 64-column ranges.  B operands: lane l
-// reads S[j0 + l/16][c0 + l%16] (16 lanes = 128 B per row); D[i][c] sits in
-// lane c + 16·(i/4), element i%4, so stores are 128-B runs as well.
+// block's four waves take consecutive 64-column ranges.  Lane l = r + 16·g
+// owns the four consecutive columns c0 + 4r .. c0 + 4r + 3, column c0 + 4r + b
+// belonging to tile b: its B operand for rows j0 + g is two 16-byte loads,
+// and D[i][r] of tile b (lane r + 16·(i/4), element i%4) puts the four
+// columns of output row i in one lane, stored as two 16-byte runs.  The rows
+// of S stream through a register ring of kCPrefetch steps of 4 rows.  Every
+// output element is the same MFMA chain in j order whatever the column
+// mapping.  Each wave reads all m_in rows of its columns before it stores,
+// so with one row tile (gridDim.y = 1) out may be S itself.  m_out ≥ 1.
+// vec: n even and S, X, out 16-byte aligned (float4 access); else per element.
+#ifndef EFN_COMBINE_PREFETCH
+#define EFN_COMBINE_PREFETCH 2
+#endif
+constexpr int kCPrefetch = EFN_COMBINE_PREFETCH;
+
 template <int NA>
-__global__ __launch_bounds__(256) void combine_mfma_kernel(const float2* __restrict__ Call, int m_in,
-                                                           int m_out, CombinePtrs cp, long long n,
-                                                           float alpha) {
+__global__ __launch_bounds__(256, 2) void combine_mfma_kernel(const float2* __restrict__ Call, int m_in,
+                                                              int m_out, CombinePtrs cp, long long n,
+                                                              float alpha, int vec) {
   const float2* __restrict__ C = Call + (long long)blockIdx.z * m_in * m_out;
-  const float2* __restrict__ S = cp.s[blockIdx.z];
-  const float2* __restrict__ X = cp.x[blockIdx.z];
-  float2* __restrict__ out = cp.out[blockIdx.z];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float2* S = cp.s[blockIdx.z];
+  const float2* X = cp.x[blockIdx.z];
+  float2* out = cp.out[blockIdx.z];
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
   const long long c0 = ((long long)blockIdx.x * 4 + wave) * 64;
+  const long long cl = c0 + 4 * r;
+  const bool wfull = vec && c0 + 64 <= n;  // wave-uniform: all 64 columns, 16-byte access
+  const bool full = vec && cl + 3 < n;
   const int i0 = blockIdx.y * 16 * NA;
+  // C column of each A row; rows past m_out read column m_out − 1 and count as zero
+  int ic[NA];
+  bool iok[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    const int i = i0 + 16 * a + r;
+    iok[a] = i < m_out;
+    ic[a] = iok[a] ? i : m_out - 1;
+  }
   floatx4 re[NA][4], im[NA][4];
 #pragma unroll
   for (int a = 0; a < NA; ++a)
@@ -439,49 +567,126 @@ __global__ __launch_bounds__(256) void combine_mfma_kernel(const float2* __restr
       re[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
       im[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
-  for (int j0 = 0; j0 < m_in; j0 += 4) {
+  // rows j0 + g of S (this lane's 4 columns) and of C.  fast: rows j0..j0+3
+  // exist and the wave's columns are full — unconditional loads; fetch: any.
+  auto fast = [&](int j0, float4 (&sv)[2], float2 (&cv)[NA]) {
     const int j = j0 + g;
-    float cr[NA], ci[NA];
+    const float2* row = S + (long long)j * n;
+    sv[0] = *reinterpret_cast<const float4*>(row + cl);
+    sv[1] = *reinterpret_cast<const float4*>(row + cl + 2);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) cv[a] = C[(long long)j * m_out + ic[a]];
+  };
+  auto fetch = [&](int j0, float4 (&sv)[2], float2 (&cv)[NA]) {
+    const int j = j0 + g;
+    sv[0] = sv[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) cv[a] = make_float2(0.f, 0.f);
+    if (j >= m_in) return;
+    const float2* row = S + (long long)j * n;
+    if (full) {
+      sv[0] = *reinterpret_cast<const float4*>(row + cl);
+      sv[1] = *reinterpret_cast<const float4*>(row + cl + 2);
+    } else {
+      float2 e[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) e[b] = cl + b < n ? row[cl + b] : make_float2(0.f, 0.f);
+      sv[0] = make_float4(e[0].x, e[0].y, e[1].x, e[1].y);
+      sv[1] = make_float4(e[2].x, e[2].y, e[3].x, e[3].y);
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) cv[a] = C[(long long)j * m_out + ic[a]];
+  };
+  auto step = [&](const float4 (&sv)[2], const float2 (&cv_)[NA]) {
+    const float sr[4] = {sv[0].x, sv[0].z, sv[1].x, sv[1].z};
+    const float si[4] = {sv[0].y, sv[0].w, sv[1].y, sv[1].w};
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
-      const int i = i0 + 16 * a + r;
-      const float2 cv = (j < m_in && i < m_out) ? C[(long long)j * m_out + i] : make_float2(0.f, 0.f);
-      cr[a] = cv.x;
-      ci[a] = cv.y;
-    }
-    float2 sv[4];
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const long long col = c0 + 16 * b + r;
-      sv[b] = (j < m_in && col < n) ? S[(long long)j * n + col] : make_float2(0.f, 0.f);
-    }
-#pragma unroll
-    for (int a = 0; a < NA; ++a)
+      const float2 cv = iok[a] ? cv_[a] : make_float2(0.f, 0.f);
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        re[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(cr[a], sv[b].x, re[a][b], 0, 0, 0);
-        re[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ci[a], sv[b].y, re[a][b], 0, 0, 0);
-        im[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(cr[a], sv[b].y, im[a][b], 0, 0, 0);
-        im[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ci[a], sv[b].x, im[a][b], 0, 0, 0);
+        re[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(cv.x, sr[b], re[a][b], 0, 0, 0);
+        re[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-cv.y, si[b], re[a][b], 0, 0, 0);
+        im[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(cv.x, si[b], im[a][b], 0, 0, 0);
+        im[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(cv.y, sr[b], im[a][b], 0, 0, 0);
       }
+    }
+  };
+  constexpr int D = kCPrefetch;
+  float4 sb[D][2];
+  float2 cb[D][NA];
+  const int nst = (m_in + 3) / 4, nfull = m_in / 4;
+  int t = 0;
+  if (wfull && nfull >= 2 * D) {
+    // branch-free steady state from the prologue on, slot consumed then
+    // refilled with a full step (as in the Gram kernel: D − 1 steps in flight)
+#pragma unroll
+    for (int d = 0; d < D; ++d) {  // in slot order, so slot 0 is the oldest
+      fast(4 * d, sb[d], cb[d]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    for (; t + 2 * D <= nfull; t += D) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        step(sb[d], cb[d]);
+        fast(4 * (t + d + D), sb[d], cb[d]);
+        __builtin_amdgcn_sched_barrier(0);  // steps kept in place: measured faster here
+      }
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d) fetch(4 * d, sb[d], cb[d]);
+  }
+  for (; t < nst; t += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (t + d < nst) {  // wave-uniform
+        const float4 sv[2] = {sb[d][0], sb[d][1]};
+        float2 cv[NA];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) cv[a] = cb[d][a];
+        fetch(4 * (t + d + D), sb[d], cb[d]);
+        step(sv, cv);
+      }
+    }
   }
 #pragma unroll
   for (int a = 0; a < NA; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const long long col = c0 + 16 * b + r;
-      if (col >= n) continue;
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + 16 * a + 4 * g + q;
+      if (i >= m_out) continue;
+      float2 v[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = i0 + 16 * a + 4 * g + q;
-        if (i >= m_out) continue;
-        float2 v = make_float2(re[a][b][q], im[a][b][q]);
+      for (int b = 0; b < 4; ++b) v[b] = make_float2(re[a][b][q], im[a][b][q]);
+      float2* orow = out + (long long)i * n;
+      if (full) {
         if (X != nullptr) {
-          const float2 xv = X[(long long)i * n + col];
-          v.x = fmaf(alpha, xv.x, v.x);
-          v.y = fmaf(alpha, xv.y, v.y);
+          const float2* xrow = X + (long long)i * n;
+          const float4 x0 = *reinterpret_cast<const float4*>(xrow + cl);
+          const float4 x1 = *reinterpret_cast<const float4*>(xrow + cl + 2);
+          const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            v[b].x = fmaf(alpha, xs[2 * b], v[b].x);
+            v[b].y = fmaf(alpha, xs[2 * b + 1], v[b].y);
+          }
         }
-        out[(long long)i * n + col] = v;
+        *reinterpret_cast<float4*>(orow + cl) = make_float4(v[0].x, v[0].y, v[1].x, v[1].y);
+        *reinterpret_cast<float4*>(orow + cl + 2) = make_float4(v[2].x, v[2].y, v[3].x, v[3].y);
+      } else {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const long long col = cl + b;
+          if (col >= n) continue;
+          float2 w = v[b];
+          if (X != nullptr) {
+            const float2 xv = X[(long long)i * n + col];
+            w.x = fmaf(alpha, xv.x, w.x);
+            w.y = fmaf(alpha, xv.y, w.y);
+          }
+          orow[col] = w;
+        }
       }
     }
 }

@@ -272,6 +272,9 @@ int gram_nsplit(int, int, long long n) {
 
 std::atomic<int> g_gram_engine{EFN_GRAM_MFMA_F64};
 std::atomic<int> g_combine_mfma{1};
+// largest m_out for which combine runs as one row tile on either engine
+// (vector ALU: 32 rows per tile; matrix cores: up to 64), so out may be S
+constexpr int kCombineInPlaceMax = 32;
 
 int rowreduce_nsplit(long long n) {
   long long ns = (n + 65535) / 65536;
@@ -758,11 +761,11 @@ int gram_multi_impl(int nprob, const void* const* A, const void* const* B, int m
     hipLaunchKernelGGL(efn::gram_partial_kernel, grid, dim3(256), 0, s, pp, ma, mb, tiles_j, (long long)n,
                        kper, (double2*)work);
   else if (engine == EFN_GRAM_MFMA_F64)
-    hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF64>, grid, dim3(256), 0, s, pp, ma, mb, tiles_j,
-                       (long long)n, kper, (double2*)work);
+    hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF64>, dim3((unsigned)(ns * ntile), 1, (unsigned)nprob),
+                       dim3(256), 0, s, pp, ma, mb, tiles_j, ntile, ns, (long long)n, kper, (double2*)work);
   else
-    hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF32>, grid, dim3(256), 0, s, pp, ma, mb, tiles_j,
-                       (long long)n, kper, (double2*)work);
+    hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF32>, dim3((unsigned)(ns * ntile), 1, (unsigned)nprob),
+                       dim3(256), 0, s, pp, ma, mb, tiles_j, ntile, ns, (long long)n, kper, (double2*)work);
   EFN_LAUNCH_CHECK();
   const long long count = (long long)ma * mb;
   hipLaunchKernelGGL(efn::sum_partials_kernel, dim3((unsigned)((count + 255) / 256), (unsigned)nprob),
@@ -798,24 +801,37 @@ int efn_combine_multi(int nprob, const void* C, const void* const* S, int m_in, 
   if (!out || (m_in > 0 && (!C || !S))) return fail(EFN_EINVAL, "NULL pointer");
   efn::CombinePtrs cp;
   memset(&cp, 0, sizeof(cp));
+  auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
+  int vec = (n & 1) == 0;
   for (int i = 0; i < nprob; ++i) {
     if (!out[i] || (m_in > 0 && !S[i])) return fail(EFN_EINVAL, "NULL block pointer");
-    if (m_in > 0 && out[i] == S[i]) return fail(EFN_EINVAL, "out must not alias S");
+    // in place needs one row tile: every reader of a column is its writer
+    if (m_in > 0 && out[i] == S[i] && m_out > kCombineInPlaceMax)
+      return fail(EFN_EINVAL, "out may alias S only for m_out <= " + std::to_string(kCombineInPlaceMax));
     cp.s[i] = m_in > 0 ? (const float2*)S[i] : nullptr;
     cp.x[i] = X ? (const float2*)X[i] : nullptr;
     cp.out[i] = (float2*)out[i];
+    if (!al16(cp.out[i]) || (cp.s[i] && !al16(cp.s[i])) || (cp.x[i] && !al16(cp.x[i]))) vec = 0;
   }
   hipStream_t s = (hipStream_t)stream;
   const unsigned gx = (unsigned)((n + 255) / 256);
-  // matrix cores: 64 columns per wave, 16·NA output rows per block row
+  // matrix cores: 64 columns per wave, 16·NA output rows per block row (one
+  // row tile up to 64 outputs, so S is read once)
   if (g_combine_mfma.load() && m_out > 8) {
     const unsigned gxm = (unsigned)((n + 255) / 256);
+    const float2* Cc = (const float2*)C;
     if (m_out <= 16)
-      hipLaunchKernelGGL((efn::combine_mfma_kernel<1>), dim3(gxm, 1, nprob), dim3(256), 0, s,
-                         (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
+      hipLaunchKernelGGL((efn::combine_mfma_kernel<1>), dim3(gxm, 1, nprob), dim3(256), 0, s, Cc, m_in,
+                         m_out, cp, (long long)n, alpha, vec);
+    else if (m_out <= 32)
+      hipLaunchKernelGGL((efn::combine_mfma_kernel<2>), dim3(gxm, 1, nprob), dim3(256), 0, s, Cc, m_in,
+                         m_out, cp, (long long)n, alpha, vec);
+    else if (m_out <= 48)
+      hipLaunchKernelGGL((efn::combine_mfma_kernel<3>), dim3(gxm, 1, nprob), dim3(256), 0, s, Cc, m_in,
+                         m_out, cp, (long long)n, alpha, vec);
     else
-      hipLaunchKernelGGL((efn::combine_mfma_kernel<2>), dim3(gxm, (m_out + 31) / 32, nprob), dim3(256),
-                         0, s, (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
+      hipLaunchKernelGGL((efn::combine_mfma_kernel<4>), dim3(gxm, (m_out + 63) / 64, nprob), dim3(256),
+                         0, s, Cc, m_in, m_out, cp, (long long)n, alpha, vec);
     EFN_LAUNCH_CHECK();
     return EFN_OK;
   }

@@ -95,6 +95,11 @@ def gram_herm(A: torch.Tensor, B: torch.Tensor | None = None) -> torch.Tensor:
     return gram_multi([A.contiguous()], [B.contiguous()], hermitian=True)[0]
 
 
+# combine may write over its own input block (out is S) up to this many output
+# rows: one row tile, each column read in full before it is written
+COMBINE_IN_PLACE_MAX = 32
+
+
 def combine_into(C: torch.Tensor, S: torch.Tensor, out: torch.Tensor, X=None,
                  alpha: float = 0.0) -> torch.Tensor:
     """out_i = alpha·X_i + Σ_j C[j, i]·S_j with S one contiguous block (m_in rows)."""

@@ -20,10 +20,10 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .linalg import (DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_DROP, apply_chunked, combine,
-                     combine_into, deflate, deflate_chunk_rows, eigh, gram, gram_herm, normalize_rows,
-                     ortho_block, row_axpby, row_axpby_norm_strided, rowdot, rownorm, rownorm2,
-                     rowstats_strided, svqb_coeffs)
+from .linalg import (COMBINE_IN_PLACE_MAX, DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_DROP, apply_chunked,
+                     combine, combine_into, deflate, deflate_chunk_rows, eigh, gram, gram_herm,
+                     normalize_rows, ortho_block, row_axpby, row_axpby_norm_strided, rowdot, rownorm,
+                     rownorm2, rowstats_strided, svqb_coeffs)
 
 __all__ = ["gram", "combine", "deflate", "ortho_block", "apply_chunked", "deflate_chunk_rows",
            "SolveStats", "lobpcg_blocks", "lobpcg_blocks_resumable", "_flat",
@@ -58,12 +58,21 @@ def _deflate_inplace(Y, L, nl):
 
 
 def _svqb_into(Y, tmp, drop=SVQB_DROP, passes=2):
-    """SVQB of Y (two sweeps), result written back into Y; tmp has Y's shape."""
+    """SVQB of Y (two sweeps), result written back into Y; tmp has Y's shape
+    (unused when the combine can run in place)."""
     for _ in range(passes):
         Cw = svqb_coeffs(gram_herm(Y), drop)
-        combine_into(Cw, Y, tmp)
-        Y.copy_(tmp)
+        _combine_back(Cw, Y, tmp)
     return Y
+
+
+def _combine_back(C, Y, tmp):
+    """Y ← Cᵀ·Y: in place up to COMBINE_IN_PLACE_MAX rows, else through tmp."""
+    if int(Y.shape[0]) <= COMBINE_IN_PLACE_MAX:
+        combine_into(C, Y, Y)
+    else:
+        combine_into(C, Y, tmp)
+        Y.copy_(tmp)
 
 
 def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
@@ -105,8 +114,10 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
     # fixed device buffers
     S = torch.empty((3 * m,) + shape, dtype=torch.complex64, device=dev)    # [X; W; P]
     HS = torch.empty_like(S)                                                 # [ΘX; ΘW; ΘP]
-    T1 = torch.empty((m,) + shape, dtype=torch.complex64, device=dev)
-    T2 = torch.empty_like(T1)
+    # [T1; T2]: the new X and P rows of one Rayleigh–Ritz update come from one
+    # combine over S (2m outputs, S read once)
+    T12 = torch.empty((2 * m,) + shape, dtype=torch.complex64, device=dev)
+    T1, T2 = T12[:m], T12[m:]
     X, W, P = S[:m], S[m:2 * m], S[2 * m:]
     HX = HS[:m]
     n = 2 * G ** 3  # complex entries per vector
@@ -182,8 +193,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                     scale = torch.where(keepp, 1.0 / torch.sqrt(torch.where(keepp, wp,
                                                                             torch.ones_like(wp))),
                                         torch.zeros_like(wp))
-                    combine_into(Vp * scale[None, :], P, T1)
-                    P.copy_(T1)
+                    _combine_back(Vp * scale[None, :], P, T1)
             nb = (3 if have_p else 2) * m
             Sb, HSb = S[:nb], HS[:nb]
             op.theta_into(Sb[m:], HSb[m:])
@@ -194,11 +204,10 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             penalty = max(RR_DEAD_PENALTY, 100.0 * float(torch.diagonal(A).abs().max().item()))
             A = A + torch.diag(torch.where(dead, penalty, 0.0).to(A.dtype))
             wA, VA = eigh(A)
-            Cx = VA[:, :m].contiguous()
-            combine_into(Cx, Sb, T1)                   # Xn
+            Cx = VA[:, :m]
             Cp = Cx.clone()
-            Cp[:m] = 0
-            combine_into(Cp, Sb, T2)                   # Pn (X rows zeroed)
+            Cp[:m] = 0                                 # Pn: X rows zeroed
+            combine_into(torch.cat([Cx, Cp], dim=1), Sb, T12)   # [Xn; Pn]
             _deflate_inplace(T1, L, nl)
             normalize_rows(T1, floor=1e-20, out=X)
             normalize_rows(T2, floor=1e-20, out=P)
@@ -244,7 +253,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
               f"recovered out of order; re-sorting.", flush=True)
     order = np.argsort(locked_vals, kind="stable")
     locked = locked_buf[:n_locked_now][torch.as_tensor(order, device=dev)]
-    del S, HS, T1, T2, locked_buf
+    del S, HS, T12, T1, T2, locked_buf
     gc.collect()
     return locked_vals[order][:nev], locked, stats
 

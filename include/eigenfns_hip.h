@@ -163,7 +163,8 @@ int efn_gram_multi_herm(int nprob, const void* const* A_c64, const void* const* 
 
 /* Linear combination out_i = alpha·X_i + Σ_j C[j][i]·S_j (solver.py:49-57,
  * :106, :125): C is m_in×m_out complex64 on the device, S m_in rows, X
- * (optional, NULL = none) and out m_out rows of n complex64.  out must not
+ * (optional, NULL = none) and out m_out rows of n complex64.  out may be S
+ * itself (in place) when m_out <= 32 and m_out <= m_in; otherwise it must not
  * overlap S. */
 int efn_combine(const void* C_c64, int m_in, int m_out, const void* S_c64, int64_t n,
                 const void* X_c64, float alpha, void* out_c64, void* stream);
@@ -215,7 +216,7 @@ int efn_row_axpby_norm_strided(const void* X_c64, int64_t stride_x, const double
                                int m, int64_t n, void* out_c64, int64_t stride_out,
                                double* norm2_c128, void* work, size_t work_bytes, void* stream);
 
-/* out_i = a[i]·X_i + b[i]·Y_i with per-row FP64 coefficients on the device/* out_i = a[i]·X_i + b[i]·Y_i with per-row FP64 coefficients on the device
+/* out_i = a[i]·X_i + b[i]·Y_i with per-row FP64 coefficients on the device
  * (Y, b may be NULL).  R = HX − λX (solver.py:229), row scaling (:306-307). */
 int efn_row_axpby(const void* X_c64, const double* a_dev, const void* Y_c64, const double* b_dev,
                   int m, int64_t n, void* out_c64, void* stream);

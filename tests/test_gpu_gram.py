@@ -134,7 +134,8 @@ def test_combine_engines_match_complex128(engine):
     try:
         rng = np.random.default_rng(17)
         for m_in, m_out, n in [(3, 5, 700), (96, 12, 5000), (97, 32, 8193), (40, 40, 4096),
-                               (96, 96, 2 * 16 ** 3)]:
+                               (96, 96, 2 * 16 ** 3), (72, 48, 2 * 16 ** 3 + 6), (72, 47, 1030),
+                               (64, 64, 4099), (90, 61, 2 * 12 ** 3)]:
             S = _cblock(rng, m_in, n)
             Cm = (rng.standard_normal((m_in, m_out)) + 1j * rng.standard_normal((m_in, m_out))).astype(np.complex64)
             X = _cblock(rng, m_out, n)
@@ -150,5 +151,41 @@ def test_combine_engines_match_complex128(engine):
             for o in outs:
                 assert np.abs(o.cpu().numpy() - plain).max() < 2e-5 * np.abs(plain).max()
             assert torch.equal(outs[0], outs[1])
+    finally:
+        combine_engine(prev)
+
+
+@pytest.mark.parametrize("engine", ["valu", "mfma"])
+def test_combine_in_place_and_unaligned_rows(engine):
+    """out may be S itself for m_out <= 32 (one row tile: every column is read
+    in full before it is written) — same values as into a fresh buffer; larger
+    m_out in place is refused.  Rows starting 8 bytes off a 16-byte boundary
+    take the per-element path with the same results."""
+    import torch
+    from eigenfns_amd.linalg import combine_engine, combine_into
+    prev = combine_engine(engine)
+    try:
+        rng = np.random.default_rng(23)
+        for m, n in [(24, 2 * 16 ** 3), (32, 4098), (13, 777), (32, 2 * 8 ** 3 + 1)]:
+            S = _cblock(rng, m, n)
+            Cm = (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))).astype(np.complex64)
+            Sd, Cd = torch.from_numpy(S).cuda(), torch.from_numpy(Cm).cuda()
+            fresh = torch.empty_like(Sd)
+            combine_into(Cd, Sd, fresh)
+            combine_into(Cd, Sd, Sd)
+            assert torch.equal(Sd, fresh), (m, n)
+            ref = Cm.astype(np.complex128).T @ S.astype(np.complex128)
+            assert np.abs(fresh.cpu().numpy() - ref).max() < 2e-5 * np.abs(ref).max()
+            # the same rows at an 8-byte offset (no 16-byte access possible)
+            flat = torch.zeros(m * n + 1, dtype=torch.complex64, device="cuda")
+            Su = flat[1:].view(m, n)
+            Su.copy_(torch.from_numpy(S).cuda())
+            ou = torch.zeros(m * n + 1, dtype=torch.complex64, device="cuda")[1:].view(m, n)
+            combine_into(Cd, Su, ou)
+            assert torch.equal(ou, fresh), (m, n)
+        S = torch.from_numpy(_cblock(rng, 40, 1000)).cuda()
+        C = torch.zeros((40, 40), dtype=torch.complex64, device="cuda")
+        with pytest.raises(ValueError):
+            combine_into(C, S, S)
     finally:
         combine_engine(prev)
