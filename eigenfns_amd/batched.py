@@ -74,6 +74,17 @@ class _Slot:
         self.n_lock = 0
 
 
+def _runs(idx):
+    """Maximal runs of consecutive slot indices as (first, count), ascending."""
+    out = []
+    for s in sorted(idx):
+        if out and out[-1][0] + out[-1][1] == s:
+            out[-1][1] += 1
+        else:
+            out.append([s, 1])
+    return [tuple(r) for r in out]
+
+
 class BatchedLOBPCG:
     """Lockstep LOBPCG over up to `nslots` (≤ 32) problems of grid size G.
 
@@ -308,26 +319,18 @@ class BatchedLOBPCG:
         combine_multi(torch.cat([Ct, Cp], dim=2), Sbs,
                       [self.T12[2 * s * m:2 * (s + 1) * m] for s in idx])
         self._deflate(idx, self._t1)
-        # normalised rows back into X and P of every slot (rows of slots not
-        # iterating hold nothing that is read again before they are rebuilt)
-        nrm = rowreduce_strided(0, self.T12, None, 2 * m, 2 * m, 2 * m, 2 * self.nslots * m,
-                                self.n).real.view(self.nslots, 2, m)
-        nrm1, nrm2 = nrm[:, 0].reshape(-1), nrm[:, 1].reshape(-1)
-        live = torch.zeros(self.nslots * m, dtype=torch.bool, device=self.S.device)
-        for s in idx:
-            live[s * m:(s + 1) * m] = True
-        a1 = torch.where(live, 1.0 / torch.clamp(torch.sqrt(nrm1), min=1e-20), torch.ones_like(nrm1))
-        a2 = torch.where(live, 1.0 / torch.clamp(torch.sqrt(nrm2), min=1e-20), torch.ones_like(nrm2))
-        # X and P rows of non-iterating slots are preserved: copied through T12 first
-        keep = [s for s in range(self.nslots) if s not in set(idx)]
-        if keep:
-            copy_multi([self._X(s) for s in keep], [self._t1(s) for s in keep])
-            copy_multi([self._P(s) for s in keep], [self._t2(s) for s in keep])
-        row_axpby_strided(self.T12, 2 * m, a1, None, 2 * m, None, m, self.nslots * m, self.n,
-                          self.S, 3 * m)
-        P0 = self.S[2 * m:]
-        row_axpby_strided(self.T12[m:], 2 * m, a2, None, 2 * m, None, m, self.nslots * m, self.n,
-                          P0, 3 * m)
+        # normalised rows back into X and P (solver.py:306-307), over each run of
+        # consecutive iterating slots; other slots' rows are not touched
+        for s0, cnt in _runs(idx):
+            T = self.T12[2 * s0 * m:2 * (s0 + cnt) * m]
+            nrm = rowreduce_strided(0, T, None, 2 * m, 2 * m, 2 * m, 2 * cnt * m,
+                                    self.n).real.view(cnt, 2, m)
+            a1 = 1.0 / torch.clamp(torch.sqrt(nrm[:, 0].reshape(-1)), min=1e-20)
+            a2 = 1.0 / torch.clamp(torch.sqrt(nrm[:, 1].reshape(-1)), min=1e-20)
+            S0 = self.S[3 * s0 * m:]
+            row_axpby_strided(T, 2 * m, a1, None, 2 * m, None, m, cnt * m, self.n, S0, 3 * m)
+            row_axpby_strided(T[m:], 2 * m, a2, None, 2 * m, None, m, cnt * m, self.n,
+                              S0[2 * m:], 3 * m)
         t = self._tick("update_XP", t)
         self._theta(idx, self.S, 0, self.HS, 0, 3 * m, 3 * m, m)
         t = self._tick("theta_X", t)

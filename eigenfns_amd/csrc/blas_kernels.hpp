@@ -207,28 +207,20 @@ __device__ __forceinline__ void gram_block_of(int L, int ntile, int nsplit, int&
 
 // grid: (splits·tiles, 1, problems) with 32×32 tiles; kper a multiple of
 // 4·kMStep; rows 16-byte aligned (n even, aligned bases).
-template <class P>
-__global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, int mb, int tiles_j,
-                                                        int ntile, int nsplit, long long n,
-                                                        long long kper, double2* __restrict__ part) {
+// One block's 32×32 tile over its k range.  NA × NB of the 2 × 2 sub-tiles
+// of 16×16 hold rows < ma / columns < mb; SKIP10 drops sub-tile (1, 0) of a
+// diagonal tile of a Hermitian product (strictly below the diagonal, never
+// read).  Only the kept sub-tiles are loaded and multiplied.
+template <class P, int NA, int NB, bool SKIP10>
+__device__ __forceinline__ void gram_tile(const float2* __restrict__ A, const float2* __restrict__ B,
+                                          int ma, int mb, int i0, int j0, long long n, long long kb,
+                                          long long ke, int wave, double2 (*red)[kMWaveTile * kMWaveTile],
+                                          double2* __restrict__ out) {
   typedef typename P::in_t T;
   typedef typename P::acc_t Acc;
-  __shared__ double2 red[2][kMWaveTile * kMWaveTile];
-  const float2* __restrict__ A = pp.a[blockIdx.z];
-  const float2* __restrict__ B = pp.b[blockIdx.z];
-  // wave index through readfirstlane: the compiler then keeps the k range in
-  // scalar registers and the step checks below are scalar branches
-  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   const int r = lane & 15, g = lane >> 4;
-  int split, tile, ti_, tj_;
-  gram_block_of(blockIdx.x, ntile, nsplit, split, tile);
-  tile_of(tile, tiles_j, ti_, tj_);
-  const int i0 = ti_ * kMWaveTile, j0 = tj_ * kMWaveTile;
-  const long long kb_blk = (long long)split * kper;
-  const long long ke_blk = kb_blk + kper < n ? kb_blk + kper : n;
-  const long long kw = kper / 4;
-  const long long kb = kb_blk + wave * kw;
-  const long long ke = kb + kw < ke_blk ? kb + kw : ke_blk;
+  auto keep = [](int a, int b) { return a < NA && b < NB && !(SKIP10 && a == 1 && b == 0); };
   const bool ra[2] = {i0 + r < ma, i0 + 16 + r < ma};
   const bool rb[2] = {j0 + r < mb, j0 + 16 + r < mb};
   const float2* ap[2] = {A + (long long)(ra[0] ? i0 + r : 0) * n, A + (long long)(ra[1] ? i0 + 16 + r : 0) * n};
@@ -250,20 +242,18 @@ __global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, 
     }
   int steps = 0;
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  // register ring of kMPrefetch steps: each step's loads are issued
-  // kMPrefetch − 1 steps (of 24 or 32 MFMAs each) before they are consumed
   // rows past ma / mb were loaded from a clamped (valid) row and count as zero
   auto step = [&](const float4 (&av_)[2], const float4 (&bv_)[2]) {
     T x[2][4], y[2][4], w[2][4];
-    const float4 av[2] = {ra[0] ? av_[0] : z4, ra[1] ? av_[1] : z4};
-    const float4 bv[2] = {rb[0] ? bv_[0] : z4, rb[1] ? bv_[1] : z4};
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      x[a][0] = (T)av[a].x; x[a][1] = (T)av[a].y; x[a][2] = (T)av[a].z; x[a][3] = (T)av[a].w;
+    for (int a = 0; a < NA; ++a) {
+      const float4 av = ra[a] ? av_[a] : z4;
+      x[a][0] = (T)av.x; x[a][1] = (T)av.y; x[a][2] = (T)av.z; x[a][3] = (T)av.w;
     }
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      y[b][0] = (T)bv[b].x; y[b][1] = (T)bv[b].y; y[b][2] = (T)bv[b].z; y[b][3] = (T)bv[b].w;
+    for (int b = 0; b < NB; ++b) {
+      const float4 bv = rb[b] ? bv_[b] : z4;
+      y[b][0] = (T)bv.x; y[b][1] = (T)bv.y; y[b][2] = (T)bv.z; y[b][3] = (T)bv.w;
     }
     if constexpr (P::kGauss) {
       // entries h = 0, 1 of the float4: (re, im) at components 2h, 2h + 1
@@ -271,31 +261,33 @@ __global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, 
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a) sx[a][h] = x[a][2 * h] - x[a][2 * h + 1];
+        for (int a = 0; a < NA; ++a) sx[a][h] = x[a][2 * h] - x[a][2 * h + 1];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) sy[b][h] = y[b][2 * h] + y[b][2 * h + 1];
+        for (int b = 0; b < NB; ++b) sy[b][h] = y[b][2 * h] + y[b][2 * h + 1];
       }
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < NA; ++a)
 #pragma unroll
-          for (int b = 0; b < 2; ++b) {
+          for (int b = 0; b < NB; ++b) {
+            if (!keep(a, b)) continue;
             cre[a][b] = P::mfma(x[a][2 * h], y[b][2 * h], cre[a][b]);
             cim[a][b] = P::mfma(x[a][2 * h + 1], y[b][2 * h + 1], cim[a][b]);
             c3[a][b] = P::mfma(sx[a][h], sy[b][h], c3[a][b]);
           }
     } else {
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
+      for (int b = 0; b < NB; ++b) {
         w[b][0] = y[b][1]; w[b][1] = -y[b][0]; w[b][2] = y[b][3]; w[b][3] = -y[b][2];
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < NA; ++a)
 #pragma unroll
-          for (int b = 0; b < 2; ++b) {
+          for (int b = 0; b < NB; ++b) {
+            if (!keep(a, b)) continue;
             cre[a][b] = P::mfma(x[a][q], y[b][q], cre[a][b]);
             cim[a][b] = P::mfma(x[a][q], w[b][q], cim[a][b]);
           }
@@ -303,9 +295,9 @@ __global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, 
     if (P::kFlush && ++steps == kMFlush) {
       steps = 0;
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             dre[a][b][q] += (double)cre[a][b][q];
@@ -315,23 +307,23 @@ __global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, 
           }
     }
   };
-  // Loads of one step.  fast: the step lies wholly inside [kb, ke) — four
-  // unconditional 16-byte loads.  fetch: any step (per-entry bounds, zero
-  // beyond ke, no access there).
+  // Loads of one step (rows of kept sub-tiles only).  fast: the step lies
+  // wholly inside [kb, ke) — unconditional 16-byte loads.  fetch: any step
+  // (per-entry bounds, zero beyond ke, no access there).
   auto fast = [&](long long kw0, float4 (&av)[2], float4 (&bv)[2]) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a) av[a] = *reinterpret_cast<const float4*>(ap[a] + kw0 + 2 * g);
+    for (int a = 0; a < NA; ++a) av[a] = *reinterpret_cast<const float4*>(ap[a] + kw0 + 2 * g);
 #pragma unroll
-    for (int b = 0; b < 2; ++b) bv[b] = *reinterpret_cast<const float4*>(bp[b] + kw0 + 2 * g);
+    for (int b = 0; b < NB; ++b) bv[b] = *reinterpret_cast<const float4*>(bp[b] + kw0 + 2 * g);
   };
   auto fetch = [&](long long kw0, float4 (&av)[2], float4 (&bv)[2]) {
     if (kw0 + kMStep <= ke) {
       fast(kw0, av, bv);
     } else {
 #pragma unroll
-      for (int a = 0; a < 2; ++a) av[a] = load_c2(ap[a], kw0 + 2 * g, ke);
+      for (int a = 0; a < NA; ++a) av[a] = load_c2(ap[a], kw0 + 2 * g, ke);
 #pragma unroll
-      for (int b = 0; b < 2; ++b) bv[b] = load_c2(bp[b], kw0 + 2 * g, ke);
+      for (int b = 0; b < NB; ++b) bv[b] = load_c2(bp[b], kw0 + 2 * g, ke);
     }
   };
   constexpr int D = kMPrefetch;
@@ -375,9 +367,9 @@ __global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, 
   // this wave's tile: element (P::row(g, q), r) of sub-tile (a, b)
   double2 v[2][2][4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < NA; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if constexpr (P::kGauss) {
@@ -392,19 +384,21 @@ __global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, 
   auto lds_at = [&](int a, int b, int q) { return (16 * a + P::row(g, q)) * kMWaveTile + 16 * b + r; };
   if (wave >= 2)
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NA; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) red[wave - 2][lds_at(a, b, q)] = v[a][b][q];
+        for (int q = 0; q < 4; ++q)
+          if (keep(a, b)) red[wave - 2][lds_at(a, b, q)] = v[a][b][q];
   __syncthreads();
   if (wave < 2)
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NA; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
+          if (!keep(a, b)) continue;
           const double2 o = red[wave][lds_at(a, b, q)];
           v[a][b][q].x += o.x;
           v[a][b][q].y += o.y;
@@ -412,24 +406,62 @@ __global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, 
   __syncthreads();
   if (wave == 1)
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NA; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < NB; ++b)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) red[0][lds_at(a, b, q)] = v[a][b][q];
+        for (int q = 0; q < 4; ++q)
+          if (keep(a, b)) red[0][lds_at(a, b, q)] = v[a][b][q];
   __syncthreads();
   if (wave != 0) return;
-  double2* out = part + ((long long)blockIdx.z * nsplit + split) * ma * mb;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < NA; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
+        if (!keep(a, b)) continue;
         const int i = i0 + 16 * a + P::row(g, q), j = j0 + 16 * b + r;
         const double2 o = red[0][lds_at(a, b, q)];
         if (i < ma && j < mb) out[(long long)i * mb + j] = make_double2(v[a][b][q].x + o.x, v[a][b][q].y + o.y);
       }
+}
+
+template <class P>
+__global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, int mb, int tiles_j,
+                                                        int ntile, int nsplit, long long n,
+                                                        long long kper, double2* __restrict__ part) {
+  __shared__ double2 red[2][kMWaveTile * kMWaveTile];
+  const float2* __restrict__ A = pp.a[blockIdx.z];
+  const float2* __restrict__ B = pp.b[blockIdx.z];
+  // wave index through readfirstlane: the compiler then keeps the k range in
+  // scalar registers and the step checks are scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int split, tile, ti_, tj_;
+  gram_block_of(blockIdx.x, ntile, nsplit, split, tile);
+  tile_of(tile, tiles_j, ti_, tj_);
+  const int i0 = ti_ * kMWaveTile, j0 = tj_ * kMWaveTile;
+  const long long kb_blk = (long long)split * kper;
+  const long long ke_blk = kb_blk + kper < n ? kb_blk + kper : n;
+  const long long kw = kper / 4;
+  const long long kb = kb_blk + wave * kw;
+  const long long ke = kb + kw < ke_blk ? kb + kw : ke_blk;
+  double2* out = part + ((long long)blockIdx.z * nsplit + split) * ma * mb;
+  // sub-tile shape of this tile (block-uniform)
+  const bool fa = i0 + 16 < ma, fb = j0 + 16 < mb;
+  const bool diag = tiles_j < 0 && ti_ == tj_;
+  if (fa && fb) {
+    if (diag)
+      gram_tile<P, 2, 2, true>(A, B, ma, mb, i0, j0, n, kb, ke, wave, red, out);
+    else
+      gram_tile<P, 2, 2, false>(A, B, ma, mb, i0, j0, n, kb, ke, wave, red, out);
+  } else if (fa) {
+    gram_tile<P, 2, 1, false>(A, B, ma, mb, i0, j0, n, kb, ke, wave, red, out);
+  } else if (fb) {
+    gram_tile<P, 1, 2, false>(A, B, ma, mb, i0, j0, n, kb, ke, wave, red, out);
+  } else {
+    gram_tile<P, 1, 1, false>(A, B, ma, mb, i0, j0, n, kb, ke, wave, red, out);
+  }
 }
 
 // zero fill of count double2 (a kernel rather than a memset node, so the
@@ -450,24 +482,35 @@ __global__ void sum_partials_kernel(const double2* __restrict__ part, int nsplit
   if (e >= count) return;
   part += (long long)blockIdx.y * nsplit * count;
   out += (long long)blockIdx.y * count;
-  long long src = e;
-  double sign = 1.0;
-  if (herm) {
-    const int i = (int)(e / mb), j = (int)(e - (long long)i * mb);
-    if (i > j) {
-      src = (long long)j * mb + i;
-      sign = -1.0;
-    } else if (i == j) {
-      sign = 0.0;
+  const int i = (int)(e / mb), j = (int)(e - (long long)i * mb);
+  if (herm && i > j) return;  // written by the thread of (j, i)
+  // four independent chains (split s into chain s % 4 while four remain),
+  // then (c0 + c1) + (c2 + c3): a fixed order that depends on nsplit only
+  double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
+  int s = 0;
+  for (; s + 4 <= nsplit; s += 4) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double2 p = part[(long long)(s + c) * count + e];
+      re[c] += p.x;
+      im[c] += p.y;
     }
   }
-  double re = 0.0, im = 0.0;
-  for (int s = 0; s < nsplit; ++s) {
-    const double2 p = part[(long long)s * count + src];
-    re += p.x;
-    im += p.y;
+  for (; s < nsplit; ++s) {
+    const double2 p = part[(long long)s * count + e];
+    re[0] += p.x;
+    im[0] += p.y;
   }
-  out[e] = make_double2(re, sign * im);
+  const double vr = (re[0] + re[1]) + (re[2] + re[3]);
+  const double vi = (im[0] + im[1]) + (im[2] + im[3]);
+  if (!herm) {
+    out[e] = make_double2(vr, vi);
+  } else if (i == j) {
+    out[e] = make_double2(vr, 0.0);
+  } else {
+    out[e] = make_double2(vr, vi);
+    out[(long long)j * mb + i] = make_double2(vr, -vi);
+  }
 }
 
 // ---------------------------------------------------------------------------
