@@ -124,17 +124,21 @@ __host__ __device__ constexpr int xtile_for(int G) { return G >= EFN_XTILE ? EFN
 
 // Rows per zin/zout block when all three components are in LDS at once: a
 // power of two dividing G² (a block never spans two vectors).  Measured on
-// MI355X (Θ, 32-vector blocks): 4 rows at G ≥ 128 and 8 at 48 ≤ G < 128 beat
+// MI355X (Θ, 32-vector blocks): 4 rows at G ≥ 128 and 8 at 64 ≤ G < 128 beat
 // 8/16 by 7 % at 192³ and 14 % at 96³ — more, smaller blocks per CU hide the
-// HBM latency of the direct stage-1/stage-3 accesses better than fuller ones.
+// HBM latency of the direct stage-1/stage-3 accesses better than fuller ones;
+// below 64, 16 rows beat 8 and 32 by 5-6 % at 48³ and 32³ (64-96 vectors).
 #ifndef EFN_ZROWS_BIG
 #define EFN_ZROWS_BIG 4
 #endif
 #ifndef EFN_ZROWS_MID
 #define EFN_ZROWS_MID 8
 #endif
+#ifndef EFN_ZROWS_SMALL
+#define EFN_ZROWS_SMALL 16
+#endif
 __host__ __device__ constexpr int zrows3_for(int G) {
-  int target = G >= 128 ? EFN_ZROWS_BIG : (G >= 48 ? EFN_ZROWS_MID : 32);
+  int target = G >= 128 ? EFN_ZROWS_BIG : (G >= 64 ? EFN_ZROWS_MID : EFN_ZROWS_SMALL);
   int p = 1;
   while (p * 2 <= target && (G * G) % (p * 2) == 0) p *= 2;
   return p;
