@@ -59,18 +59,25 @@ def _slot_bytes(grid, m, guard, max_nev):
     return (2 * 3 * m + 4 * m + cap) * 2 * grid ** 3 * 8
 
 
-def plan_slots(grids, m, guard, max_nev, max_slots, hbm_budget):
+def plan_slots(grids, m, guard, max_nev, max_slots, hbm_budget, engine_voxels=None):
     """Slots per grid under one HBM budget: every grid gets the same count
     first (slot bytes grow as G³, so equal counts spend the budget in
     proportion to each grid's work per request), then what is left goes to the
-    cheapest grids, up to `max_slots` each."""
+    cheapest grids, up to `max_slots` each.
+
+    engine_voxels (optional) caps slots·G³ per engine, i.e. the work of one
+    lock-step iteration: fewer large-grid requests run at once, each finishing
+    sooner, while the rest wait in the FIFO — a latency (p99) rather than a
+    throughput setting."""
     grids = sorted(set(grids))
+    cap = {g: max_slots if engine_voxels is None else
+           int(min(max_slots, max(1, engine_voxels // g ** 3))) for g in grids}
     cost = {g: _slot_bytes(g, m, guard, max_nev) for g in grids}
     base = int(min(max_slots, max(1, hbm_budget // sum(cost.values()))))
-    slots = {g: base for g in grids}
-    left = hbm_budget - sum(cost[g] * base for g in grids)
+    slots = {g: min(base, cap[g]) for g in grids}
+    left = hbm_budget - sum(cost[g] * slots[g] for g in grids)
     for g in grids:                                    # smallest grid first
-        extra = int(min(max_slots - slots[g], max(0, left) // cost[g]))
+        extra = int(min(cap[g] - slots[g], max(0, left) // cost[g]))
         slots[g] += extra
         left -= extra * cost[g]
     return slots
@@ -90,7 +97,7 @@ def _summary(done, wall, turns, engines):
 
 def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slots=32,
                   hbm_budget=200e9, verbose=False, threads=True, progress=False,
-                  engines_per_grid=1):
+                  engines_per_grid=1, engine_voxels=None):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
 
     threads=True: each grid's engine runs in its own host thread on its own HIP
@@ -103,7 +110,7 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     cache = cache or StructureCache()
     grids = sorted({sp.grid for sp in specs})
     max_nev = max(sp.nev for sp in specs)
-    plan = plan_slots(grids, m, guard, max_nev, max_slots, hbm_budget)
+    plan = plan_slots(grids, m, guard, max_nev, max_slots, hbm_budget, engine_voxels)
     if not threads:
         return _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose)
     import threading

@@ -223,15 +223,17 @@ def run_c4cheb(outers: int | None = None, degree: int | None = None):
             "projected_full_protocol_s": round(full / (vdeg / filt_s), 0)}
 
 
-def run_c5(n: int, rate: float | None, seed: int = 0, plain: bool = True, grids=None):
+def run_c5(n: int, rate: float | None, seed: int = 0, plain: bool = True, grids=None,
+           engine_voxels=None):
     from .serving import StructureCache, serve_batched, serve_plain
     specs = wl.request_stream(n, seed, rate=rate, grids=grids or wl.C5["grids"])
     cache = StructureCache()
     for sp in specs:
         cache.get(sp.n_points, sp.grid)
-    res, summ = serve_batched(specs, cache, max_slots=32, progress=True)
+    res, summ = serve_batched(specs, cache, max_slots=32, progress=True,
+                              engine_voxels=engine_voxels)
     out = {"case": "C5", "requests": n, "rate": rate, "grids": list(grids or wl.C5["grids"]),
-           "batched": summ}
+           "engine_voxels": engine_voxels, "batched": summ}
     if plain:
         pres, psumm = serve_plain(specs, cache, progress=True)
         by = {r.rid: r.vals for r in res}
@@ -252,6 +254,8 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=200)
     ap.add_argument("--rate", type=float, default=None)
     ap.add_argument("--grids", default=None)
+    ap.add_argument("--engine-voxels", type=float, default=None,
+                    help="C5: cap slots x G^3 per engine (latency setting)")
     args = ap.parse_args(argv)
     if args.case == "c1":
         out = run_c1()
@@ -265,7 +269,8 @@ def main(argv=None):
         out = run_c4cheb(args.outers, args.degree)
     else:
         grids = tuple(int(g) for g in args.grids.split(",")) if args.grids else None
-        out = run_c5(args.n, args.rate, plain=not args.no_plain, grids=grids)
+        out = run_c5(args.n, args.rate, plain=not args.no_plain, grids=grids,
+                     engine_voxels=args.engine_voxels)
     print(json.dumps(out, default=float), flush=True)
 
 
