@@ -31,9 +31,9 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .linalg import (COMBINE_IN_PLACE_MAX, RR_DEAD_PENALTY, SVQB_DROP, combine_multi, copy_multi,
-                     eigh_batched_host, gram_multi, rowreduce_strided, row_axpby_strided,
-                     row_axpby_norm_strided, rowstats_strided, MAX_PROBLEMS)
+from .linalg import (COMBINE_IN_PLACE_MAX, RR_DEAD_PENALTY, SVQB_DROP, combine_multi,
+                     combine_multi_split, copy_multi, eigh_batched_host, gram_multi,
+                     rowreduce_strided, row_axpby_norm_strided, rowstats_strided, MAX_PROBLEMS)
 from .operator import MaxwellOperator, precondition_multi, theta_multi
 
 
@@ -72,17 +72,6 @@ class _Slot:
         self.phase = "init"
         self.it = 0
         self.n_lock = 0
-
-
-def _runs(idx):
-    """Maximal runs of consecutive slot indices as (first, count), ascending."""
-    out = []
-    for s in sorted(idx):
-        if out and out[-1][0] + out[-1][1] == s:
-            out[-1][1] += 1
-        else:
-            out.append([s, 1])
-    return [tuple(r) for r in out]
 
 
 class BatchedLOBPCG:
@@ -124,9 +113,10 @@ class BatchedLOBPCG:
         self.S = torch.zeros((nslots * m3,) + shape, dtype=c64, device=dev)
         self.HS = torch.zeros_like(self.S)
         self.R = torch.zeros((nslots * self.m,) + shape, dtype=c64, device=dev)
-        # per slot [T1_s; T2_s] (2m rows): the new X and P rows of a Rayleigh–Ritz
-        # update come from one combine over S; T1_s is also SVQB scratch
-        self.T12 = torch.zeros((2 * nslots * self.m,) + shape, dtype=c64, device=dev)
+        # per slot [T1_s; T2_s] (2m rows) of scratch, needed only when a combine
+        # cannot run in place (2m > COMBINE_IN_PLACE_MAX)
+        self.T12 = (torch.zeros((2 * nslots * self.m,) + shape, dtype=c64, device=dev)
+                    if 2 * self.m > COMBINE_IN_PLACE_MAX else None)
         self.CARRY = torch.zeros_like(self.R)
         self.L = torch.zeros((nslots * self.cap,) + shape, dtype=c64, device=dev)
         self.slots: list[_Slot | None] = [None] * nslots
@@ -313,24 +303,29 @@ class BatchedLOBPCG:
         from .linalg import eigh_stack_host
         _, C = eigh_stack_host(A, lowest=m)
         t = self._tick("RR_gram+eigh", t)
-        Ct = torch.as_tensor(C, device=self.S.device)
-        Cp = Ct.clone()
-        Cp[:, :m] = 0                                   # Pn: X rows zeroed
-        combine_multi(torch.cat([Ct, Cp], dim=2), Sbs,
-                      [self.T12[2 * s * m:2 * (s + 1) * m] for s in idx])
-        self._deflate(idx, self._t1)
-        # normalised rows back into X and P (solver.py:306-307), over each run of
-        # consecutive iterating slots; other slots' rows are not touched
-        for s0, cnt in _runs(idx):
-            T = self.T12[2 * s0 * m:2 * (s0 + cnt) * m]
-            nrm = rowreduce_strided(0, T, None, 2 * m, 2 * m, 2 * m, 2 * cnt * m,
-                                    self.n).real.view(cnt, 2, m)
-            a1 = 1.0 / torch.clamp(torch.sqrt(nrm[:, 0].reshape(-1)), min=1e-20)
-            a2 = 1.0 / torch.clamp(torch.sqrt(nrm[:, 1].reshape(-1)), min=1e-20)
-            S0 = self.S[3 * s0 * m:]
-            row_axpby_strided(T, 2 * m, a1, None, 2 * m, None, m, cnt * m, self.n, S0, 3 * m)
-            row_axpby_strided(T[m:], 2 * m, a2, None, 2 * m, None, m, cnt * m, self.n,
-                              S0[2 * m:], 3 * m)
+        # X ← S·c_i/‖S·c_i‖ and P ← S·c'_i/‖S·c'_i‖ (c' = c with its X rows
+        # zeroed; solver.py:298-307).  [X; W; P] is orthonormal by construction
+        # (SVQB, projections, whitening) and dead rows get exactly zero weight,
+        # so ‖S·c‖ = ‖c‖ to FP32 rounding: the norms are taken on the host and
+        # folded into the coefficients, and one split combine writes both
+        # updates straight into the slot's X and P rows.
+        Cp = C.copy()
+        Cp[:, :m] = 0
+        nx = np.maximum(np.linalg.norm(C, axis=1), 1e-20)
+        npn = np.maximum(np.linalg.norm(Cp, axis=1), 1e-20)
+        C2 = torch.as_tensor(np.concatenate([C / nx[:, None, :], Cp / npn[:, None, :]], axis=2),
+                             device=self.S.device)
+        Xs = [self._X(s) for s in idx]
+        Ps = [self._P(s) for s in idx]
+        if 2 * m <= COMBINE_IN_PLACE_MAX:
+            combine_multi_split(C2, Sbs, Xs, Ps, m)
+        else:
+            T1s = [self._t1(s) for s in idx]
+            T2s = [self._t2(s) for s in idx]
+            combine_multi_split(C2, Sbs, T1s, T2s, m)
+            copy_multi(T1s, Xs)
+            copy_multi(T2s, Ps)
+        self._deflate(idx, self._X)
         t = self._tick("update_XP", t)
         self._theta(idx, self.S, 0, self.HS, 0, 3 * m, 3 * m, m)
         t = self._tick("theta_X", t)

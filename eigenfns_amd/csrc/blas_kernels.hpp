@@ -21,11 +21,19 @@ struct PairPtrs {
   const float2* a[kMaxProb];
   const float2* b[kMaxProb];
 };
+// Output row i < split goes to out + i·n, row i ≥ split to out2 + (i − split)·n
+// (split = m_out: one contiguous output block).
 struct CombinePtrs {
   const float2* s[kMaxProb];
   const float2* x[kMaxProb];
   float2* out[kMaxProb];
+  float2* out2[kMaxProb];
+  int split;
 };
+
+__device__ __forceinline__ float2* combine_row(const CombinePtrs& cp, int prob, int i, long long n) {
+  return i < cp.split ? cp.out[prob] + (long long)i * n : cp.out2[prob] + (long long)(i - cp.split) * n;
+}
 
 // Output tile of blockIdx.y: tiles_j > 0 — row-major over a tiles_i × tiles_j
 // grid; tiles_j = −T — the upper block triangle ti ≤ tj of a T × T grid, row by
@@ -520,11 +528,10 @@ template <int TI>
 __global__ __launch_bounds__(256) void combine_kernel(const float2* __restrict__ Call, int m_in, int m_out,
                                                       CombinePtrs cp, long long n, float alpha) {
   // each thread reads all m_in rows of its column before it stores, so with
-  // one row tile (gridDim.y = 1) out may be S itself
+  // one row tile (gridDim.y = 1) the outputs may lie in S
   const float2* __restrict__ C = Call + (long long)blockIdx.z * m_in * m_out;
   const float2* S = cp.s[blockIdx.z];
   const float2* X = cp.x[blockIdx.z];
-  float2* out = cp.out[blockIdx.z];
   const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
   const int i0 = blockIdx.y * TI;
   if (k >= n) return;
@@ -552,7 +559,7 @@ __global__ __launch_bounds__(256) void combine_kernel(const float2* __restrict__
         r.x = fmaf(alpha, x.x, r.x);
         r.y = fmaf(alpha, x.y, r.y);
       }
-      out[(long long)(i0 + i) * n + k] = r;
+      combine_row(cp, blockIdx.z, i0 + i, n)[k] = r;
     }
   }
 }
@@ -571,7 +578,7 @@ __global__ __launch_bounds__(256) void combine_kernel(const float2* __restrict__
 // of S stream through a register ring of kCPrefetch steps of 4 rows.  Every
 // output element is the same MFMA chain in j order whatever the column
 // mapping.  Each wave reads all m_in rows of its columns before it stores,
-// so with one row tile (gridDim.y = 1) out may be S itself.  m_out ≥ 1.
+// so with one row tile (gridDim.y = 1) the outputs may lie in S.  m_out ≥ 1.
 // vec: n even and S, X, out 16-byte aligned (float4 access); else per element.
 #ifndef EFN_COMBINE_PREFETCH
 #define EFN_COMBINE_PREFETCH 2
@@ -585,7 +592,6 @@ __global__ __launch_bounds__(256, 2) void combine_mfma_kernel(const float2* __re
   const float2* __restrict__ C = Call + (long long)blockIdx.z * m_in * m_out;
   const float2* S = cp.s[blockIdx.z];
   const float2* X = cp.x[blockIdx.z];
-  float2* out = cp.out[blockIdx.z];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 15, g = lane >> 4;
   const long long c0 = ((long long)blockIdx.x * 4 + wave) * 64;
@@ -702,7 +708,7 @@ __global__ __launch_bounds__(256, 2) void combine_mfma_kernel(const float2* __re
       float2 v[4];
 #pragma unroll
       for (int b = 0; b < 4; ++b) v[b] = make_float2(re[a][b][q], im[a][b][q]);
-      float2* orow = out + (long long)i * n;
+      float2* orow = combine_row(cp, blockIdx.z, i, n);
       if (full) {
         if (X != nullptr) {
           const float2* xrow = X + (long long)i * n;

@@ -273,8 +273,9 @@ int gram_nsplit(int, int, long long n) {
 std::atomic<int> g_gram_engine{EFN_GRAM_MFMA_F64};
 std::atomic<int> g_combine_mfma{1};
 // largest m_out for which combine runs as one row tile on either engine
-// (vector ALU: 32 rows per tile; matrix cores: up to 64), so out may be S
-constexpr int kCombineInPlaceMax = 32;
+// (vector ALU up to 64 rows per tile, matrix cores 16·NA ≤ 64), so the
+// outputs may lie in S
+constexpr int kCombineInPlaceMax = 64;
 
 int rowreduce_nsplit(long long n) {
   long long ns = (n + 65535) / 65536;
@@ -792,62 +793,83 @@ int efn_gram(const void* A, int ma, const void* B, int mb, int64_t n, void* out,
   return gram_multi_impl(1, &A, &B, ma, mb, n, out, work, work_bytes, false, stream);
 }
 
-int efn_combine_multi(int nprob, const void* C, const void* const* S, int m_in, int m_out, int64_t n,
-                      const void* const* X, float alpha, void* const* out, void* stream) {
+namespace {
+int combine_impl(int nprob, const void* C, const void* const* S, int m_in, int m_out, int64_t n,
+                 const void* const* X, float alpha, void* const* out, void* const* out2, int split,
+                 void* stream) {
   if (nprob < 0 || nprob > efn::kMaxProb)
     return fail(EFN_EINVAL, "nprob must be in [0, " + std::to_string(efn::kMaxProb) + "]");
   if (m_in < 0 || m_out < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
   if (nprob == 0 || m_out == 0 || n == 0) return EFN_OK;
   if (!out || (m_in > 0 && (!C || !S))) return fail(EFN_EINVAL, "NULL pointer");
+  if (split < 0 || split > m_out) return fail(EFN_EINVAL, "split must be in [0, m_out]");
+  if (split < m_out && !out2) return fail(EFN_EINVAL, "NULL second output");
   efn::CombinePtrs cp;
   memset(&cp, 0, sizeof(cp));
+  cp.split = split;
   auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
   int vec = (n & 1) == 0;
   for (int i = 0; i < nprob; ++i) {
-    if (!out[i] || (m_in > 0 && !S[i])) return fail(EFN_EINVAL, "NULL block pointer");
+    if ((split > 0 && !out[i]) || (split < m_out && !out2[i]) || (m_in > 0 && !S[i]))
+      return fail(EFN_EINVAL, "NULL block pointer");
     // in place needs one row tile: every reader of a column is its writer
-    if (m_in > 0 && out[i] == S[i] && m_out > kCombineInPlaceMax)
-      return fail(EFN_EINVAL, "out may alias S only for m_out <= " + std::to_string(kCombineInPlaceMax));
+    if (m_in > 0 && (out[i] == S[i] || (out2 && out2[i] == S[i])) && m_out > kCombineInPlaceMax)
+      return fail(EFN_EINVAL, "outputs may lie in S only for m_out <= " + std::to_string(kCombineInPlaceMax));
     cp.s[i] = m_in > 0 ? (const float2*)S[i] : nullptr;
     cp.x[i] = X ? (const float2*)X[i] : nullptr;
     cp.out[i] = (float2*)out[i];
-    if (!al16(cp.out[i]) || (cp.s[i] && !al16(cp.s[i])) || (cp.x[i] && !al16(cp.x[i]))) vec = 0;
+    cp.out2[i] = out2 ? (float2*)out2[i] : nullptr;
+    if ((cp.out[i] && !al16(cp.out[i])) || (cp.out2[i] && !al16(cp.out2[i])) ||
+        (cp.s[i] && !al16(cp.s[i])) || (cp.x[i] && !al16(cp.x[i])))
+      vec = 0;
   }
   hipStream_t s = (hipStream_t)stream;
   const unsigned gx = (unsigned)((n + 255) / 256);
+  const float2* Cc = (const float2*)C;
   // matrix cores: 64 columns per wave, 16·NA output rows per block row (one
   // row tile up to 64 outputs, so S is read once)
   if (g_combine_mfma.load() && m_out > 8) {
-    const unsigned gxm = (unsigned)((n + 255) / 256);
-    const float2* Cc = (const float2*)C;
     if (m_out <= 16)
-      hipLaunchKernelGGL((efn::combine_mfma_kernel<1>), dim3(gxm, 1, nprob), dim3(256), 0, s, Cc, m_in,
+      hipLaunchKernelGGL((efn::combine_mfma_kernel<1>), dim3(gx, 1, nprob), dim3(256), 0, s, Cc, m_in,
                          m_out, cp, (long long)n, alpha, vec);
     else if (m_out <= 32)
-      hipLaunchKernelGGL((efn::combine_mfma_kernel<2>), dim3(gxm, 1, nprob), dim3(256), 0, s, Cc, m_in,
+      hipLaunchKernelGGL((efn::combine_mfma_kernel<2>), dim3(gx, 1, nprob), dim3(256), 0, s, Cc, m_in,
                          m_out, cp, (long long)n, alpha, vec);
     else if (m_out <= 48)
-      hipLaunchKernelGGL((efn::combine_mfma_kernel<3>), dim3(gxm, 1, nprob), dim3(256), 0, s, Cc, m_in,
+      hipLaunchKernelGGL((efn::combine_mfma_kernel<3>), dim3(gx, 1, nprob), dim3(256), 0, s, Cc, m_in,
                          m_out, cp, (long long)n, alpha, vec);
     else
-      hipLaunchKernelGGL((efn::combine_mfma_kernel<4>), dim3(gxm, (m_out + 63) / 64, nprob), dim3(256),
+      hipLaunchKernelGGL((efn::combine_mfma_kernel<4>), dim3(gx, (m_out + 63) / 64, nprob), dim3(256),
                          0, s, Cc, m_in, m_out, cp, (long long)n, alpha, vec);
     EFN_LAUNCH_CHECK();
     return EFN_OK;
   }
-  // one row tile per 32 outputs: S is read ceil(m_out / 32) times
-  if (m_out <= 8) {
-    hipLaunchKernelGGL((efn::combine_kernel<8>), dim3(gx, 1, nprob), dim3(256), 0, s,
-                       (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
-  } else if (m_out <= 16) {
-    hipLaunchKernelGGL((efn::combine_kernel<16>), dim3(gx, 1, nprob), dim3(256), 0, s,
-                       (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
-  } else {
-    hipLaunchKernelGGL((efn::combine_kernel<32>), dim3(gx, (m_out + 31) / 32, nprob), dim3(256), 0, s,
-                       (const float2*)C, m_in, m_out, cp, (long long)n, alpha);
-  }
+  // vector ALU: one row tile up to 64 outputs, then tiles of 64
+  if (m_out <= 8)
+    hipLaunchKernelGGL((efn::combine_kernel<8>), dim3(gx, 1, nprob), dim3(256), 0, s, Cc, m_in, m_out, cp,
+                       (long long)n, alpha);
+  else if (m_out <= 16)
+    hipLaunchKernelGGL((efn::combine_kernel<16>), dim3(gx, 1, nprob), dim3(256), 0, s, Cc, m_in, m_out, cp,
+                       (long long)n, alpha);
+  else if (m_out <= 32)
+    hipLaunchKernelGGL((efn::combine_kernel<32>), dim3(gx, 1, nprob), dim3(256), 0, s, Cc, m_in, m_out, cp,
+                       (long long)n, alpha);
+  else
+    hipLaunchKernelGGL((efn::combine_kernel<64>), dim3(gx, (m_out + 63) / 64, nprob), dim3(256), 0, s, Cc,
+                       m_in, m_out, cp, (long long)n, alpha);
   EFN_LAUNCH_CHECK();
   return EFN_OK;
+}
+}  // namespace
+
+int efn_combine_multi(int nprob, const void* C, const void* const* S, int m_in, int m_out, int64_t n,
+                      const void* const* X, float alpha, void* const* out, void* stream) {
+  return combine_impl(nprob, C, S, m_in, m_out, n, X, alpha, out, nullptr, m_out, stream);
+}
+
+int efn_combine_multi_split(int nprob, const void* C, const void* const* S, int m_in, int m_out,
+                            int64_t n, void* const* out, void* const* out2, int split, void* stream) {
+  return combine_impl(nprob, C, S, m_in, m_out, n, nullptr, 0.f, out, out2, split, stream);
 }
 
 int efn_combine(const void* C, int m_in, int m_out, const void* S, int64_t n, const void* X,

@@ -95,9 +95,9 @@ def gram_herm(A: torch.Tensor, B: torch.Tensor | None = None) -> torch.Tensor:
     return gram_multi([A.contiguous()], [B.contiguous()], hermitian=True)[0]
 
 
-# combine may write over its own input block (out is S) up to this many output
+# combine may write into its own input block (out in S) up to this many output
 # rows: one row tile, each column read in full before it is written
-COMBINE_IN_PLACE_MAX = 32
+COMBINE_IN_PLACE_MAX = 64
 
 
 def combine_into(C: torch.Tensor, S: torch.Tensor, out: torch.Tensor, X=None,
@@ -310,6 +310,21 @@ def combine_multi(C: torch.Tensor, Ss, outs, Xs=None, alpha: float = 0.0) -> Non
     _lib.check(_lib.load().efn_combine_multi(nprob, C.data_ptr(), _ptrs(Ss), m_in, m_out, n,
                                              None if Xs is None else _ptrs(Xs), float(alpha),
                                              _ptrs(outs), stream_ptr()), "combine_multi")
+
+
+def combine_multi_split(C: torch.Tensor, Ss, outs, outs2, split: int) -> None:
+    """Rows i < split of C_pᵀ·S_p into outs[p], rows i ≥ split into outs2[p]
+    (C: (nprob, m_in, m_out)); both may lie inside S_p when m_out ≤
+    COMBINE_IN_PLACE_MAX."""
+    nprob = len(Ss)
+    if nprob > MAX_PROBLEMS:
+        raise ValueError(f"combine_multi_split: at most {MAX_PROBLEMS} problems per launch")
+    C = C.to(device=Ss[0].device, dtype=torch.complex64).contiguous()
+    m_in, m_out = int(C.shape[1]), int(C.shape[2])
+    n = _flat_n(Ss[0])
+    _lib.check(_lib.load().efn_combine_multi_split(nprob, C.data_ptr(), _ptrs(Ss), m_in, m_out, n,
+                                                   _ptrs(outs), _ptrs(outs2), int(split),
+                                                   stream_ptr()), "combine_multi_split")
 
 
 def copy_multi(srcs, dsts) -> None:

@@ -53,10 +53,13 @@ def _stats(lat):
 
 
 def _slot_bytes(grid, m, guard, max_nev):
-    """Device bytes one engine slot holds: [X; W; P] and Θ of it, R, two
-    temporaries, the guard carry and the locked buffer (BatchedLOBPCG)."""
+    """Device bytes one engine slot holds: [X; W; P] and Θ of it, R, the guard
+    carry, the locked buffer and, when combines cannot run in place, two
+    temporaries (BatchedLOBPCG)."""
+    from .linalg import COMBINE_IN_PLACE_MAX
     cap = int(np.ceil(max_nev / (m - guard))) * (m - guard) + m
-    return (2 * 3 * m + 4 * m + cap) * 2 * grid ** 3 * 8
+    temps = 2 * m if 2 * m > COMBINE_IN_PLACE_MAX else 0
+    return (2 * 3 * m + 2 * m + temps + cap) * 2 * grid ** 3 * 8
 
 
 def plan_slots(grids, m, guard, max_nev, max_slots, hbm_budget, engine_voxels=None):

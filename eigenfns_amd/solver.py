@@ -21,9 +21,10 @@ import numpy as np
 import torch
 
 from .linalg import (COMBINE_IN_PLACE_MAX, DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_DROP, apply_chunked,
-                     combine, combine_into, deflate, deflate_chunk_rows, eigh, gram, gram_herm,
-                     normalize_rows, ortho_block, row_axpby, row_axpby_norm_strided, rowdot, rownorm,
-                     rownorm2, rowstats_strided, svqb_coeffs)
+                     combine, combine_into, combine_multi_split, deflate, deflate_chunk_rows, eigh,
+                     gram, gram_herm, normalize_rows, ortho_block, row_axpby,
+                     row_axpby_norm_strided, rowdot, rownorm, rownorm2, rowstats_strided,
+                     svqb_coeffs)
 
 __all__ = ["gram", "combine", "deflate", "ortho_block", "apply_chunked", "deflate_chunk_rows",
            "SolveStats", "lobpcg_blocks", "lobpcg_blocks_resumable", "_flat",
@@ -204,13 +205,23 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             penalty = max(RR_DEAD_PENALTY, 100.0 * float(torch.diagonal(A).abs().max().item()))
             A = A + torch.diag(torch.where(dead, penalty, 0.0).to(A.dtype))
             wA, VA = eigh(A)
+            # Xn = S·c/‖S·c‖, Pn = S·c'/‖S·c'‖ (c' = c with its X rows zeroed;
+            # solver.py:298-307).  S is orthonormal by construction and dead rows
+            # get zero weight, so ‖S·c‖ = ‖c‖ to FP32 rounding: the norms are
+            # folded into the coefficients and one split combine writes both
+            # updates into S's X and P rows (in place when 2m ≤ 64).
             Cx = VA[:, :m]
             Cp = Cx.clone()
-            Cp[:m] = 0                                 # Pn: X rows zeroed
-            combine_into(torch.cat([Cx, Cp], dim=1), Sb, T12)   # [Xn; Pn]
-            _deflate_inplace(T1, L, nl)
-            normalize_rows(T1, floor=1e-20, out=X)
-            normalize_rows(T2, floor=1e-20, out=P)
+            Cp[:m] = 0
+            C2 = torch.cat([Cx / torch.clamp(torch.linalg.vector_norm(Cx, dim=0), min=1e-20),
+                            Cp / torch.clamp(torch.linalg.vector_norm(Cp, dim=0), min=1e-20)], dim=1)
+            if 2 * m <= COMBINE_IN_PLACE_MAX:
+                combine_multi_split(C2[None], [Sb], [X], [P], m)
+            else:
+                combine_multi_split(C2[None], [Sb], [T1], [T2], m)
+                X.copy_(T1)
+                P.copy_(T2)
+            _deflate_inplace(X, L, nl)
             have_p = True
             op.theta_into(X, HX)
             stats.theta_applications += m

@@ -164,7 +164,8 @@ int efn_gram_multi_herm(int nprob, const void* const* A_c64, const void* const* 
 /* Linear combination out_i = alpha·X_i + Σ_j C[j][i]·S_j (solver.py:49-57,
  * :106, :125): C is m_in×m_out complex64 on the device, S m_in rows, X
  * (optional, NULL = none) and out m_out rows of n complex64.  out may be S
- * itself (in place) when m_out <= 32 and m_out <= m_in; otherwise it must not
+ * itself (in place) when m_out <= 64 and m_out <= m_in (one row tile: every
+ * column is read in full before it is written); otherwise it must not
  * overlap S. */
 int efn_combine(const void* C_c64, int m_in, int m_out, const void* S_c64, int64_t n,
                 const void* X_c64, float alpha, void* out_c64, void* stream);
@@ -181,6 +182,16 @@ int efn_combine_engine(int engine);
 int efn_combine_multi(int nprob, const void* C_c64, const void* const* S_c64, int m_in, int m_out,
                       int64_t n, const void* const* X_c64, float alpha, void* const* out_c64,
                       void* stream);
+
+/* efn_combine_multi without X, its m_out output rows split over two blocks:
+ * row i < split to out[p] + i·n, row i >= split to out2[p] + (i − split)·n.
+ * With m_out <= 64 both blocks may lie inside S's rows (one row tile), so
+ * the two Rayleigh–Ritz updates X ← S·Cx and P ← S·Cp (solver.py:298-307)
+ * land in S's X and P rows in one pass over S.  No reference counterpart (the
+ * reference forms X and P in separate products). */
+int efn_combine_multi_split(int nprob, const void* C_c64, const void* const* S_c64, int m_in,
+                            int m_out, int64_t n, void* const* out_c64, void* const* out2_c64,
+                            int split, void* stream);
 
 /* Row reductions in FP64 (solver.py:228-231, interior.py:75-82):
  *   op 0: out[i] = (Σ_k |A[i][k]|², 0)      op 1: out[i] = Σ_k conj(A[i][k])·B[i][k]

@@ -157,7 +157,7 @@ def test_combine_engines_match_complex128(engine):
 
 @pytest.mark.parametrize("engine", ["valu", "mfma"])
 def test_combine_in_place_and_unaligned_rows(engine):
-    """out may be S itself for m_out <= 32 (one row tile: every column is read
+    """out may be S itself for m_out <= 64 (one row tile: every column is read
     in full before it is written) — same values as into a fresh buffer; larger
     m_out in place is refused.  Rows starting 8 bytes off a 16-byte boundary
     take the per-element path with the same results."""
@@ -166,7 +166,8 @@ def test_combine_in_place_and_unaligned_rows(engine):
     prev = combine_engine(engine)
     try:
         rng = np.random.default_rng(23)
-        for m, n in [(24, 2 * 16 ** 3), (32, 4098), (13, 777), (32, 2 * 8 ** 3 + 1)]:
+        for m, n in [(24, 2 * 16 ** 3), (32, 4098), (13, 777), (32, 2 * 8 ** 3 + 1), (40, 3000),
+                     (64, 2 * 12 ** 3)]:
             S = _cblock(rng, m, n)
             Cm = (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))).astype(np.complex64)
             Sd, Cd = torch.from_numpy(S).cuda(), torch.from_numpy(Cm).cuda()
@@ -183,9 +184,42 @@ def test_combine_in_place_and_unaligned_rows(engine):
             ou = torch.zeros(m * n + 1, dtype=torch.complex64, device="cuda")[1:].view(m, n)
             combine_into(Cd, Su, ou)
             assert torch.equal(ou, fresh), (m, n)
-        S = torch.from_numpy(_cblock(rng, 40, 1000)).cuda()
-        C = torch.zeros((40, 40), dtype=torch.complex64, device="cuda")
+        S = torch.from_numpy(_cblock(rng, 72, 1000)).cuda()
+        C = torch.zeros((72, 72), dtype=torch.complex64, device="cuda")
         with pytest.raises(ValueError):
             combine_into(C, S, S)
+    finally:
+        combine_engine(prev)
+
+
+@pytest.mark.parametrize("engine", ["valu", "mfma"])
+def test_combine_split_into_own_rows(engine):
+    """efn_combine_multi_split: rows < split to one block, the rest to another;
+    both blocks may be rows of S itself (the Rayleigh–Ritz update writes X and
+    P into S's X and P rows) — bitwise the same as separate combines into
+    fresh buffers; up to 64 outputs in place, more refused."""
+    import torch
+    from eigenfns_amd.linalg import combine_engine, combine_multi, combine_multi_split
+    prev = combine_engine(engine)
+    try:
+        rng = np.random.default_rng(29)
+        for m, n in [(24, 2 * 16 ** 3), (32, 4098), (7, 1001), (16, 2 * 12 ** 3)]:
+            S0 = torch.from_numpy(_cblock(rng, 3 * m, n)).cuda()
+            C = torch.from_numpy((rng.standard_normal((2, 3 * m, 2 * m))
+                                  + 1j * rng.standard_normal((2, 3 * m, 2 * m))).astype(np.complex64)).cuda()
+            ref = [torch.empty((2 * m, n), dtype=torch.complex64, device="cuda") for _ in range(2)]
+            Ss = [S0.clone(), S0.clone()]
+            combine_multi(C, [S0, S0], ref)
+            combine_multi_split(C, Ss, [S[:m] for S in Ss], [S[2 * m:] for S in Ss], m)
+            for S, r in zip(Ss, ref):
+                assert torch.equal(S[:m], r[:m]), (m, n)
+                assert torch.equal(S[2 * m:], r[m:]), (m, n)
+                assert torch.equal(S[m:2 * m], S0[m:2 * m])       # W rows untouched
+            plain = C[0].cpu().numpy().astype(np.complex128).T @ S0.cpu().numpy().astype(np.complex128)
+            assert np.abs(ref[0].cpu().numpy() - plain).max() < 2e-5 * np.abs(plain).max()
+        S = torch.from_numpy(_cblock(rng, 96, 1000)).cuda()
+        C = torch.zeros((1, 96, 96), dtype=torch.complex64, device="cuda")
+        with pytest.raises(ValueError):
+            combine_multi_split(C, [S], [S[:48]], [S], 48)
     finally:
         combine_engine(prev)
