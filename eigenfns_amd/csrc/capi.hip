@@ -286,7 +286,7 @@ int rowreduce_nsplit(long long n) {
 
 extern "C" {
 
-int efn_version(void) { return 100; }
+int efn_version(void) { return 200; }  // 0.2.0: adds efn_combine_multi_split
 
 const char* efn_last_error(void) { return g_last_error.c_str(); }
 
