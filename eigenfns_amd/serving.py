@@ -100,7 +100,7 @@ def _summary(done, wall, turns, engines):
 
 def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slots=32,
                   hbm_budget=200e9, verbose=False, threads=True, progress=False,
-                  engines_per_grid=1, engine_voxels=None):
+                  engines_per_grid=2, engine_voxels=None):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
 
     threads=True: each grid's engine runs in its own host thread on its own HIP
@@ -109,7 +109,9 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     engines can share the GPU.  Requests are admitted FIFO per grid.
     engines_per_grid > 1 splits each grid's slots over that many engines
     (threads, streams) pulling from the same queue, so one half's host work
-    overlaps the other half's kernels."""
+    (five small eigensolves and syncs per iteration) overlaps the other
+    half's kernels: 2 measured +16 % solves/s over 1 (48 requests, 32³–64³),
+    3 no further gain."""
     cache = cache or StructureCache()
     grids = sorted({sp.grid for sp in specs})
     max_nev = max(sp.nev for sp in specs)
@@ -258,7 +260,7 @@ if __name__ == "__main__":
     ap.add_argument("--plain", action="store_true")
     ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--sequential", action="store_true", help="one host thread, one stream")
-    ap.add_argument("--engines-per-grid", type=int, default=1)
+    ap.add_argument("--engines-per-grid", type=int, default=2)
     args = ap.parse_args()
     grids = tuple(int(g) for g in args.grids.split(","))
     specs = request_stream(args.n, args.seed, grids=grids, nev_range=(8, args.nev_max))
