@@ -11,6 +11,7 @@
 
 #include "../../include/eigenfns_hip.h"
 #include "blas_kernels.hpp"
+#include "eig_kernels.hpp"
 #include "operator_kernels.hpp"
 #include "raster_kernels.hpp"
 
@@ -870,6 +871,27 @@ int efn_combine_multi(int nprob, const void* C, const void* const* S, int m_in, 
 int efn_combine_multi_split(int nprob, const void* C, const void* const* S, int m_in, int m_out,
                             int64_t n, void* const* out, void* const* out2, int split, void* stream) {
   return combine_impl(nprob, C, S, m_in, m_out, n, nullptr, 0.f, out, out2, split, stream);
+}
+
+int efn_heev_batched(int nprob, int n, const void* A, double* w, void* V, int max_sweeps, void* stream) {
+  if (nprob < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
+  if (n > efn::kEigMaxN) return fail(EFN_EINVAL, "n must be <= " + std::to_string(efn::kEigMaxN));
+  if (nprob == 0 || n == 0) return EFN_OK;
+  if (!A || !w || !V) return fail(EFN_EINVAL, "NULL pointer");
+  if (A == V) return fail(EFN_EINVAL, "A and V must not alias");
+  if (max_sweeps <= 0) max_sweeps = 30;
+  const int P = (n + (n & 1)) / 2;
+  const size_t lds = (size_t)n * (n + 1) * sizeof(double2) + 3 * (size_t)P * sizeof(double2) +
+                     (size_t)n * sizeof(double);
+  if (lds > 65536) {
+    const hipError_t e = hipFuncSetAttribute((const void*)efn::heev_jacobi_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return fail(EFN_EHIP, std::string("heev LDS size: ") + hipGetErrorString(e));
+  }
+  hipLaunchKernelGGL(efn::heev_jacobi_kernel, dim3((unsigned)nprob), dim3(efn::kEigThreads), lds,
+                     (hipStream_t)stream, (const double2*)A, n, max_sweeps, w, (double2*)V);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
 }
 
 int efn_combine(const void* C, int m_in, int m_out, const void* S, int64_t n, const void* X,

@@ -428,6 +428,21 @@ def eigh_stack_host(a: np.ndarray, lowest: int | None = None):
     return w, V
 
 
+def eigh_batched_device(A: torch.Tensor, lowest: int | None = None, max_sweeps: int = 0):
+    """(w, V) of a (nprob, k, k) complex128 device stack (k ≤ 96) on the device
+    (efn_heev_batched, FP64 Jacobi); ascending, V[:, :, j] the vector of w[:, j];
+    with `lowest`, the lowest eigenpairs.  No host round trip."""
+    A = A.to(torch.complex128).contiguous()
+    nprob, k = int(A.shape[0]), int(A.shape[1])
+    w = torch.empty((nprob, k), dtype=torch.float64, device=A.device)
+    V = torch.empty_like(A)
+    _lib.check(_lib.load().efn_heev_batched(nprob, k, A.data_ptr(), w.data_ptr(), V.data_ptr(),
+                                            int(max_sweeps), stream_ptr()), "heev_batched")
+    if lowest is not None and lowest < k:
+        return w[:, :lowest], V[:, :, :lowest]
+    return w, V
+
+
 def eigh_batched_host(A: torch.Tensor, lowest: int | None = None):
     """FP64 Hermitian eigendecomposition of a (nprob, k, k) stack on the host (LAPACK)."""
     return eigh_stack_host(A.detach().cpu().numpy(), lowest)

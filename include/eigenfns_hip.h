@@ -170,6 +170,18 @@ int efn_gram_multi_herm(int nprob, const void* const* A_c64, const void* const* 
 int efn_combine(const void* C_c64, int m_in, int m_out, const void* S_c64, int64_t n,
                 const void* X_c64, float alpha, void* out_c64, void* stream);
 
+/* Batched Hermitian eigendecomposition of nprob small matrices (n ≤ 96) on
+ * the device: A (nprob × n × n complex128, row-major; only its upper triangle
+ * and the real part of its diagonal are read) → w (nprob × n, ascending) and
+ * V (nprob × n × n complex128, column j the eigenvector of w[j]).  Cyclic
+ * two-sided Jacobi in FP64, one workgroup per matrix, relative convergence
+ * test (small eigenvalues keep their relative accuracy); max_sweeps <= 0
+ * means 30.  Replaces the host eigensolves of SVQB and Rayleigh–Ritz
+ * (solver.py:119-121, :294-297) so a LOBPCG iteration needs no host round
+ * trip for them. */
+int efn_heev_batched(int nprob, int n, const void* A_c128, double* w, void* V_c128, int max_sweeps,
+                     void* stream);
+
 /* Process-wide combine engine: 1 = matrix cores (v_mfma_f32_16x16x4f32,
  * default, for m_out > 8), 0 = vector ALU; FP32 accumulation either way.
  * Returns the previous engine (engine < 0: query only).  No reference
