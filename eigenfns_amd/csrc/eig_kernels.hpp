@@ -7,8 +7,9 @@
 // = component i of eigenvector j).  A sweep is N − 1 rounds of the circle
 // ordering over N = n rounded up to even (an index n is a bye); a round
 // rotates its N/2 disjoint pairs in parallel: rotation parameters, then
-// A ← A·U (columns), then A ← Uᴴ·A (rows), V ← V·U (each thread owns a row of
-// V, so V needs no barrier of its own).
+// A ← A·U with V ← V·U (columns), then A ← Uᴴ·A (rows), three barriers a
+// round; each update phase spreads its (pair, row) items over the whole
+// workgroup and issues a thread's loads before its stores.
 //
 // The rotation for pair (p, q): with b = A[p][q] = |b|e^{iφ}, D = diag(1,
 // e^{−iφ}) makes the 2 × 2 block real, [[a, |b|], [|b|, d]], which the real
@@ -28,6 +29,9 @@ namespace efn {
 
 constexpr int kEigMaxN = 96;
 constexpr int kEigThreads = 256;
+// (pair, row) work items per thread in one update phase: ⌈(N/2)·n / threads⌉
+constexpr int kEigItems = (kEigMaxN / 2 * kEigMaxN + kEigThreads - 1) / kEigThreads;
+constexpr int kEigChunk = 6;  // items whose loads are in flight together (register budget)
 constexpr double kJacobiTol = 1e-14;  // ≈ n·ε: below it round-off would re-trigger rotations
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
@@ -107,52 +111,85 @@ __global__ __launch_bounds__(kEigThreads) void heev_jacobi_kernel(const double2*
         rot[3 * tid + 2] = ph;
       }
       __syncthreads();
-      // columns: A[i][p], A[i][q] ← (c·x − s·e^{−iφ}·y, s·x + c·e^{−iφ}·y)
-      for (int e = tid; e < P * n; e += kEigThreads) {
-        const int k = e / n, i = e - k * n;
-        const double c = rot[3 * k].x, s = rot[3 * k + 1].x;
-        if (s == 0.0) continue;
-        const int p = pq[2 * k], q = pq[2 * k + 1];
-        const double2 ph = rot[3 * k + 2];
-        const double2 x = A[i * ld + p], y = A[i * ld + q];
-        const double2 ey = zmul(ph, y);
-        A[i * ld + p] = make_double2(c * x.x - s * ey.x, c * x.y - s * ey.y);
-        A[i * ld + q] = make_double2(s * x.x + c * ey.x, s * x.y + c * ey.y);
-      }
-      // eigenvectors: thread i < n owns row i of V
-      if (tid < n) {
-        double2* vr = V + (long long)tid * n;
-        for (int k = 0; k < P; ++k) {
-          const double c = rot[3 * k].x, s = rot[3 * k + 1].x;
-          if (s == 0.0) continue;
-          const int p = pq[2 * k], q = pq[2 * k + 1];
-          const double2 ey = zmul(rot[3 * k + 2], vr[q]);
-          const double2 x = vr[p];
-          vr[p] = make_double2(c * x.x - s * ey.x, c * x.y - s * ey.y);
-          vr[q] = make_double2(s * x.x + c * ey.x, s * x.y + c * ey.y);
+      // columns of A and V: A[i][p], A[i][q] ← (c·x − s·e^{−iφ}·y, s·x + c·e^{−iφ}·y),
+      // the same for V.  Work item (pair k, row i), all of a thread's loads
+      // issued before its stores (the items touch distinct entries).
+      for (int it0 = 0; it0 < kEigItems; it0 += kEigChunk) {
+        double2 ax[kEigChunk], ay[kEigChunk], vx[kEigChunk], vy[kEigChunk];
+#pragma unroll
+        for (int it = 0; it < kEigChunk; ++it) {
+          const int e = tid + (it0 + it) * kEigThreads;
+          if (e < P * n) {
+            const int k = e / n, i = e - k * n;
+            if (rot[3 * k + 1].x != 0.0) {
+              const int p = pq[2 * k], q = pq[2 * k + 1];
+              ax[it] = A[i * ld + p];
+              ay[it] = A[i * ld + q];
+              vx[it] = V[(long long)i * n + p];
+              vy[it] = V[(long long)i * n + q];
+            }
+          }
+        }
+#pragma unroll
+        for (int it = 0; it < kEigChunk; ++it) {
+          const int e = tid + (it0 + it) * kEigThreads;
+          if (e < P * n) {
+            const int k = e / n, i = e - k * n;
+            const double c = rot[3 * k].x, s = rot[3 * k + 1].x;
+            if (s != 0.0) {
+              const int p = pq[2 * k], q = pq[2 * k + 1];
+              const double2 ph = rot[3 * k + 2];
+              const double2 ea = zmul(ph, ay[it]), ev = zmul(ph, vy[it]);
+              A[i * ld + p] = make_double2(c * ax[it].x - s * ea.x, c * ax[it].y - s * ea.y);
+              A[i * ld + q] = make_double2(s * ax[it].x + c * ea.x, s * ax[it].y + c * ea.y);
+              V[(long long)i * n + p] = make_double2(c * vx[it].x - s * ev.x, c * vx[it].y - s * ev.y);
+              V[(long long)i * n + q] = make_double2(s * vx[it].x + c * ev.x, s * vx[it].y + c * ev.y);
+            }
+          }
         }
       }
       __syncthreads();
-      // rows: A[p][j], A[q][j] ← (c·x − s·e^{iφ}·y, s·x + c·e^{iφ}·y)
-      for (int e = tid; e < P * n; e += kEigThreads) {
-        const int k = e / n, j = e - k * n;
-        const double c = rot[3 * k].x, s = rot[3 * k + 1].x;
-        if (s == 0.0) continue;
-        const int p = pq[2 * k], q = pq[2 * k + 1];
-        const double2 ph = zconj(rot[3 * k + 2]);
-        const double2 x = A[p * ld + j], y = A[q * ld + j];
-        const double2 ey = zmul(ph, y);
-        A[p * ld + j] = make_double2(c * x.x - s * ey.x, c * x.y - s * ey.y);
-        A[q * ld + j] = make_double2(s * x.x + c * ey.x, s * x.y + c * ey.y);
-      }
-      __syncthreads();
-      // the rotated 2 × 2 blocks exactly: diagonal a − t|b|, d + t|b|, zero coupling
-      if (tid < P && rot[3 * tid + 1].x != 0.0) {
-        const int p = pq[2 * tid], q = pq[2 * tid + 1];
-        A[p * ld + p] = make_double2(rot[3 * tid].y, 0.0);
-        A[q * ld + q] = make_double2(rot[3 * tid + 1].y, 0.0);
-        A[p * ld + q] = make_double2(0.0, 0.0);
-        A[q * ld + p] = make_double2(0.0, 0.0);
+      // rows: A[p][j], A[q][j] ← (c·x − s·e^{iφ}·y, s·x + c·e^{iφ}·y); the 2 × 2
+      // block itself is set exactly: diagonal a − t|b|, d + t|b|, zero coupling
+      for (int it0 = 0; it0 < kEigItems; it0 += kEigChunk) {
+        double2 ax[kEigChunk], ay[kEigChunk];
+#pragma unroll
+        for (int it = 0; it < kEigChunk; ++it) {
+          const int e = tid + (it0 + it) * kEigThreads;
+          if (e < P * n) {
+            const int k = e / n, j = e - k * n;
+            if (rot[3 * k + 1].x != 0.0) {
+              const int p = pq[2 * k], q = pq[2 * k + 1];
+              ax[it] = A[p * ld + j];
+              ay[it] = A[q * ld + j];
+            }
+          }
+        }
+#pragma unroll
+        for (int it = 0; it < kEigChunk; ++it) {
+          const int e = tid + (it0 + it) * kEigThreads;
+          if (e < P * n) {
+            const int k = e / n, j = e - k * n;
+            const double c = rot[3 * k].x, s = rot[3 * k + 1].x;
+            if (s != 0.0) {
+              const int p = pq[2 * k], q = pq[2 * k + 1];
+              double2 np_, nq;
+              if (j == p) {
+                np_ = make_double2(rot[3 * k].y, 0.0);
+                nq = make_double2(0.0, 0.0);
+              } else if (j == q) {
+                np_ = make_double2(0.0, 0.0);
+                nq = make_double2(rot[3 * k + 1].y, 0.0);
+              } else {
+                const double2 ey = zmul(zconj(rot[3 * k + 2]), ay[it]);
+                np_ = make_double2(c * ax[it].x - s * ey.x, c * ax[it].y - s * ey.y);
+                nq = make_double2(s * ax[it].x + c * ey.x, s * ax[it].y + c * ey.y);
+              }
+              A[p * ld + j] = np_;
+              A[q * ld + j] = nq;
+            }
+          }
+        }
       }
       __syncthreads();
     }

@@ -100,7 +100,7 @@ def _summary(done, wall, turns, engines):
 
 def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slots=32,
                   hbm_budget=200e9, verbose=False, threads=True, progress=False,
-                  engines_per_grid=2, engine_voxels=None):
+                  engines_per_grid=2, engine_voxels=None, small_dense="host"):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
 
     threads=True: each grid's engine runs in its own host thread on its own HIP
@@ -117,7 +117,8 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     max_nev = max(sp.nev for sp in specs)
     plan = plan_slots(grids, m, guard, max_nev, max_slots, hbm_budget, engine_voxels)
     if not threads:
-        return _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose)
+        return _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose,
+                                 small_dense)
     import threading
     import torch
     queues = {g: sorted([sp for sp in specs if sp.grid == g], key=lambda s: s.t_arrive)
@@ -134,7 +135,8 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
             with torch.cuda.stream(stream):
                 pending = queues[g]
                 nslots = max(1, min(-(-plan[g] // k), -(-len(pending) // k)))
-                eng = BatchedLOBPCG(g, m, guard, nslots, max_nev, tol, maxit)
+                eng = BatchedLOBPCG(g, m, guard, nslots, max_nev, tol, maxit,
+                                    small_dense=small_dense)
                 with lock:
                     engines[str(g) if k == 1 else f"{g}.{e}"] = eng
                 while True:
@@ -181,7 +183,8 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     return done, _summary(done, time.perf_counter() - t0, turns[0], engines)
 
 
-def _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose):
+def _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose,
+                      small_dense="host"):
     """One host thread, one stream: every loop turn steps each engine in turn."""
     engines = {}
     pending = sorted(specs, key=lambda s: s.t_arrive)
@@ -198,7 +201,7 @@ def _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose
             eng = engines.get(sp.grid)
             if eng is None:
                 eng = engines[sp.grid] = BatchedLOBPCG(sp.grid, m, guard, plan[sp.grid], max_nev,
-                                                       tol, maxit)
+                                                       tol, maxit, small_dense=small_dense)
             if eng.free_slots():
                 req = Request(_make_op(cache, sp, workspace=False), sp.nev, seed=sp.seed,
                               rid=sp.rid, meta={"grid": sp.grid, "n_points": sp.n_points})
@@ -261,12 +264,14 @@ if __name__ == "__main__":
     ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--sequential", action="store_true", help="one host thread, one stream")
     ap.add_argument("--engines-per-grid", type=int, default=2)
+    ap.add_argument("--small-dense", choices=["device", "host", "hybrid"], default="host")
     args = ap.parse_args()
     grids = tuple(int(g) for g in args.grids.split(","))
     specs = request_stream(args.n, args.seed, grids=grids, nev_range=(8, args.nev_max))
     cache = StructureCache()
     res, summ = serve_batched(specs, cache, max_slots=args.slots, verbose=True,
-                              threads=not args.sequential, engines_per_grid=args.engines_per_grid)
+                              threads=not args.sequential, engines_per_grid=args.engines_per_grid,
+                              small_dense=args.small_dense)
     print(json.dumps({"batched": summ}), flush=True)
     if args.plain:
         pres, psumm = serve_plain(specs, cache)
