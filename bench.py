@@ -230,6 +230,67 @@ def cpu_baseline(eps, L, G, budget_s=12.0):
                       f"{cores} workers, {el:.1f} s); cpu_count={os.cpu_count()}; {cpu_model}"}
 
 
+def cpu_side_by_side(stream, budget_s=8.0):
+    """Two more shapes timed on the host cores (the oracle, complex64 SciPy FFT,
+    all cores) and on the GPU (the product path): the Θ-apply at 128³ on a
+    32-vector block (1000-point network) and one KPM doubling step at 256³ for
+    one probe (10,000-point network).  CPU legs are bounded samples."""
+    import torch
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import operator_np
+    from eigenfns_amd import workloads as wl
+    from eigenfns_amd.operator import MaxwellOperator
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    operator_np.set_workers(cores)
+    rng = np.random.default_rng(0)
+    out = {"cores": int(cores)}
+
+    def cpu_loop(fn):
+        fn()
+        n, t0 = 0, time.perf_counter()
+        while True:
+            fn()
+            n += 1
+            el = time.perf_counter() - t0
+            if el > budget_s or n >= 32:
+                return n, el
+
+    # Θ at 128³, m = 32
+    G = 128
+    eps, L = wl.network(1000, G)
+    op = MaxwellOperator(eps, L)
+    X = torch.randn((32, 2, G, G, G), dtype=torch.complex64, device="cuda")
+    Y = torch.empty_like(X)
+    t = _launch_seconds(lambda: op.theta_into(X, Y), 5, stream)
+    opn = operator_np.MaxwellOperatorNP(eps, L, fft_dtype=np.complex64, out_dtype=np.complex64)
+    Xc = (rng.standard_normal((1, 2, G, G, G)) + 1j * rng.standard_normal((1, 2, G, G, G))).astype(np.complex64)
+    n, el = cpu_loop(lambda: opn.theta(Xc))
+    out["theta_128_m32"] = {"gpu_per_s": round(32 / t, 1), "cpu_per_s": round(n / el, 2),
+                            "unit": "vector-applies/s",
+                            "cpu_sample": f"{n} single-vector applies in {el:.1f} s"}
+    del X, Y, op
+    # one KPM doubling step at 256³, one probe: T2 = 2(αΘT1 − T1) − T0 and the two dots
+    G = 256
+    eps, L = wl.network(10000, G)
+    op = MaxwellOperator(eps, L)
+    alpha = 2.0 / 4.0e3
+    T1 = torch.randn((1, 2, G, G, G), dtype=torch.complex64, device="cuda")
+    T0 = torch.randn_like(T1)
+    mom = torch.zeros((1, 64), dtype=torch.float64, device="cuda")
+    t = _launch_seconds(lambda: op.kpm_step(T1, T0, mom, 5, alpha), 5, stream)
+    opn = operator_np.MaxwellOperatorNP(eps, L, fft_dtype=np.complex64, out_dtype=np.complex64)
+    t1 = (rng.standard_normal((1, 2, G, G, G)) + 1j * rng.standard_normal((1, 2, G, G, G))).astype(np.complex64)
+    t0 = t1.copy()
+
+    def cpu_step():
+        t2 = 2.0 * (alpha * opn.theta(t1) - t1) - t0
+        return np.vdot(t1, t1).real, np.vdot(t2, t1).real
+    n, el = cpu_loop(cpu_step)
+    out["kpm_step_256_1probe"] = {"gpu_ms": round(t * 1e3, 3), "cpu_ms": round(el / n * 1e3, 1),
+                                  "cpu_sample": f"{n} steps in {el:.1f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +300,8 @@ def main():
     ap.add_argument("--m", type=int, default=M_BENCH)
     ap.add_argument("--chunk", type=int, default=0, help="vectors per workspace pass (0 = auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-extra", action="store_true",
+                    help="also time Θ at 128³ (m = 32) and a KPM step at 256³ on CPU and GPU")
     ap.add_argument("--no-paths", action="store_true",
                     help="skip the other path kernels (preconditioner, Chebyshev/KPM steps, Gram, combine)")
     ap.add_argument("--no-serving", action="store_true",
@@ -338,6 +401,8 @@ def main():
         line["serving"] = serving_sample()
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(eps, L, G)
+    if args.cpu_extra:
+        line["cpu_side_by_side"] = cpu_side_by_side(stream)
     if args.cpu_c1:
         from eigenfns_amd import workloads as wl
         from eigenfns_amd.solve_bench import run_c1
