@@ -13,6 +13,7 @@ solver.py:228-238 / interior.py:75-82.
 from __future__ import annotations
 
 import os
+import threading
 
 import numpy as np
 import torch
@@ -392,6 +393,7 @@ def row_axpby_norm_strided(X, stride_x, a, Y, stride_y, b, nrow, m, n, out, stri
 
 
 _POOL = None
+_POOL_LOCK = threading.Lock()
 
 
 def _pool():
@@ -401,14 +403,15 @@ def _pool():
     cores (measured 48 ms for 21 eigensolves of 72 × 72 with 8 BLAS threads
     per call, 5–13 ms with one)."""
     global _POOL
-    if _POOL is None:
-        from concurrent.futures import ThreadPoolExecutor
-        try:
-            from threadpoolctl import threadpool_limits
-            threadpool_limits(limits=1, user_api="blas")
-        except ImportError:  # without threadpoolctl the BLAS keeps its threads
-            pass
-        _POOL = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+    with _POOL_LOCK:  # serving engines on several host threads may get here together
+        if _POOL is None:
+            from concurrent.futures import ThreadPoolExecutor
+            try:
+                from threadpoolctl import threadpool_limits
+                threadpool_limits(limits=1, user_api="blas")
+            except ImportError:  # without threadpoolctl the BLAS keeps its threads
+                pass
+            _POOL = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
     return _POOL
 
 
