@@ -101,14 +101,13 @@ class BatchedLOBPCG:
         self.normalized_rq = normalized_rq
         self.verify_dead_rows = verify_dead_rows  # also measure the row norms and compare (tests)
         # the SVQB / whitening / Rayleigh–Ritz eigensolves: "host" (LAPACK over a
-        # thread pool, the default), "device" (efn_heev_batched, no host round
-        # trip) or "hybrid" (device up to 48 × 48, i.e. SVQB and whitening; host
-        # above).  Measured on 48 serving requests at block 24 with two engines
-        # per grid: host 7.57, hybrid 7.61, device 6.4 solves/s — the device
-        # Jacobi takes 7.7 ms per 72 × 72 Rayleigh–Ritz matrix (its eigenvectors
-        # live in global memory: A and V do not both fit in LDS at n = 72)
-        # against 0.4 ms for one host LAPACK call, and the engines' syncs are
-        # already overlapped by the second engine per grid.
+        # thread pool, masks and normalisation in NumPy; the default), "device"
+        # (efn_heev_batched and the masks on the device: no host round trip) or
+        # "hybrid" (SVQB and whitening on the device, Rayleigh–Ritz on the
+        # host).  The device Jacobi takes 7.7 ms per 72 × 72 Rayleigh–Ritz
+        # matrix (its eigenvectors live in global memory: A and V do not both
+        # fit in LDS at n = 72) against 0.4 ms for one host LAPACK call, and the
+        # engines' syncs are already overlapped by the second engine per grid.
         if small_dense not in ("device", "host", "hybrid"):
             raise ValueError(f"small_dense must be 'device', 'host' or 'hybrid', got {small_dense!r}")
         self.small_dense = small_dense
@@ -216,14 +215,6 @@ class BatchedLOBPCG:
         C = gram_multi(Ls, Ts)
         combine_multi(-C, Ls, Ts, Xs=Ts, alpha=1.0)
 
-    def _eigh(self, G, lowest=None):
-        """Ascending eigenpairs of a (nprob, k, k) complex128 device stack, as
-        device tensors; on the device or through host LAPACK (self.small_dense)."""
-        if self.small_dense == "device" or (self.small_dense == "hybrid" and G.shape[-1] <= 48):
-            return eigh_batched_device(G, lowest)
-        w, V = eigh_stack_host(G.cpu().numpy(), lowest)
-        return (torch.as_tensor(w, device=G.device), torch.as_tensor(V, device=G.device))
-
     @staticmethod
     def _whiten(w, V):
         """V·w^{-1/2} with directions below SVQB_DROP·max(w) (or all, if max(w) = 0)
@@ -234,16 +225,81 @@ class BatchedLOBPCG:
                             torch.zeros_like(w))
         return V * scale[:, None, :], keep
 
+    def _whiten_small(self, G):
+        """(V·w^{-1/2} with dropped directions zeroed, keep mask) of a Gram stack:
+        on the host in NumPy when the eigensolve is there (one copy of the
+        coefficients back; masks as NumPy), on the device otherwise."""
+        if self.small_dense == "host":
+            w, V = eigh_stack_host(G.cpu().numpy())
+            wmax = w.max(axis=1, keepdims=True)
+            keep = (w > SVQB_DROP * np.maximum(wmax, 1e-300)) & (wmax > 0)
+            scale = np.where(keep, 1.0 / np.sqrt(np.where(keep, w, 1.0)), 0.0)
+            return torch.as_tensor(V * scale[:, None, :], device=G.device), keep
+        return self._whiten(*eigh_batched_device(G))
+
     def _svqb(self, idx, block, passes=2):
         """SVQB of block(s) for slots idx, two sweeps, written back in place.
-        Returns the last sweep's keep mask (len(idx), rows; device bool): a
-        dropped direction is an exactly zero row of the result."""
+        Returns the last sweep's keep mask (len(idx), rows): a dropped direction
+        is an exactly zero row of the result."""
         keep = None
         for _ in range(passes):
             Bs = [block(s) for s in idx]
-            Cw, keep = self._whiten(*self._eigh(gram_multi(Bs, Bs, hermitian=True)))
+            Cw, keep = self._whiten_small(gram_multi(Bs, Bs, hermitian=True))
             self._combine_back(idx, Cw, Bs)
         return keep
+
+    def _check_dead(self, idx, dead):
+        """Compare the dead-row bookkeeping with measured row norms (tests)."""
+        m = self.m
+        rn2 = rowreduce_strided(0, self.S, None, 3 * m, 3 * m, 3 * m, self.nslots * 3 * m,
+                                self.n).real.cpu().numpy().reshape(self.nslots, 3 * m)
+        for j, s in enumerate(idx):
+            if not np.array_equal(dead[j], np.sqrt(rn2[s]) < 0.5):
+                raise AssertionError(f"slot {s}: dead-row bookkeeping disagrees with row norms")
+
+    def _rr_host(self, idx, A, keep_w, keepp):
+        """Rayleigh–Ritz coefficients [c/‖c‖ | c'/‖c'‖] with the eigensolve on the
+        host: penalty, eigensolve and normalisation in NumPy, one copy back.
+        Dead rows (solver.py:296-297, norm < 0.5) without another pass over S:
+        X norms from this step's residual statistics; W and P rows are unit or
+        exactly zero after SVQB / whitening (their keep masks)."""
+        m = self.m
+        A = A.cpu().numpy()
+        keep_w = keep_w.cpu().numpy() if torch.is_tensor(keep_w) else keep_w
+        keepp = keepp.cpu().numpy() if torch.is_tensor(keepp) else keepp
+        dead = np.stack([~np.concatenate([self._xn[s] >= 0.5, keep_w[j], keepp[j]])
+                         for j, s in enumerate(idx)])
+        if self.verify_dead_rows:
+            self._check_dead(idx, dead)
+        for j in range(len(idx)):   # symmetrised, dead rows penalised (solver.py:294-297)
+            A[j] = (A[j] + A[j].conj().T) / 2
+            penalty = max(RR_DEAD_PENALTY, 100.0 * float(np.abs(np.diag(A[j])).max()))
+            A[j][np.diag_indices(3 * m)] += np.where(dead[j], penalty, 0.0)
+        _, C = eigh_stack_host(A, lowest=m)
+        Cp = C.copy()
+        Cp[:, :m] = 0
+        nx = np.maximum(np.linalg.norm(C, axis=1), 1e-20)
+        npn = np.maximum(np.linalg.norm(Cp, axis=1), 1e-20)
+        return torch.as_tensor(np.concatenate([C / nx[:, None, :], Cp / npn[:, None, :]], axis=2),
+                               device=self.S.device)
+
+    def _rr_device(self, idx, A, keep_w, keepp):
+        """_rr_host with the penalty, eigensolve and normalisation on the device."""
+        m = self.m
+        xn = torch.stack([self._xn_dev[s] for s in idx])   # views: no host-to-device copy
+        dead = ~torch.cat([xn >= 0.5, keep_w, keepp], dim=1)
+        if self.verify_dead_rows:
+            self._check_dead(idx, dead.cpu().numpy())
+        A = (A + A.conj().transpose(1, 2)) / 2
+        penalty = torch.clamp(100.0 * torch.diagonal(A, dim1=1, dim2=2).abs().max(dim=1).values,
+                              min=RR_DEAD_PENALTY)
+        A = A + torch.diag_embed(torch.where(dead, penalty[:, None], 0.0).to(A.dtype))
+        _, C = eigh_batched_device(A, lowest=m)
+        Cp = C.clone()
+        Cp[:, :m] = 0
+        nx = torch.clamp(torch.linalg.vector_norm(C, dim=1), min=1e-20)
+        npn = torch.clamp(torch.linalg.vector_norm(Cp, dim=1), min=1e-20)
+        return torch.cat([C / nx[:, None, :], Cp / npn[:, None, :]], dim=2)
 
     def _theta(self, idx, src, row0_src, dst, row0_dst, stride_src, stride_dst, nrow):
         for c in range(0, len(idx), MAX_PROBLEMS):
@@ -300,7 +356,7 @@ class BatchedLOBPCG:
             Ps = [self._P(s) for s in idx]
             combine_multi(-gram_multi(XWs, Ps), XWs, Ps, Xs=Ps, alpha=1.0)
         Ps = [self._P(s) for s in idx]
-        Cp, keepp = self._whiten(*self._eigh(gram_multi(Ps, Ps, hermitian=True)))
+        Cp, keepp = self._whiten_small(gram_multi(Ps, Ps, hermitian=True))
         self._combine_back(idx, Cp, Ps)
         t = self._tick("P", t)
         # Θ(W), Θ(P) fresh; Rayleigh–Ritz on [X; W; P] (solver.py:275-302)
@@ -308,36 +364,17 @@ class BatchedLOBPCG:
         t = self._tick("theta_WP", t)
         Sbs = [self._Sb(s) for s in idx]
         A = gram_multi(Sbs, [self._HSb(s) for s in idx], hermitian=True)
-        # dead rows (solver.py:296-297, norm < 0.5) without another pass over S:
-        # X norms come from this step's residual statistics; W and P rows are
-        # unit or exactly zero after SVQB / whitening (their keep masks)
-        xn = torch.stack([self._xn_dev[s] for s in idx])   # views: no host-to-device copy
-        dead = ~torch.cat([xn >= 0.5, keep_w, keepp], dim=1)
-        if self.verify_dead_rows:
-            rn2 = rowreduce_strided(0, self.S, None, 3 * m, 3 * m, 3 * m, self.nslots * 3 * m,
-                                    self.n).real.cpu().numpy().reshape(self.nslots, 3 * m)
-            dh = dead.cpu().numpy()
-            for j, s in enumerate(idx):
-                if not np.array_equal(dh[j], np.sqrt(rn2[s]) < 0.5):
-                    raise AssertionError(f"slot {s}: dead-row bookkeeping disagrees with row norms")
-        # symmetrised, dead rows penalised (solver.py:294-297)
-        A = (A + A.conj().transpose(1, 2)) / 2
-        penalty = torch.clamp(100.0 * torch.diagonal(A, dim1=1, dim2=2).abs().max(dim=1).values,
-                              min=RR_DEAD_PENALTY)
-        A = A + torch.diag_embed(torch.where(dead, penalty[:, None], 0.0).to(A.dtype))
-        _, C = self._eigh(A, lowest=m)
-        t = self._tick("RR_gram+eigh", t)
         # X ← S·c_i/‖S·c_i‖ and P ← S·c'_i/‖S·c'_i‖ (c' = c with its X rows
         # zeroed; solver.py:298-307).  [X; W; P] is orthonormal by construction
         # (SVQB, projections, whitening) and dead rows get exactly zero weight,
         # so ‖S·c‖ = ‖c‖ to FP32 rounding: the norms of the coefficient vectors
         # are folded into them, and one split combine writes both updates
         # straight into the slot's X and P rows.
-        Cp = C.clone()
-        Cp[:, :m] = 0
-        nx = torch.clamp(torch.linalg.vector_norm(C, dim=1), min=1e-20)
-        npn = torch.clamp(torch.linalg.vector_norm(Cp, dim=1), min=1e-20)
-        C2 = torch.cat([C / nx[:, None, :], Cp / npn[:, None, :]], dim=2)
+        if self.small_dense == "device":
+            C2 = self._rr_device(idx, A, keep_w, keepp)
+        else:
+            C2 = self._rr_host(idx, A, keep_w, keepp)
+        t = self._tick("RR_gram+eigh", t)
         Xs = [self._X(s) for s in idx]
         Ps = [self._P(s) for s in idx]
         if 2 * m <= COMBINE_IN_PLACE_MAX:
@@ -408,11 +445,12 @@ class BatchedLOBPCG:
             lam = lam / torch.clamp(xn2, min=1e-30)
         rr2 = row_axpby_norm_strided(self.HS, 3 * m, torch.ones_like(lam), self.S, 3 * m, -lam, m,
                                      M, self.n, self.R, m)
-        self._xn_dev = torch.sqrt(xn2).view(self.nslots, m)
+        self._xn_dev = torch.sqrt(xn2).view(self.nslots, m)   # device masks (small_dense="device")
         stats = torch.stack([lam, xn2, rr2, hx2]).cpu().numpy()
         t = self._tick("residuals+sync", t)
         lam_h = stats[0].reshape(self.nslots, m)
         xn = np.sqrt(stats[1]).reshape(self.nslots, m)
+        self._xn = xn
         rn_h = (np.sqrt(stats[2]) / np.maximum(np.sqrt(stats[3]), 1e-30)).reshape(self.nslots, m)
         cont = []
         for s in act:
