@@ -17,7 +17,15 @@ def device() -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
 
 
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream_ptr() -> int:
+    """hipStream_t of the current stream.  Called once per library call, so it
+    reads the raw handle directly (≈1 µs) rather than building a Stream
+    object (≈9 µs, measured as the largest Python cost of a serving step)."""
+    if _RAW_STREAM is not None:
+        return _RAW_STREAM(torch.cuda.current_device())
     return torch.cuda.current_stream().cuda_stream
 
 
