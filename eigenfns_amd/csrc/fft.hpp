@@ -231,16 +231,18 @@ __device__ __forceinline__ void step1_store(float2 (&v)[Plan<N>::N1], int n2, fl
     if (S > 0) w.y = -w.y;
     v[k1] = cmul(v[k1], w);
   }
+  float2* col = pencil + n2;  // pos(N2·k1 + n2) = (N2 + 1)·k1 + n2 for n2 < N2
 #pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) pencil[P::pos(N2 * k1 + n2)] = v[k1];
+  for (int k1 = 0; k1 < N1; ++k1) col[(N2 + 1) * k1] = v[k1];
 }
 
 template <int N, int S>
 __device__ __forceinline__ void step3_load(float2 (&r)[Plan<N>::N2], int k1, const float2* __restrict__ pencil) {
   using P = Plan<N>;
   constexpr int N2 = P::N2;
+  const float2* run = pencil + (N2 + 1) * k1;  // pos(N2·k1 + n2) = (N2 + 1)·k1 + n2
 #pragma unroll
-  for (int n2 = 0; n2 < N2; ++n2) r[n2] = pencil[P::pos(N2 * k1 + n2)];
+  for (int n2 = 0; n2 < N2; ++n2) r[n2] = run[n2];
   DFT<N2, S>::run(r);
 }
 
@@ -288,7 +290,7 @@ __device__ __forceinline__ void lds_fft(float2* __restrict__ buf, const float2* 
     float2* base = buf + p * P::kStride;
     float2 v[N1];
 #pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[P::pos(N2 * n1 + n2)];
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[(N2 + 1) * n1 + n2];  // pos(N2·n1 + n2)
     step1_store<N, S>(v, n2, base, twid);
   }
   __syncthreads();

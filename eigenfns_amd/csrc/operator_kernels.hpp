@@ -20,6 +20,19 @@
 
 #include "fft.hpp"
 
+#ifndef EFN_GLD
+#define EFN_GLD 1
+#endif
+#ifndef EFN_FRAME_RSQ
+#define EFN_FRAME_RSQ 1
+#endif
+#ifndef EFN_ZWAVE_ROWS
+#define EFN_ZWAVE_ROWS 1
+#endif
+#ifndef EFN_TWID_LATE
+#define EFN_TWID_LATE 1
+#endif
+
 namespace efn {
 
 enum OpKind { kTheta = 0, kPrecond = 1, kCartH = 2 };
@@ -64,8 +77,9 @@ struct VecRef {
 };
 
 __device__ __forceinline__ VecRef map_vector(const VecMap& vm, long long gv) {
-  const int slot = int(gv / vm.nrow);
-  const long long row = gv - (long long)slot * vm.nrow;
+  // a launch covers fewer than 2³¹ vectors: 32-bit division
+  const int slot = int(gv) / vm.nrow;
+  const long long row = int(gv) - slot * vm.nrow;
   VecRef r;
   r.desc = vm.desc[slot];
   r.in_vec = vm.in_off[slot] + row;
@@ -83,33 +97,63 @@ struct RowFrame {
   float Kx, Ky, rho2, rrho;
 };
 
+// Loads through pointers that were themselves read from memory (the operator
+// descriptor's tables) compile to flat loads, which count against the LDS
+// counter too and so make every LDS wait also wait for them; these tables are
+// always in global memory, so load them as such.
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+#if EFN_GLD
+  return *(const __attribute__((address_space(1))) T*)p;
+#else
+  return *p;
+#endif
+}
+
+// 1/√x of the frame: the hardware reciprocal square root (≤ 1 ulp) in place
+// of a correctly rounded square root and division (about 20 instructions).
+__device__ __forceinline__ float frame_rsqrt(float x) {
+#if EFN_FRAME_RSQ
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / sqrtf(x);
+#endif
+}
+
 __device__ __forceinline__ RowFrame row_frame(const FrameTables& ft, int x, int y) {
   RowFrame rf;
-  rf.Kx = ft.kx[x];
-  rf.Ky = ft.ky[y];
+  rf.Kx = gld(ft.kx + x);
+  rf.Ky = gld(ft.ky + y);
   rf.rho2 = fmaf(rf.Kx, rf.Kx, rf.Ky * rf.Ky);
-  rf.rrho = 1.0f / sqrtf(rf.rho2);
+  rf.rrho = frame_rsqrt(rf.rho2);
   return rf;
 }
 
-__device__ __forceinline__ Frame point_frame(const FrameTables& ft, const RowFrame& rf, int z, long long vox) {
-  const bool sw = (ft.swap[vox >> 5] >> (vox & 31)) & 1u;
-  const bool sing = vox == ft.sing_voxel;
-  const float Kx = rf.Kx, Ky = rf.Ky, Kz = ft.kz[z];
+// Branch-free: the reference axis picks operands, one product per component;
+// at K = 0 both scale factors are zero, which zeroes every component.
+__device__ __forceinline__ Frame frame_at(const RowFrame& rf, float Kz, bool sw, bool sing) {
+  const float Kx = rf.Kx, Ky = rf.Ky;
   const float kn2 = fmaf(Kz, Kz, rf.rho2);
-  const float rkn = 1.0f / sqrtf(kn2);
+  const float rkn = frame_rsqrt(kn2);
   const float sig2 = fmaf(Ky, Ky, Kz * Kz);
-  const float rsig = 1.0f / sqrtf(sig2);
+  const float rsig = frame_rsqrt(sig2);
+  const float r1 = sing ? 0.f : (sw ? rsig : rf.rrho);  // 1/ρ (axis ẑ) or 1/σ (axis x̂)
+  const float r2 = sing ? 0.f : r1 * rkn;
   Frame f;
-  const float a = rf.rrho * rkn, b = rsig * rkn;
   f.kn = sing ? 0.f : kn2 * rkn;
-  f.t1x = sing ? 0.f : (sw ? 0.f : Ky * rf.rrho);
-  f.t1y = sing ? 0.f : (sw ? Kz * rsig : -Kx * rf.rrho);
-  f.t1z = sing ? 0.f : (sw ? -Ky * rsig : 0.f);
-  f.t2x = sing ? 0.f : (sw ? -sig2 * b : Kx * Kz * a);
-  f.t2y = sing ? 0.f : (sw ? Kx * Ky * b : Ky * Kz * a);
-  f.t2z = sing ? 0.f : (sw ? Kx * Kz * b : -rf.rho2 * a);
+  f.t1x = (sw ? 0.f : Ky) * r1;
+  f.t1y = (sw ? Kz : -Kx) * r1;
+  f.t1z = (sw ? -Ky : 0.f) * r1;
+  f.t2x = (sw ? -sig2 : Kx * Kz) * r2;
+  f.t2y = (Ky * (sw ? Kx : Kz)) * r2;
+  f.t2z = (sw ? Kx * Kz : -rf.rho2) * r2;
   return f;
+}
+
+__device__ __forceinline__ bool swap_bit(uint32_t word, long long vox) { return (word >> (vox & 31)) & 1u; }
+
+__device__ __forceinline__ Frame point_frame(const FrameTables& ft, const RowFrame& rf, int z, long long vox) {
+  return frame_at(rf, gld(ft.kz + z), swap_bit(gld(ft.swap + (vox >> 5)), vox), vox == ft.sing_voxel);
 }
 
 // z-tile width of the strided (y, x) passes: 16 complex = 128 B runs.
@@ -160,6 +204,34 @@ constexpr int kThreads = 256;
 // Cartesian components in natural order to LDS; the four-step FFT runs on the
 // 3·R pencils; step-3 outputs go straight to HBM.
 template <int G, int KIND>
+__device__ __forceinline__ void curl_in(const Frame& f, float2 a, float2 b, float kn_floor, float2 (&o)[3]) {
+  float2 c1, c2;
+  if (KIND == kTheta) {  // c1 = −i·kn·b, c2 = i·kn·a
+    c1 = make_float2(f.kn * b.y, -f.kn * b.x);
+    c2 = make_float2(-f.kn * a.y, f.kn * a.x);
+  } else if (KIND == kCartH) {  // Cartesian H = a·t̂₁ + b·t̂₂ (operator.py:169-172)
+    c1 = a;
+    c2 = b;
+  } else {               // c1 = i·b/knr, c2 = −i·a/knr
+    const float inv = 1.0f / fmaxf(f.kn, kn_floor);
+    c1 = make_float2(-b.y * inv, b.x * inv);
+    c2 = make_float2(a.y * inv, -a.x * inv);
+  }
+  o[0] = make_float2(fmaf(c1.x, f.t1x, c2.x * f.t2x), fmaf(c1.y, f.t1x, c2.y * f.t2x));
+  o[1] = make_float2(fmaf(c1.x, f.t1y, c2.x * f.t2y), fmaf(c1.y, f.t1y, c2.y * f.t2y));
+  o[2] = make_float2(fmaf(c1.x, f.t1z, c2.x * f.t2z), fmaf(c1.y, f.t1z, c2.y * f.t2z));
+}
+
+// Wave-row layout of the z passes' pointwise phase: when a row is a whole
+// number of 64-point runs and the R rows split evenly over the four waves,
+// wave w owns rows w·R/4 .. (w+1)·R/4 − 1, so (x, y) and the row frame are
+// uniform across the wave and each lane's points are z = lane + 64·j.
+template <int G>
+__host__ __device__ constexpr bool z_wave_rows() {
+  return EFN_ZWAVE_ROWS && G % 64 == 0 && zrows3<G>() % (kThreads / 64) == 0;
+}
+
+template <int G, int KIND>
 __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict__ in,
                                                        float2* __restrict__ ws, VecMap vm,
                                                        long long v0,
@@ -170,7 +242,9 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
   __shared__ float2 buf[3 * R * P::kStride];
   __shared__ float2 twid[G];
+#if !EFN_TWID_LATE
   for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+#endif
   const long long row0 = (long long)blockIdx.x * R;  // flattened (v, x, y)
   const int v = int(row0 / G2);
   const long long xy0 = row0 - (long long)v * G2;
@@ -178,53 +252,93 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
   const FrameTables ft = vr.desc->ft;
   const float2* a_in = in + vr.in_vec * 2 * G3;
   const float2* b_in = a_in + G3;
-  // every input point of this thread loaded before any frame arithmetic, so
-  // the loads are in flight together (measured +1.3 % on Θ at 192³)
-  constexpr int NE = (R * G + kThreads - 1) / kThreads;
-  float2 apre[NE], bpre[NE];
+  if constexpr (z_wave_rows<G>()) {
+    constexpr int RW = R / (kThreads / 64);  // rows per wave
+    constexpr int ZJ = G / 64;               // points per lane per row
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int rw0 = w * RW;
+    // frame tables first, then the data: vector loads complete in issue order,
+    // so the frame arithmetic can start while the data is still in flight
+    float kz[ZJ];
+    uint32_t swp[RW][ZJ];
 #pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    const int e = threadIdx.x + k * kThreads;
-    if ((R * G) % kThreads == 0 || e < R * G) {
-      const long long vox = xy0 * G + e;
-      apre[k] = a_in[vox];
-      bpre[k] = b_in[vox];
+    for (int j = 0; j < ZJ; ++j) kz[j] = gld(ft.kz + lane + 64 * j);
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int j = 0; j < ZJ; ++j) {
+        const int vox = int((xy0 + rw0 + r) * G) + lane + 64 * j;
+        swp[r][j] = gld(ft.swap + (vox >> 5));
+      }
+    __builtin_amdgcn_sched_barrier(0);  // keep the table loads ahead of the data
+    float2 apre[RW][ZJ], bpre[RW][ZJ];
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int j = 0; j < ZJ; ++j) {
+        const int vox = int((xy0 + rw0 + r) * G) + lane + 64 * j;
+        apre[r][j] = a_in[vox];
+        bpre[r][j] = b_in[vox];
+      }
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int xy = int(xy0) + rw0 + r;
+      const int x = xy / G, y = xy - x * G;
+      const RowFrame rf = row_frame(ft, x, y);
+#pragma unroll
+      for (int j = 0; j < ZJ; ++j) {
+        const int z = lane + 64 * j;
+        const int vox = xy * G + z;
+        const Frame f = frame_at(rf, kz[j], swap_bit(swp[r][j], vox), vox == ft.sing_voxel);
+        float2 o[3];
+        curl_in<G, KIND>(f, apre[r][j], bpre[r][j], ft.kn_floor, o);
+        const int p = P::pos(z);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) buf[(c * R + rw0 + r) * P::kStride + p] = o[c];
+      }
+    }
+  } else {
+    // every input point of this thread loaded before any frame arithmetic, so
+    // the loads are in flight together (measured +1.3 % on Θ at 192³)
+    constexpr int NE = (R * G + kThreads - 1) / kThreads;
+    float2 apre[NE], bpre[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int e = threadIdx.x + k * kThreads;
+      if ((R * G) % kThreads == 0 || e < R * G) {
+        const long long vox = xy0 * G + e;
+        apre[k] = a_in[vox];
+        bpre[k] = b_in[vox];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int e = threadIdx.x + k * kThreads;
+      if (!((R * G) % kThreads == 0 || e < R * G)) continue;
+      const int r = e / G, z = e - r * G;
+      const long long xy = xy0 + r;
+      const int x = int(xy / G), y = int(xy - (long long)x * G);
+      const long long vox = xy * G + z;
+      const Frame f = point_frame(ft, row_frame(ft, x, y), z, vox);
+      float2 o[3];
+      curl_in<G, KIND>(f, apre[k], bpre[k], ft.kn_floor, o);
+      const int p = P::pos(z);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) buf[(c * R + r) * P::kStride + p] = o[c];
     }
   }
-#pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    const int e = threadIdx.x + k * kThreads;
-    if (!((R * G) % kThreads == 0 || e < R * G)) continue;
-    const int r = e / G, z = e - r * G;
-    const long long xy = xy0 + r;
-    const int x = int(xy / G), y = int(xy - (long long)x * G);
-    const long long vox = xy * G + z;
-    const float2 a = apre[k], b = bpre[k];
-    const Frame f = point_frame(ft, row_frame(ft, x, y), z, vox);
-    float2 c1, c2;
-    if (KIND == kTheta) {  // c1 = −i·kn·b, c2 = i·kn·a
-      c1 = make_float2(f.kn * b.y, -f.kn * b.x);
-      c2 = make_float2(-f.kn * a.y, f.kn * a.x);
-    } else if (KIND == kCartH) {  // Cartesian H = a·t̂₁ + b·t̂₂ (operator.py:169-172)
-      c1 = a;
-      c2 = b;
-    } else {               // c1 = i·b/knr, c2 = −i·a/knr
-      const float inv = 1.0f / fmaxf(f.kn, ft.kn_floor);
-      c1 = make_float2(-b.y * inv, b.x * inv);
-      c2 = make_float2(a.y * inv, -a.x * inv);
-    }
-    const int p = P::pos(z);
-    buf[(0 * R + r) * P::kStride + p] = make_float2(fmaf(c1.x, f.t1x, c2.x * f.t2x), fmaf(c1.y, f.t1x, c2.y * f.t2x));
-    buf[(1 * R + r) * P::kStride + p] = make_float2(fmaf(c1.x, f.t1y, c2.x * f.t2y), fmaf(c1.y, f.t1y, c2.y * f.t2y));
-    buf[(2 * R + r) * P::kStride + p] = make_float2(fmaf(c1.x, f.t1z, c2.x * f.t2z), fmaf(c1.y, f.t1z, c2.y * f.t2z));
-  }
+#if EFN_TWID_LATE
+  // staged after the data loads are issued, so its round trip overlaps theirs
+  for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+#endif
   __syncthreads();
   for (int t = threadIdx.x; t < 3 * R * N2; t += kThreads) {
     const int p = t / N2, n2 = t - p * N2;
     float2* pen = buf + p * P::kStride;
     float2 vv[N1];
 #pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) vv[n1] = pen[P::pos(N2 * n1 + n2)];
+    for (int n1 = 0; n1 < N1; ++n1) vv[n1] = pen[(N2 + 1) * n1 + n2];  // pos(N2·n1 + n2)
     step1_store<G, +1>(vv, n2, pen, twid);
   }
   __syncthreads();
@@ -265,8 +379,10 @@ __global__ __launch_bounds__(kThreads) void strided_kernel(float2* __restrict__ 
   constexpr long long stride = AXIS == 1 ? G : G2;
   __shared__ float2 buf[ZT * P::kStride];
   __shared__ float2 twid[G];
+#if !EFN_TWID_LATE
   for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
   __syncthreads();
+#endif
   constexpr int nzt = (G + ZT - 1) / ZT;
   const long long fo = blockIdx.x / nzt;  // (field, other-axis index)
   const int zt = blockIdx.x - int(fo * nzt);
@@ -274,13 +390,28 @@ __global__ __launch_bounds__(kThreads) void strided_kernel(float2* __restrict__ 
   const int o = int(fo - field * G);
   const int z0 = zt * ZT;
   float2* base = ws + field * G3 + (AXIS == 1 ? (long long)o * G2 : (long long)o * G) + z0;
-  for (int t = threadIdx.x; t < ZT * N2; t += kThreads) {
+  auto active = [&](int t) { return t < ZT * N2 && z0 + t % ZT < G; };
+  auto load1 = [&](int t, float2 (&v)[N1]) {
     const int z = t % ZT, n2 = t / ZT;
-    if (z0 + z < G) {
-      float2 v[N1];
 #pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[(N2 * n1 + n2) * stride + z];
-      step1_store<G, S>(v, n2, buf + z * P::kStride, twid);
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[(N2 * n1 + n2) * stride + z];
+  };
+  int t = threadIdx.x;
+#if EFN_TWID_LATE
+  {  // the first step-1 inputs are in flight before the twiddle table is staged
+    float2 v[N1];
+    if (active(t)) load1(t, v);
+    for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+    __syncthreads();
+    if (active(t)) step1_store<G, S>(v, t / ZT, buf + (t % ZT) * P::kStride, twid);
+    t += kThreads;
+  }
+#endif
+  for (; t < ZT * N2; t += kThreads) {
+    if (active(t)) {
+      float2 v[N1];
+      load1(t, v);
+      step1_store<G, S>(v, t / ZT, buf + (t % ZT) * P::kStride, twid);
     }
   }
   __syncthreads();
@@ -315,8 +446,10 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   constexpr int kBufStride = kFStride > P::kStride ? kFStride : P::kStride;
   __shared__ float2 buf[ZT * kBufStride];
   __shared__ float2 twid[G];
+#if !EFN_TWID_LATE
   for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
   __syncthreads();
+#endif
   const long long b = blockIdx.x;
   long long tile;
   int c;
@@ -338,13 +471,28 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   constexpr int T3 = ZT * N1;
   constexpr int TPT = (T3 + kThreads - 1) / kThreads;
   // inverse step 1: HBM → registers → LDS
-  for (int t = threadIdx.x; t < ZT * N2; t += kThreads) {
+  auto active = [&](int t) { return t < ZT * N2 && z0 + t % ZT < G; };
+  auto load1 = [&](int t, float2 (&vv)[N1]) {
     const int z = t % ZT, n2 = t / ZT;
-    if (z0 + z < G) {
-      float2 vv[N1];
 #pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) vv[n1] = base[(N2 * n1 + n2) * G2 + z];
-      step1_store<G, +1>(vv, n2, buf + z * P::kStride, twid);
+    for (int n1 = 0; n1 < N1; ++n1) vv[n1] = base[(N2 * n1 + n2) * G2 + z];
+  };
+  int t1 = threadIdx.x;
+#if EFN_TWID_LATE
+  {  // the first step-1 inputs are in flight before the twiddle table is staged
+    float2 vv[N1];
+    if (active(t1)) load1(t1, vv);
+    for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+    __syncthreads();
+    if (active(t1)) step1_store<G, +1>(vv, t1 / ZT, buf + (t1 % ZT) * P::kStride, twid);
+    t1 += kThreads;
+  }
+#endif
+  for (; t1 < ZT * N2; t1 += kThreads) {
+    if (active(t1)) {
+      float2 vv[N1];
+      load1(t1, vv);
+      step1_store<G, +1>(vv, t1 / ZT, buf + (t1 % ZT) * P::kStride, twid);
     }
   }
   __syncthreads();
@@ -358,7 +506,7 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
       step3_load<G, +1>(r[j], k1, buf + z * P::kStride);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
-        const float sc = eps[(long long)(k1 + N1 * k2) * G2 + z];
+        const float sc = gld(eps + (long long)(k1 + N1 * k2) * G2 + z);
         r[j][k2] = cscale(r[j][k2], sc);
       }
     }
@@ -422,6 +570,61 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;  // valid in thread 0
 }
 
+// The pointwise epilogue of zout for one voxel: project the three Cartesian
+// components d on t̂₁, t̂₂, curl back, and apply the fused update at output
+// index i0/i1 (components 0/1) and input index j0/j1.
+template <int KIND, int EPI>
+__device__ __forceinline__ void zout_point(const Frame& f, float2 d0, float2 d1, float2 d2, float kn_floor,
+                                           const EpiArgs& ea, long long i0, long long i1, long long j0,
+                                           long long j1, double& s0, double& s1) {
+  const float2 f1 = make_float2(fmaf(f.t1x, d0.x, fmaf(f.t1y, d1.x, f.t1z * d2.x)),
+                                fmaf(f.t1x, d0.y, fmaf(f.t1y, d1.y, f.t1z * d2.y)));
+  const float2 f2 = make_float2(fmaf(f.t2x, d0.x, fmaf(f.t2y, d1.x, f.t2z * d2.x)),
+                                fmaf(f.t2x, d0.y, fmaf(f.t2y, d1.y, f.t2z * d2.y)));
+  const float kn = f.kn;
+  float2 o0, o1;
+  if (KIND == kTheta) {  // (−i·kn·f2, i·kn·f1)
+    o0 = make_float2(kn * f2.y, -kn * f2.x);
+    o1 = make_float2(-kn * f1.y, kn * f1.x);
+  } else {               // (i·f2/knr, −i·f1/knr)
+    const float inv = 1.0f / fmaxf(kn, kn_floor);
+    o0 = make_float2(-f2.y * inv, f2.x * inv);
+    o1 = make_float2(f1.y * inv, -f1.x * inv);
+  }
+  if (EPI == kStore) {
+    if (ea.beta != 0.f) {
+      const float2 x0 = ea.x[j0], x1 = ea.x[j1];
+      o0 = make_float2(fmaf(ea.alpha, o0.x, ea.beta * x0.x), fmaf(ea.alpha, o0.y, ea.beta * x0.y));
+      o1 = make_float2(fmaf(ea.alpha, o1.x, ea.beta * x1.x), fmaf(ea.alpha, o1.y, ea.beta * x1.y));
+    } else if (ea.alpha != 1.f) {
+      o0 = cscale(o0, ea.alpha);
+      o1 = cscale(o1, ea.alpha);
+    }
+    ea.out[i0] = o0;
+    ea.out[i1] = o1;
+  } else {
+    // T_{k+1} = 2·(alpha·Θ T_k − T_k) − T_{k−1}   (interior.py:341-348, exp_kpm_dos.py:148)
+    const float2 x0 = ea.x[j0], x1 = ea.x[j1];
+    const float2 p0 = ea.out[i0], p1 = ea.out[i1];
+    const float2 n0 = make_float2(2.f * fmaf(ea.alpha, o0.x, -x0.x) - p0.x, 2.f * fmaf(ea.alpha, o0.y, -x0.y) - p0.y);
+    const float2 n1 = make_float2(2.f * fmaf(ea.alpha, o1.x, -x1.x) - p1.x, 2.f * fmaf(ea.alpha, o1.y, -x1.y) - p1.y);
+    ea.out[i0] = n0;
+    ea.out[i1] = n1;
+    if (EPI == kCheb) {
+      const float2 y0 = ea.y[i0], y1 = ea.y[i1];
+      ea.y[i0] = make_float2(fmaf(ea.ck, n0.x, y0.x), fmaf(ea.ck, n0.y, y0.y));
+      ea.y[i1] = make_float2(fmaf(ea.ck, n1.x, y1.x), fmaf(ea.ck, n1.y, y1.y));
+    } else if (EPI == kKpm) {
+      // ⟨T_k, T_k⟩ and Re⟨T_{k+1}, T_k⟩ in FP64 (products of FP32 are exact in FP64)
+      s0 += (double)x0.x * x0.x + (double)x0.y * x0.y + (double)x1.x * x1.x + (double)x1.y * x1.y;
+      s1 += (double)n0.x * x0.x + (double)n0.y * x0.y + (double)n1.x * x1.x + (double)n1.y * x1.y;
+    } else {  // kDot: Re⟨z, T_{k+1}⟩
+      const float2 z0 = ea.z[i0], z1 = ea.z[i1];
+      s0 += (double)z0.x * n0.x + (double)z0.y * n0.y + (double)z1.x * n1.x + (double)z1.y * n1.y;
+    }
+  }
+}
+
 // zout: W (nvec,3,G³) → forward z-FFT → project on t̂₁, t̂₂ → curl back →
 // epilogue.  Step-1 inputs of the three components come straight from HBM;
 // step 3 writes back to LDS in natural order; the epilogue reads the three
@@ -435,94 +638,120 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
   constexpr int N1 = P::N1, N2 = P::N2;
   constexpr int R = zrows3<G>();
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
+  constexpr bool kWaveRows = z_wave_rows<G>();
+  constexpr int RW = kWaveRows ? R / (kThreads / 64) : 1;  // rows per wave
+  constexpr int ZJ = kWaveRows ? G / 64 : 1;               // points per lane per row
   __shared__ float2 buf[3 * R * P::kStride];
   __shared__ float2 twid[G];
   __shared__ double red[2 * kThreads / 64];
-  for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
-  __syncthreads();
   const long long row0 = (long long)blockIdx.x * R;
   const int v = int(row0 / G2);
   const long long xy0 = row0 - (long long)v * G2;
   const VecRef vr = map_vector(vm, v0 + v);
   const FrameTables ft = vr.desc->ft;
   const float2* w = ws + (long long)v * 3 * G3;
-  for (int t = threadIdx.x; t < 3 * R * N2; t += kThreads) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  // wave-row layout: the epilogue's frame tables are fetched first of all
+  float kz[ZJ];
+  uint32_t swp[RW][ZJ];
+  if constexpr (kWaveRows) {
+#pragma unroll
+    for (int j = 0; j < ZJ; ++j) kz[j] = gld(ft.kz + lane + 64 * j);
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int j = 0; j < ZJ; ++j) {
+        const int vox = int((xy0 + wv * RW + r) * G) + lane + 64 * j;
+        swp[r][j] = gld(ft.swap + (vox >> 5));
+      }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  constexpr int T1 = 3 * R * N2;
+  auto step1_src = [&](int t, float2 (&vv)[N1]) {
     const int p = t / N2, n2 = t - p * N2;
     const int c = p / R, r = p - c * R;
     const float2* src = w + c * G3 + (xy0 + r) * G;
-    float2 vv[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) vv[n1] = src[N2 * n1 + n2];
-    step1_store<G, -1>(vv, n2, buf + p * P::kStride, twid);
+  };
+#if EFN_TWID_LATE
+  // the first step-1 inputs are in flight before the twiddle table is staged
+  {
+    float2 vv[N1];
+    const int t = threadIdx.x;
+    if (t < T1) step1_src(t, vv);
+    for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+    __syncthreads();
+    if (t < T1) step1_store<G, -1>(vv, t % N2, buf + (t / N2) * P::kStride, twid);
+  }
+  for (int t = threadIdx.x + kThreads; t < T1; t += kThreads) {
+    float2 vv[N1];
+    step1_src(t, vv);
+    step1_store<G, -1>(vv, t % N2, buf + (t / N2) * P::kStride, twid);
+  }
+#else
+  for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+  __syncthreads();
+  for (int t = threadIdx.x; t < T1; t += kThreads) {
+    float2 vv[N1];
+    step1_src(t, vv);
+    step1_store<G, -1>(vv, t % N2, buf + (t / N2) * P::kStride, twid);
+  }
+#endif
+  __syncthreads();
+  // step 3 in place: the task (pencil, k1) reads its own run and writes its
+  // results X[k1 + N1·k2] back into that run at (N2 + 1)·k1 + k2 (no other task
+  // touches the run, so no barrier between the reads and the writes); the
+  // epilogue finds z at zpos(z) = (N2 + 1)·(z mod N1) + z / N1
+  for (int t = threadIdx.x; t < 3 * R * N1; t += kThreads) {
+    const int p = t / N1, k1 = t - p * N1;
+    float2 r[N2];
+    step3_load<G, -1>(r, k1, buf + p * P::kStride);
+    float2* run = buf + p * P::kStride + (N2 + 1) * k1;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) run[k2] = r[k2];
   }
   __syncthreads();
-  lds_step3_natural<G, -1, 3 * R, kThreads>(buf);
+  auto zpos = [](int z) { return (N2 + 1) * (z % N1) + z / N1; };
   const long long vbase = vr.out_vec * 2 * G3;   // out, y, z blocks
   const long long xbase = vr.in_vec * 2 * G3;    // the operator's input block
   double s0 = 0.0, s1 = 0.0;
-  // a fixed trip count, unrolled, so the epilogue loads of all of a thread's
-  // points can be issued together (+0.8 % on Θ, +1.4-1.8 % fused steps at 192³)
-  constexpr int NE = (R * G + kThreads - 1) / kThreads;
+  if constexpr (kWaveRows) {
 #pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    const int e = threadIdx.x + k * kThreads;
-    if ((R * G) % kThreads == 0 || e < R * G) {
-      const int r = e / G, z = e - r * G;
-      const long long xy = xy0 + r;
-      const int x = int(xy / G), y = int(xy - (long long)x * G);
-      const long long vox = xy * G + z;
-      const int p = P::pos(z);
-      const float2 d0 = buf[(0 * R + r) * P::kStride + p];
-      const float2 d1 = buf[(1 * R + r) * P::kStride + p];
-      const float2 d2 = buf[(2 * R + r) * P::kStride + p];
-      const Frame f = point_frame(ft, row_frame(ft, x, y), z, vox);
-      const float2 f1 = make_float2(fmaf(f.t1x, d0.x, fmaf(f.t1y, d1.x, f.t1z * d2.x)),
-                                    fmaf(f.t1x, d0.y, fmaf(f.t1y, d1.y, f.t1z * d2.y)));
-      const float2 f2 = make_float2(fmaf(f.t2x, d0.x, fmaf(f.t2y, d1.x, f.t2z * d2.x)),
-                                    fmaf(f.t2x, d0.y, fmaf(f.t2y, d1.y, f.t2z * d2.y)));
-      const float kn = f.kn;
-      float2 o0, o1;
-      if (KIND == kTheta) {  // (−i·kn·f2, i·kn·f1)
-        o0 = make_float2(kn * f2.y, -kn * f2.x);
-        o1 = make_float2(-kn * f1.y, kn * f1.x);
-      } else {               // (i·f2/knr, −i·f1/knr)
-        const float inv = 1.0f / fmaxf(kn, ft.kn_floor);
-        o0 = make_float2(-f2.y * inv, f2.x * inv);
-        o1 = make_float2(f1.y * inv, -f1.x * inv);
+    for (int r = 0; r < RW; ++r) {
+      const int rr = wv * RW + r;
+      const int xy = int(xy0) + rr;
+      const int x = xy / G, y = xy - x * G;
+      const RowFrame rf = row_frame(ft, x, y);
+#pragma unroll
+      for (int j = 0; j < ZJ; ++j) {
+        const int z = lane + 64 * j;
+        const int vox = xy * G + z;
+        const int p = zpos(z);
+        const Frame f = frame_at(rf, kz[j], swap_bit(swp[r][j], vox), vox == ft.sing_voxel);
+        zout_point<KIND, EPI>(f, buf[(0 * R + rr) * P::kStride + p], buf[(1 * R + rr) * P::kStride + p],
+                              buf[(2 * R + rr) * P::kStride + p], ft.kn_floor, ea, vbase + vox,
+                              vbase + G3 + vox, xbase + vox, xbase + G3 + vox, s0, s1);
       }
-      const long long i0 = vbase + vox, i1 = vbase + G3 + vox;
-      const long long j0 = xbase + vox, j1 = xbase + G3 + vox;
-      if (EPI == kStore) {
-        if (ea.beta != 0.f) {
-          const float2 x0 = ea.x[j0], x1 = ea.x[j1];
-          o0 = make_float2(fmaf(ea.alpha, o0.x, ea.beta * x0.x), fmaf(ea.alpha, o0.y, ea.beta * x0.y));
-          o1 = make_float2(fmaf(ea.alpha, o1.x, ea.beta * x1.x), fmaf(ea.alpha, o1.y, ea.beta * x1.y));
-        } else if (ea.alpha != 1.f) {
-          o0 = cscale(o0, ea.alpha);
-          o1 = cscale(o1, ea.alpha);
-        }
-        ea.out[i0] = o0;
-        ea.out[i1] = o1;
-      } else {
-        // T_{k+1} = 2·(alpha·Θ T_k − T_k) − T_{k−1}   (interior.py:341-348, exp_kpm_dos.py:148)
-        const float2 x0 = ea.x[j0], x1 = ea.x[j1];
-        const float2 p0 = ea.out[i0], p1 = ea.out[i1];
-        const float2 n0 = make_float2(2.f * fmaf(ea.alpha, o0.x, -x0.x) - p0.x, 2.f * fmaf(ea.alpha, o0.y, -x0.y) - p0.y);
-        const float2 n1 = make_float2(2.f * fmaf(ea.alpha, o1.x, -x1.x) - p1.x, 2.f * fmaf(ea.alpha, o1.y, -x1.y) - p1.y);
-        ea.out[i0] = n0;
-        ea.out[i1] = n1;
-        if (EPI == kCheb) {
-          const float2 y0 = ea.y[i0], y1 = ea.y[i1];
-          ea.y[i0] = make_float2(fmaf(ea.ck, n0.x, y0.x), fmaf(ea.ck, n0.y, y0.y));
-          ea.y[i1] = make_float2(fmaf(ea.ck, n1.x, y1.x), fmaf(ea.ck, n1.y, y1.y));
-        } else if (EPI == kKpm) {
-          // ⟨T_k, T_k⟩ and Re⟨T_{k+1}, T_k⟩ in FP64 (products of FP32 are exact in FP64)
-          s0 += (double)x0.x * x0.x + (double)x0.y * x0.y + (double)x1.x * x1.x + (double)x1.y * x1.y;
-          s1 += (double)n0.x * x0.x + (double)n0.y * x0.y + (double)n1.x * x1.x + (double)n1.y * x1.y;
-        } else {  // kDot: Re⟨z, T_{k+1}⟩
-          const float2 z0 = ea.z[i0], z1 = ea.z[i1];
-          s0 += (double)z0.x * n0.x + (double)z0.y * n0.y + (double)z1.x * n1.x + (double)z1.y * n1.y;
-        }
+    }
+  } else {
+    // a fixed trip count, unrolled, so the epilogue loads of all of a thread's
+    // points can be issued together (+0.8 % on Θ, +1.4-1.8 % fused steps at 192³)
+    constexpr int NE = (R * G + kThreads - 1) / kThreads;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int e = threadIdx.x + k * kThreads;
+      if ((R * G) % kThreads == 0 || e < R * G) {
+        const int r = e / G, z = e - r * G;
+        const long long xy = xy0 + r;
+        const int x = int(xy / G), y = int(xy - (long long)x * G);
+        const long long vox = xy * G + z;
+        const int p = zpos(z);
+        const Frame f = point_frame(ft, row_frame(ft, x, y), z, vox);
+        zout_point<KIND, EPI>(f, buf[(0 * R + r) * P::kStride + p], buf[(1 * R + r) * P::kStride + p],
+                              buf[(2 * R + r) * P::kStride + p], ft.kn_floor, ea, vbase + vox,
+                              vbase + G3 + vox, xbase + vox, xbase + G3 + vox, s0, s1);
       }
     }
   }
@@ -566,7 +795,7 @@ __global__ __launch_bounds__(kThreads) void real_epilogue_kernel(const float2* _
     return;
   }
   const float rs = lam ? 1.0f / sqrtf(lam[vr.in_vec]) : 1.0f;
-  const float sc = vr.desc->inv_eps[vox] * rs;
+  const float sc = gld(vr.desc->inv_eps + vox) * rs;
   const float2 e0 = cscale(d0, sc), e1 = cscale(d1, sc), e2 = cscale(d2, sc);
   if (MODE == kRealE) {
     float2* o = static_cast<float2*>(out) + vr.out_vec * 3 * G3 + vox;
@@ -574,7 +803,7 @@ __global__ __launch_bounds__(kThreads) void real_epilogue_kernel(const float2* _
     o[G3] = e1;
     o[2 * G3] = e2;
   } else {
-    const float eps = vr.desc->eps[vox] * n_f;  // ε (the table holds ε/G³)
+    const float eps = gld(vr.desc->eps + vox) * n_f;  // ε (the table holds ε/G³)
     const float s2 = (e0.x * e0.x + e0.y * e0.y) + (e1.x * e1.x + e1.y * e1.y) +
                      (e2.x * e2.x + e2.y * e2.y);
     static_cast<float*>(out)[vr.out_vec * G3 + vox] = eps * s2;
