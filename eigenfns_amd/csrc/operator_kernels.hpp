@@ -32,6 +32,9 @@
 #ifndef EFN_TWID_LATE
 #define EFN_TWID_LATE 1
 #endif
+#ifndef EFN_XMID_EPS_EARLY
+#define EFN_XMID_EPS_EARLY 1
+#endif
 
 namespace efn {
 
@@ -168,10 +171,11 @@ __host__ __device__ constexpr int xtile_for(int G) { return G >= EFN_XTILE ? EFN
 
 // Rows per zin/zout block when all three components are in LDS at once: a
 // power of two dividing G² (a block never spans two vectors).  Measured on
-// MI355X (Θ, 32-vector blocks): 4 rows at G ≥ 128 and 8 at 64 ≤ G < 128 beat
+// MI355X (Θ, 32-vector blocks): 4 rows at G ≥ 160 and 8 at 64 ≤ G < 160 beat
 // 8/16 by 7 % at 192³ and 14 % at 96³ — more, smaller blocks per CU hide the
-// HBM latency of the direct stage-1/stage-3 accesses better than fuller ones;
-// below 64, 16 rows beat 8 and 32 by 5-6 % at 48³ and 32³ (64-96 vectors).
+// HBM latency of the direct stage-1/stage-3 accesses better than fuller ones
+// (8 rows over 4 at 128³: +0.8 %, at 192³: −1.5 %); below 64, 16 rows beat 8
+// and 32 by 5-6 % at 48³ and 32³ (64-96 vectors).
 #ifndef EFN_ZROWS_BIG
 #define EFN_ZROWS_BIG 4
 #endif
@@ -182,7 +186,7 @@ __host__ __device__ constexpr int xtile_for(int G) { return G >= EFN_XTILE ? EFN
 #define EFN_ZROWS_SMALL 16
 #endif
 __host__ __device__ constexpr int zrows3_for(int G) {
-  int target = G >= 128 ? EFN_ZROWS_BIG : (G >= 64 ? EFN_ZROWS_MID : EFN_ZROWS_SMALL);
+  int target = G >= 160 ? EFN_ZROWS_BIG : (G >= 64 ? EFN_ZROWS_MID : EFN_ZROWS_SMALL);
   int p = 1;
   while (p * 2 <= target && (G * G) % (p * 2) == 0) p *= 2;
   return p;
@@ -191,6 +195,20 @@ template <int G>
 __host__ __device__ constexpr int zrows3() { return zrows3_for(G); }
 
 constexpr int kThreads = 256;
+
+// The pass kernels' twiddle table: staged per block in LDS, or (EFN_TWID_GLOBAL)
+// read where it is used from the global table, which the caches keep, leaving
+// the block's LDS to the pencils alone.
+#ifndef EFN_TWID_GLOBAL
+#define EFN_TWID_GLOBAL 0
+#endif
+#if EFN_TWID_GLOBAL
+#define EFN_TWID_DECL(G) const float2* __restrict__ twid = twid_g
+#define EFN_TWID_STAGE(G) (void)0
+#else
+#define EFN_TWID_DECL(G) __shared__ float2 twid[G]
+#define EFN_TWID_STAGE(G) for (int i = threadIdx.x; i < (G); i += kThreads) twid[i] = twid_g[i]
+#endif
 
 #ifndef EFN_ZIN_NATURAL
 #define EFN_ZIN_NATURAL 0
@@ -241,9 +259,9 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
   constexpr int R = zrows3<G>();
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
   __shared__ float2 buf[3 * R * P::kStride];
-  __shared__ float2 twid[G];
+  EFN_TWID_DECL(G);
 #if !EFN_TWID_LATE
-  for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+  EFN_TWID_STAGE(G);
 #endif
   const long long row0 = (long long)blockIdx.x * R;  // flattened (v, x, y)
   const int v = int(row0 / G2);
@@ -330,7 +348,7 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
   }
 #if EFN_TWID_LATE
   // staged after the data loads are issued, so its round trip overlaps theirs
-  for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+  EFN_TWID_STAGE(G);
 #endif
   __syncthreads();
   for (int t = threadIdx.x; t < 3 * R * N2; t += kThreads) {
@@ -378,9 +396,9 @@ __global__ __launch_bounds__(kThreads) void strided_kernel(float2* __restrict__ 
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
   constexpr long long stride = AXIS == 1 ? G : G2;
   __shared__ float2 buf[ZT * P::kStride];
-  __shared__ float2 twid[G];
+  EFN_TWID_DECL(G);
 #if !EFN_TWID_LATE
-  for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+  EFN_TWID_STAGE(G);
   __syncthreads();
 #endif
   constexpr int nzt = (G + ZT - 1) / ZT;
@@ -401,7 +419,7 @@ __global__ __launch_bounds__(kThreads) void strided_kernel(float2* __restrict__ 
   {  // the first step-1 inputs are in flight before the twiddle table is staged
     float2 v[N1];
     if (active(t)) load1(t, v);
-    for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+    EFN_TWID_STAGE(G);
     __syncthreads();
     if (active(t)) step1_store<G, S>(v, t / ZT, buf + (t % ZT) * P::kStride, twid);
     t += kThreads;
@@ -445,9 +463,9 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   constexpr int kFStride = (kFStrideRaw % 2 == 0) ? kFStrideRaw + 1 : kFStrideRaw;
   constexpr int kBufStride = kFStride > P::kStride ? kFStride : P::kStride;
   __shared__ float2 buf[ZT * kBufStride];
-  __shared__ float2 twid[G];
+  EFN_TWID_DECL(G);
 #if !EFN_TWID_LATE
-  for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+  EFN_TWID_STAGE(G);
   __syncthreads();
 #endif
   const long long b = blockIdx.x;
@@ -477,16 +495,36 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) vv[n1] = base[(N2 * n1 + n2) * G2 + z];
   };
+  // this thread's ε⁻¹ (or ε) factors for its step-3 tasks, fetched once the
+  // step-1 inputs have arrived and consumed after step 3, so their round trip
+  // overlaps the first half of the transform (measured +0.6 % on Θ at 192³,
+  // −0.3 to −0.8 % at 48³-128³, so only on the large grids)
+  constexpr bool kEpsEarly = EFN_XMID_EPS_EARLY && G >= 160;
+  float epre[TPT][N2];
+  auto load_eps = [&]() {
+#pragma unroll
+    for (int j = 0; j < TPT; ++j) {
+      const int t = threadIdx.x + j * kThreads;
+      const int z = t % ZT, k1 = t / ZT;
+      if (t < T3 && z0 + z < G) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) epre[j][k2] = gld(eps + (long long)(k1 + N1 * k2) * G2 + z);
+      }
+    }
+  };
   int t1 = threadIdx.x;
 #if EFN_TWID_LATE
   {  // the first step-1 inputs are in flight before the twiddle table is staged
     float2 vv[N1];
     if (active(t1)) load1(t1, vv);
-    for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+    EFN_TWID_STAGE(G);
     __syncthreads();
+    if constexpr (kEpsEarly) load_eps();
     if (active(t1)) step1_store<G, +1>(vv, t1 / ZT, buf + (t1 % ZT) * P::kStride, twid);
     t1 += kThreads;
   }
+#else
+  if constexpr (kEpsEarly) load_eps();
 #endif
   for (; t1 < ZT * N2; t1 += kThreads) {
     if (active(t1)) {
@@ -506,7 +544,7 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
       step3_load<G, +1>(r[j], k1, buf + z * P::kStride);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
-        const float sc = gld(eps + (long long)(k1 + N1 * k2) * G2 + z);
+        const float sc = kEpsEarly ? epre[j][k2] : gld(eps + (long long)(k1 + N1 * k2) * G2 + z);
         r[j][k2] = cscale(r[j][k2], sc);
       }
     }
@@ -642,7 +680,7 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
   constexpr int RW = kWaveRows ? R / (kThreads / 64) : 1;  // rows per wave
   constexpr int ZJ = kWaveRows ? G / 64 : 1;               // points per lane per row
   __shared__ float2 buf[3 * R * P::kStride];
-  __shared__ float2 twid[G];
+  EFN_TWID_DECL(G);
   __shared__ double red[2 * kThreads / 64];
   const long long row0 = (long long)blockIdx.x * R;
   const int v = int(row0 / G2);
@@ -681,7 +719,7 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
     float2 vv[N1];
     const int t = threadIdx.x;
     if (t < T1) step1_src(t, vv);
-    for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+    EFN_TWID_STAGE(G);
     __syncthreads();
     if (t < T1) step1_store<G, -1>(vv, t % N2, buf + (t / N2) * P::kStride, twid);
   }
@@ -691,7 +729,7 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
     step1_store<G, -1>(vv, t % N2, buf + (t / N2) * P::kStride, twid);
   }
 #else
-  for (int i = threadIdx.x; i < G; i += kThreads) twid[i] = twid_g[i];
+  EFN_TWID_STAGE(G);
   __syncthreads();
   for (int t = threadIdx.x; t < T1; t += kThreads) {
     float2 vv[N1];

@@ -46,7 +46,7 @@ CASES = [(6, (0.5, 0, 0)), (8, (0, 0, 0)), (12, (0.1, 0.2, 0.3)), (16, (0, 0, 0)
          (96, (0.5, 0.5, 0.5)), (128, (0, 0, 0)), (192, (0, 0, 0))]
 
 
-@pytest.mark.parametrize("G,k", CASES)
+@pytest.mark.parametrize("G,k", CASES + [(256, (0.1, 0, 0.2))])
 def test_theta_matches_fp64_oracle(G, k):
     import torch
     op, ref = _ops(G, k)
