@@ -190,3 +190,63 @@ def test_device_theta_is_hermitian_and_psd():
     assert np.abs(a - b).max() < 1e-5 * np.abs(a).max()
     q = np.diag(gram(X, op.theta(X)).cpu().numpy())
     assert np.all(q.real > 0) and np.abs(q.imag).max() < 1e-5 * q.real.max()
+
+
+def test_chebsi_bottom_up_matches_dense(g8):
+    """Bottom-up ChebSI with locking (chebyshev.py:103-210): the lowest 12
+    eigenvalues within 1e-6 relative of the dense FP64 eigendecomposition
+    (a restatement in FP64 with dense Θ converges in 4 outer steps here)."""
+    from eigenfns_amd.chebyshev import chebsi_bottom_up
+    op, ref, w, V = g8
+    vals, X, st = chebsi_bottom_up(op, 12, guard=12, degree=30, tol=1e-5, max_outer=40, verbose=False)
+    assert len(vals) == 12 and X.shape[0] == 12
+    assert st.outer[-1]["locked"] >= 12
+    assert (np.abs(vals - w[:12]) / w[:12]).max() < 1e-6
+
+
+def test_bandpass_subspace_and_two_stage_find_window_states(g8):
+    """The window [(λ4+λ5)/2, (λ11+λ12)/2] holds exactly the 7 states λ5..λ11:
+    the bandpass subspace iteration (interior.py:357-427) and the two-stage
+    protocol (run_interior.py:154-216) return those, within 1e-6 of dense FP64."""
+    from eigenfns_amd.interior import bandpass_subspace, two_stage
+    op, ref, w, V = g8
+    lo, hi = (w[4] + w[5]) / 2, (w[11] + w[12]) / 2
+    lam_max = 1.05 * w.max()
+    Xr, info = bandpass_subspace(op, lo, hi, lam_max, m=24, degree=300, max_outer=8, res_tol=1e-5,
+                                 target_count=7, verbose=False)
+    lam, rn = info["lam"], info["res"]
+    conv = (lam >= lo) & (lam <= hi) & (rn < 1e-5)
+    assert conv.sum() == 7
+    assert np.abs(np.sort(lam[conv]) - w[5:12]).max() < 1e-6 * w[11]
+    vals, vecs, rn2, info2 = two_stage(op, lo, hi, lam_max, m=24, build_degree=150, build_outers=2,
+                                       polish_degree=300, polish_outers=4, tol=1e-5, chunk=8,
+                                       verbose=False)
+    assert len(vals) == 7 and vecs.shape[0] == 7 and (rn2 < 1e-5).all()
+    assert np.abs(vals - w[5:12]).max() < 1e-6 * w[11]
+
+
+@pytest.mark.parametrize("target", [0.0, 5.0])
+def test_precondition_simple_matches_oracle(target):
+    """Diagonal kinetic preconditioner R/(|k+G|² + s) (operator.py:98-101,
+    163-166) at a Bloch k-point, against the FP64 oracle: rel L2 < 1e-6."""
+    import torch
+    from eigenfns_amd.operator import MaxwellOperator
+    rng = np.random.default_rng(7)
+    eps = np.where(rng.random((16, 16, 16)) < 0.3, 8.41, 1.0).astype(np.float32)
+    k = (0.2, -0.1, 0.35)
+    op, ref = MaxwellOperator(eps, 2.1, k), MaxwellOperatorNP(eps, 2.1, k)
+    X = (rng.standard_normal((3, 2, 16, 16, 16)) + 1j * rng.standard_normal((3, 2, 16, 16, 16))).astype(np.complex64)
+    got = op.precondition_simple(torch.from_numpy(X).cuda(), target).cpu().numpy()
+    want = ref.precondition_simple(X, target)
+    err = np.linalg.norm((got - want).reshape(3, -1), axis=1) / np.linalg.norm(want.reshape(3, -1), axis=1)
+    assert err.max() < 1e-6
+
+
+def test_apply_chunked_equals_one_launch(g8):
+    """apply_chunked (solver.py:74-82) over sub-chunks of 3 rows gives the
+    one-launch Θ bit for bit (each vector's passes are independent)."""
+    import torch
+    from eigenfns_amd.linalg import apply_chunked
+    op, ref, w, V = g8
+    X = op.random_block(8, seed=4)
+    assert torch.equal(apply_chunked(op.theta, X, 3), op.theta(X))

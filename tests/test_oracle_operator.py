@@ -65,6 +65,27 @@ def test_homogeneous_preconditioner_known_answer():
     assert np.abs(Y - expect).max() < 1e-10 * np.abs(expect).max()
 
 
+def test_diagonal_preconditioner_known_answer():
+    # A one-hot coefficient at reciprocal-lattice point n (Bloch k) comes back
+    # scaled by 1/((2π/L)²|n + k|² + s), n counted as in np.fft.fftfreq, with the
+    # shift s = target + 1e-6 for a target above 1e-6·⟨1/ε⟩ and
+    # (⟨1/ε⟩ + 1)·1e-6 for target 0 (operator.py:98-101, 163-166).
+    G, L, k = 6, 1.7, (0.25, -0.5, 0.1)
+    rng = np.random.default_rng(2)
+    eps = np.where(rng.random((G, G, G)) < 0.4, 6.0, 1.0).astype(np.float32)
+    op = MaxwellOperatorNP(eps, L, k)
+    for (ix, iy, iz), comp in (((0, 0, 0), 0), ((1, 5, 3), 1), ((4, 2, 0), 0)):
+        n = np.array([ix if ix < G // 2 else ix - G, iy if iy < G // 2 else iy - G,
+                      iz if iz < G // 2 else iz - G], float)
+        q2 = ((2 * np.pi / L) ** 2) * ((n + np.asarray(k)) ** 2).sum()
+        X = np.zeros((1, 2, G, G, G), complex)
+        X[0, comp, ix, iy, iz] = 1.0 - 2.0j
+        for target, s in ((0.0, (op.eps_mean_inv + 1.0) * 1e-6), (3.0, 3.0 + 1e-6)):
+            Y = op.precondition_simple(X, target)
+            assert np.count_nonzero(Y) == 1
+            assert abs(Y[0, comp, ix, iy, iz] - (1.0 - 2.0j) / (q2 + s)) < 1e-6 / (q2 + s)
+
+
 def test_dense_homogeneous_spectrum_matches_analytic():
     # reference tests/test_operator.py:27-38
     G, L, e0 = 8, 1.0, 4.0
