@@ -202,10 +202,12 @@ constexpr int plan_n1(int N) {
   return best;
 }
 
-template <int N>
+// SWAP exchanges the two factors (N1 ≥ N2): step 3 then writes runs of N1
+// consecutive outputs.
+template <int N, bool SWAP = false>
 struct Plan {
-  static constexpr int N1 = plan_n1(N);  // step-1 DFT size (stride-N2 inputs)
-  static constexpr int N2 = N / N1;      // step-3 DFT size
+  static constexpr int N1 = SWAP ? N / plan_n1(N) : plan_n1(N);  // step-1 DFT size (stride-N2 inputs)
+  static constexpr int N2 = N / N1;                              // step-3 DFT size
   static_assert(N1 > 0, "pencil length not supported by the four-step plan");
   // padded LDS position of element i: one pad slot per run of N2, pencil stride odd
   static constexpr int pos(int i) { return i + i / N2; }
@@ -219,10 +221,9 @@ struct Plan {
 //   step3: the task k1 loads A[·][k1] (the contiguous run pos(N2·k1 + n2)) and
 //          leaves X[k1 + N1·k2] in r[k2].
 // twid[e] = exp(-2πi e/N) for e in [0, N) (conjugated for S = +1).
-template <int N, int S>
-__device__ __forceinline__ void step1_store(float2 (&v)[Plan<N>::N1], int n2, float2* __restrict__ pencil,
+template <int N, int S, class P = Plan<N>>
+__device__ __forceinline__ void step1_store(float2 (&v)[P::N1], int n2, float2* __restrict__ pencil,
                                             const float2* __restrict__ twid) {
-  using P = Plan<N>;
   constexpr int N1 = P::N1, N2 = P::N2;
   DFT<N1, S>::run(v);
 #pragma unroll
@@ -236,9 +237,8 @@ __device__ __forceinline__ void step1_store(float2 (&v)[Plan<N>::N1], int n2, fl
   for (int k1 = 0; k1 < N1; ++k1) col[(N2 + 1) * k1] = v[k1];
 }
 
-template <int N, int S>
-__device__ __forceinline__ void step3_load(float2 (&r)[Plan<N>::N2], int k1, const float2* __restrict__ pencil) {
-  using P = Plan<N>;
+template <int N, int S, class P = Plan<N>>
+__device__ __forceinline__ void step3_load(float2 (&r)[P::N2], int k1, const float2* __restrict__ pencil) {
   constexpr int N2 = P::N2;
   const float2* run = pencil + (N2 + 1) * k1;  // pos(N2·k1 + n2) = (N2 + 1)·k1 + n2
 #pragma unroll
