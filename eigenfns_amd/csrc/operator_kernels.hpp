@@ -214,10 +214,7 @@ constexpr int kThreads = 256;
 #endif
 
 // ---------------------------------------------------------------------------
-// zin: (nvec, 2, G³) → W (nvec, 3, G³), curl into the Cartesian frame + inverse
-// z-FFT.  A coalesced prologue reads each input point once and writes the three
-// Cartesian components in natural order to LDS; the four-step FFT runs on the
-// 3·R pencils; step-3 outputs go straight to HBM.
+// The curl into the Cartesian frame at one voxel (zin's pointwise step).
 template <int G, int KIND>
 __device__ __forceinline__ void curl_in(const Frame& f, float2 a, float2 b, float kn_floor, float2 (&o)[3]) {
   float2 c1, c2;
@@ -246,6 +243,10 @@ __host__ __device__ constexpr bool z_wave_rows() {
   return EFN_ZWAVE_ROWS && G % 64 == 0 && zrows3<G>() % (kThreads / 64) == 0;
 }
 
+// zin: (nvec, 2, G³) → W (nvec, 3, G³), curl into the Cartesian frame + inverse
+// z-FFT.  A coalesced prologue reads each input point once and writes the three
+// Cartesian components in natural order to LDS; the four-step FFT runs on the
+// 3·R pencils; step-3 outputs go straight to HBM.
 template <int G, int KIND>
 __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict__ in,
                                                        float2* __restrict__ ws, VecMap vm,
