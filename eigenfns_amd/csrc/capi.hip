@@ -134,7 +134,7 @@ template <int G>
 int run_op(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
   using namespace efn;
   constexpr int R = zrows3<G>();
-  constexpr int nzt = (G + ztile_for(G) - 1) / ztile_for(G);
+  constexpr int nyt = (G + strided_tile<G, 1>() - 1) / strided_tile<G, 1>();
   constexpr int nxt = (G + xtile_for(G) - 1) / xtile_for(G);
   constexpr int nb = G * G / R;  // zin/zout blocks per vector
   const int use_eps = rq.kind == kTheta ? 0 : 1;
@@ -143,16 +143,16 @@ int run_op(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
     float2* const ws = c->d_ws;
     double* const partial = c->d_partial;
     const long long tiles = (long long)nv * G * nxt;  // x-pass tiles per component
-    const dim3 gz((unsigned)(nv * nb)), gs((unsigned)(nv * 3 * G * nzt)), gx((unsigned)(3 * tiles)),
-        blk(kThreads);
+    const dim3 gz((unsigned)(nv * nb)), gs((unsigned)(nv * 3 * G * nyt)), gx((unsigned)(3 * tiles)),
+        blk(kThreads), blky(strided_threads<G, 1>());
     if (rq.kind == kTheta)
       hipLaunchKernelGGL((zin_kernel<G, kTheta>), gz, blk, 0, s, rq.in, ws, rq.vm, (long long)v0, c->d_twid);
     else
       hipLaunchKernelGGL((zin_kernel<G, kPrecond>), gz, blk, 0, s, rq.in, ws, rq.vm, (long long)v0, c->d_twid);
     EFN_LAUNCH_CHECK();
-    hipLaunchKernelGGL((strided_kernel<G, 1, +1>), gs, blk, 0, s, ws, c->d_twid);
+    hipLaunchKernelGGL((strided_kernel<G, 1, +1>), gs, blky, 0, s, ws, c->d_twid);
     hipLaunchKernelGGL((xmid_kernel<G>), gx, blk, 0, s, ws, rq.vm, (long long)v0, use_eps, c->d_twid, tiles);
-    hipLaunchKernelGGL((strided_kernel<G, 1, -1>), gs, blk, 0, s, ws, c->d_twid);
+    hipLaunchKernelGGL((strided_kernel<G, 1, -1>), gs, blky, 0, s, ws, c->d_twid);
     EFN_LAUNCH_CHECK();
     EpiArgs ea = rq.ea;
     ea.x = rq.in;
@@ -200,17 +200,19 @@ int run_real(efn_ctx* c, const OpRequest& rq, int mode, const float* lam, void* 
   using namespace efn;
   constexpr int R = zrows3<G>();
   constexpr int nzt = (G + ztile_for(G) - 1) / ztile_for(G);
+  constexpr int nyt = (G + strided_tile<G, 1>() - 1) / strided_tile<G, 1>();
   constexpr int nb = G * G / R;
   constexpr long long G3 = (long long)G * G * G;
   for (int v0 = 0; v0 < rq.m; v0 += c->chunk) {
     const int nv = rq.m - v0 < c->chunk ? rq.m - v0 : c->chunk;
-    const dim3 gz((unsigned)(nv * nb)), gs((unsigned)(nv * 3 * G * nzt)), blk(kThreads);
+    const dim3 gz((unsigned)(nv * nb)), gs((unsigned)(nv * 3 * G * nzt)), gsy((unsigned)(nv * 3 * G * nyt)),
+        blk(kThreads), blky(strided_threads<G, 1>());
     if (mode == kRealH)
       hipLaunchKernelGGL((zin_kernel<G, kCartH>), gz, blk, 0, s, rq.in, c->d_ws, rq.vm, (long long)v0, c->d_twid);
     else
       hipLaunchKernelGGL((zin_kernel<G, kTheta>), gz, blk, 0, s, rq.in, c->d_ws, rq.vm, (long long)v0, c->d_twid);
     EFN_LAUNCH_CHECK();
-    hipLaunchKernelGGL((strided_kernel<G, 1, +1>), gs, blk, 0, s, c->d_ws, c->d_twid);
+    hipLaunchKernelGGL((strided_kernel<G, 1, +1>), gsy, blky, 0, s, c->d_ws, c->d_twid);
     hipLaunchKernelGGL((strided_kernel<G, 0, +1>), gs, blk, 0, s, c->d_ws, c->d_twid);
     EFN_LAUNCH_CHECK();
     const long long count = (long long)nv * G3;
