@@ -20,17 +20,11 @@
 
 #include "fft.hpp"
 
-#ifndef EFN_GLD
-#define EFN_GLD 1
-#endif
 #ifndef EFN_FRAME_RSQ
 #define EFN_FRAME_RSQ 1
 #endif
 #ifndef EFN_ZWAVE_ROWS
 #define EFN_ZWAVE_ROWS 1
-#endif
-#ifndef EFN_TWID_LATE
-#define EFN_TWID_LATE 1
 #endif
 #ifndef EFN_ZIN_SWAP
 #define EFN_ZIN_SWAP 1
@@ -112,11 +106,7 @@ struct RowFrame {
 // always in global memory, so load them as such.
 template <class T>
 __device__ __forceinline__ T gld(const T* p) {
-#if EFN_GLD
   return *(const __attribute__((address_space(1))) T*)p;
-#else
-  return *p;
-#endif
 }
 
 // 1/√x of the frame: the hardware reciprocal square root (≤ 1 ulp) in place
@@ -215,19 +205,12 @@ __host__ __device__ constexpr int zrows3() { return zrows3_for(G); }
 
 constexpr int kThreads = 256;
 
-// The pass kernels' twiddle table: staged per block in LDS, or (EFN_TWID_GLOBAL)
-// read where it is used from the global table, which the caches keep, leaving
-// the block's LDS to the pencils alone.
-#ifndef EFN_TWID_GLOBAL
-#define EFN_TWID_GLOBAL 0
-#endif
-#if EFN_TWID_GLOBAL
-#define EFN_TWID_DECL(G) const float2* __restrict__ twid = twid_g
-#define EFN_TWID_STAGE(G) (void)0
-#else
+// The pass kernels' twiddle table, staged per block in LDS once the block's
+// first data loads are in flight (read from the global table instead, which
+// leaves the LDS to the pencils and allows more blocks per CU, it measured
+// 7 % slower).
 #define EFN_TWID_DECL(G) __shared__ float2 twid[G]
 #define EFN_TWID_STAGE(G) for (int i = threadIdx.x; i < (G); i += kThreads) twid[i] = twid_g[i]
-#endif
 
 // ---------------------------------------------------------------------------
 // The curl into the Cartesian frame at one voxel (zin's pointwise step).
@@ -276,9 +259,6 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
   __shared__ float2 buf[3 * R * P::kStride];
   EFN_TWID_DECL(G);
-#if !EFN_TWID_LATE
-  EFN_TWID_STAGE(G);
-#endif
   const long long row0 = (long long)blockIdx.x * R;  // flattened (v, x, y)
   const int v = int(row0 / G2);
   const long long xy0 = row0 - (long long)v * G2;
@@ -362,10 +342,8 @@ __global__ __launch_bounds__(kThreads) void zin_kernel(const float2* __restrict_
       for (int c = 0; c < 3; ++c) buf[(c * R + r) * P::kStride + p] = o[c];
     }
   }
-#if EFN_TWID_LATE
   // staged after the data loads are issued, so its round trip overlaps theirs
   EFN_TWID_STAGE(G);
-#endif
   __syncthreads();
   for (int t = threadIdx.x; t < 3 * R * N2; t += kThreads) {
     const int p = t / N2, n2 = t - p * N2;
@@ -405,10 +383,6 @@ __global__ __launch_bounds__((strided_threads<G, AXIS>())) void strided_kernel(f
   constexpr long long stride = AXIS == 1 ? G : G2;
   __shared__ float2 buf[ZT * P::kStride];
   EFN_TWID_DECL(G);
-#if !EFN_TWID_LATE
-  EFN_TWID_STAGE(G);
-  __syncthreads();
-#endif
   constexpr int nzt = (G + ZT - 1) / ZT;
   const long long fo = blockIdx.x / nzt;  // (field, other-axis index)
   const int zt = blockIdx.x - int(fo * nzt);
@@ -423,7 +397,6 @@ __global__ __launch_bounds__((strided_threads<G, AXIS>())) void strided_kernel(f
     for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[(N2 * n1 + n2) * stride + z];
   };
   int t = threadIdx.x;
-#if EFN_TWID_LATE
   {  // the first step-1 inputs are in flight before the twiddle table is staged
     float2 v[N1];
     if (active(t)) load1(t, v);
@@ -432,7 +405,6 @@ __global__ __launch_bounds__((strided_threads<G, AXIS>())) void strided_kernel(f
     if (active(t)) step1_store<G, S>(v, t / ZT, buf + (t % ZT) * P::kStride, twid);
     t += NT;
   }
-#endif
   for (; t < ZT * N2; t += NT) {
     if (active(t)) {
       float2 v[N1];
@@ -472,10 +444,6 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   constexpr int kBufStride = kFStride > P::kStride ? kFStride : P::kStride;
   __shared__ float2 buf[ZT * kBufStride];
   EFN_TWID_DECL(G);
-#if !EFN_TWID_LATE
-  EFN_TWID_STAGE(G);
-  __syncthreads();
-#endif
   const long long b = blockIdx.x;
   long long tile;
   int c;
@@ -530,7 +498,6 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
     }
   };
   int t1 = threadIdx.x;
-#if EFN_TWID_LATE
   {  // the first step-1 inputs are in flight before the twiddle table is staged
     float2 vv[N1];
     if (active(t1)) load1(t1, vv);
@@ -540,9 +507,6 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
     if (active(t1)) step1_store<G, +1>(vv, t1 / ZT, buf + (t1 % ZT) * P::kStride, twid);
     t1 += kThreads;
   }
-#else
-  if constexpr (kEpsEarly) load_eps();
-#endif
   for (; t1 < ZT * N2; t1 += kThreads) {
     if (active(t1)) {
       float2 vv[N1];
@@ -730,7 +694,6 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) vv[n1] = src[N2 * n1 + n2];
   };
-#if EFN_TWID_LATE
   // the first step-1 inputs are in flight before the twiddle table is staged
   {
     float2 vv[N1];
@@ -745,15 +708,6 @@ __global__ __launch_bounds__(kThreads) void zout_kernel(const float2* __restrict
     step1_src(t, vv);
     step1_store<G, -1>(vv, t % N2, buf + (t / N2) * P::kStride, twid);
   }
-#else
-  EFN_TWID_STAGE(G);
-  __syncthreads();
-  for (int t = threadIdx.x; t < T1; t += kThreads) {
-    float2 vv[N1];
-    step1_src(t, vv);
-    step1_store<G, -1>(vv, t % N2, buf + (t / N2) * P::kStride, twid);
-  }
-#endif
   __syncthreads();
   // step 3 in place: the task (pencil, k1) reads its own run and writes its
   // results X[k1 + N1·k2] back into that run at (N2 + 1)·k1 + k2 (no other task
