@@ -72,6 +72,7 @@ struct efn_ctx {
   float* d_inv_eps = nullptr;    // ε⁻¹/G³
   float* d_eps = nullptr;        // ε/G³
   float* d_kn2 = nullptr;        // |k+G|² (for the diagonal preconditioner)
+  float* d_xeps = nullptr;       // ε⁻¹/G³ then ε/G³ in the x pass's tile order
   float2* d_twid = nullptr;      // exp(−2πi e/G)
   float2* d_ws = nullptr;        // chunk·3·G³ complex64
   double* d_partial = nullptr;   // chunk·(G²/R)·2
@@ -256,6 +257,7 @@ void destroy(efn_ctx* c) {
   free_dev(c->d_inv_eps);
   free_dev(c->d_eps);
   free_dev(c->d_kn2);
+  free_dev(c->d_xeps);
   free_dev(c->d_twid);
   free_dev(c->d_ws);
   free_dev(c->d_partial);
@@ -360,6 +362,18 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
     epsv[v] = (float)(e * inv_g3);
   }
   c->eps_mean_inv = sum_inv / (double)G3;
+  // the same tables in the x pass's tile order [y][z / ZT][x][z mod ZT]
+  const int xzt = efn::xtile_for(G), xnzt = (G + xzt - 1) / xzt;
+  const size_t nx = (size_t)G * xnzt * G * xzt;
+  std::vector<float> xeps(2 * nx, 0.f);
+  for (int x = 0; x < G; ++x)
+    for (int y = 0; y < G; ++y)
+      for (int z = 0; z < G; ++z) {
+        const size_t v = ((size_t)x * G + y) * G + z;
+        const size_t t = (((size_t)y * xnzt + z / xzt) * G + x) * xzt + z % xzt;
+        xeps[t] = inv_eps[v];
+        xeps[nx + t] = epsv[v];
+      }
   std::vector<float2> twid(G);
   for (int e = 0; e < G; ++e)
     twid[e] = make_float2((float)std::cos(2 * M_PI * e / G), (float)-std::sin(2 * M_PI * e / G));
@@ -391,6 +405,7 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   EFN_ALLOC(c->d_inv_eps, G3 * sizeof(float));
   EFN_ALLOC(c->d_eps, G3 * sizeof(float));
   EFN_ALLOC(c->d_kn2, G3 * sizeof(float));
+  EFN_ALLOC(c->d_xeps, xeps.size() * sizeof(float));
   EFN_ALLOC(c->d_twid, G * sizeof(float2));
   if (chunk > 0) {
     EFN_ALLOC(c->d_ws, c->ws_bytes);
@@ -405,12 +420,15 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   if (e == hipSuccess) e = hipMemcpy(c->d_inv_eps, inv_eps.data(), G3 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_eps, epsv.data(), G3 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_kn2, kn2f.data(), G3 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->d_xeps, xeps.data(), xeps.size() * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_twid, twid.data(), G * sizeof(float2), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     efn::OpDesc d;
     d.ft = c->tables();
     d.inv_eps = c->d_inv_eps;
     d.eps = c->d_eps;
+    d.inv_eps_x = c->d_xeps;
+    d.eps_x = c->d_xeps + nx;
     e = hipMemcpy(c->d_desc, &d, sizeof(d), hipMemcpyHostToDevice);
   }
   if (e != hipSuccess) {

@@ -29,6 +29,9 @@
 #ifndef EFN_ZIN_SWAP
 #define EFN_ZIN_SWAP 1
 #endif
+#ifndef EFN_XEPS_TILED
+#define EFN_XEPS_TILED 1
+#endif
 #ifndef EFN_XMID_PAIR
 #define EFN_XMID_PAIR 1
 #endif
@@ -59,6 +62,8 @@ struct OpDesc {
   FrameTables ft;
   const float* inv_eps;     // ε⁻¹/G³ per voxel
   const float* eps;         // ε/G³ per voxel
+  const float* inv_eps_x;   // the same two tables in the x pass's tile order:
+  const float* eps_x;       // [y][z / ZT][x][z mod ZT], ZT = xtile_for(G)
 };
 
 // Vector map of one launch: launch-local vector v belongs to slot v / nrow and
@@ -470,7 +475,16 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   const int z0 = (rem - o * nzt) * ZT;
   float2* base = ws + (v * 3 + c) * G3 + (long long)o * G + z0;
   const OpDesc* dsc = map_vector(vm, v0 + v).desc;
+#if EFN_XEPS_TILED
+  // a tile's factors are one contiguous run of G·ZT floats (whole 128-B lines:
+  // in the voxel order a tile row is 64 B, half a line shared with the next
+  // z-tile, which another XCD fetches again)
+  const float* eps = (use_eps ? dsc->eps_x : dsc->inv_eps_x) + ((long long)o * nzt + z0 / ZT) * G * ZT;
+  constexpr long long kEpsX = ZT;  // between consecutive x
+#else
   const float* eps = (use_eps ? dsc->eps : dsc->inv_eps) + (long long)o * G + z0;
+  constexpr long long kEpsX = G2;
+#endif
   constexpr int T3 = ZT * N1;
   constexpr int TPT = (T3 + kThreads - 1) / kThreads;
   // inverse step 1: HBM → registers → LDS
@@ -493,7 +507,7 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
       const int z = t % ZT, k1 = t / ZT;
       if (t < T3 && z0 + z < G) {
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) epre[j][k2] = gld(eps + (long long)(k1 + N1 * k2) * G2 + z);
+        for (int k2 = 0; k2 < N2; ++k2) epre[j][k2] = gld(eps + (long long)(k1 + N1 * k2) * kEpsX + z);
       }
     }
   };
@@ -525,7 +539,7 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
       step3_load<G, +1>(r[j], k1, buf + z * P::kStride);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
-        const float sc = kEpsEarly ? epre[j][k2] : gld(eps + (long long)(k1 + N1 * k2) * G2 + z);
+        const float sc = kEpsEarly ? epre[j][k2] : gld(eps + (long long)(k1 + N1 * k2) * kEpsX + z);
         r[j][k2] = cscale(r[j][k2], sc);
       }
     }
