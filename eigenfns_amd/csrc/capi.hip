@@ -152,7 +152,8 @@ int run_op(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
       hipLaunchKernelGGL((zin_kernel<G, kPrecond>), gz, blk, 0, s, rq.in, ws, rq.vm, (long long)v0, c->d_twid);
     EFN_LAUNCH_CHECK();
     hipLaunchKernelGGL((strided_kernel<G, 1, +1>), gs, blky, 0, s, ws, c->d_twid);
-    hipLaunchKernelGGL((xmid_kernel<G>), gx, blk, 0, s, ws, rq.vm, (long long)v0, use_eps, c->d_twid, tiles);
+    hipLaunchKernelGGL((xmid_kernel<G>), gx, dim3(xmid_threads<G>()), 0, s, ws, rq.vm, (long long)v0, use_eps,
+                       c->d_twid, tiles);
     hipLaunchKernelGGL((strided_kernel<G, 1, -1>), gs, blky, 0, s, ws, c->d_twid);
     EFN_LAUNCH_CHECK();
     EpiArgs ea = rq.ea;

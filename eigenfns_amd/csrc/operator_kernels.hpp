@@ -176,12 +176,17 @@ __host__ __device__ constexpr int strided_tile() {
 template <int G, int AXIS>
 __host__ __device__ constexpr int strided_threads() { return strided_tile<G, AXIS>() > 16 ? 512 : 256; }
 
-// z-tile width of the x pass with the real-space multiply (xmid).
+// z-tile width of the x pass with the real-space multiply (xmid): 32 z (256-B
+// runs per x) on 512-thread blocks from 64³ up, measured +1.2 % on Θ at 192³
+// and 128³, +0.4-0.5 % at 64³-96³ and −2 % at 48³ against 16 z.
 #ifndef EFN_XTILE
-#define EFN_XTILE 16
+#define EFN_XTILE 32
 #endif
 
-__host__ __device__ constexpr int xtile_for(int G) { return G >= EFN_XTILE ? EFN_XTILE : (G >= 16 ? 16 : G); }
+__host__ __device__ constexpr int xtile_for(int G) { return G >= 64 ? EFN_XTILE : (G >= 16 ? 16 : G); }
+// x-pass block size: one step-1 task per thread (512 threads for 32-z tiles)
+template <int G>
+__host__ __device__ constexpr int xmid_threads() { return xtile_for(G) > 16 ? 512 : 256; }
 
 // Rows per zin/zout block when all three components are in LDS at once: a
 // power of two dividing G² (a block never spans two vectors).  Measured on
@@ -434,13 +439,14 @@ __global__ __launch_bounds__((strided_threads<G, AXIS>())) void strided_kernel(f
 // the same XCD, back to back, so the rmul tile is fetched from HBM once and
 // served from that XCD's L2 to the other two.
 template <int G>
-__global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws, VecMap vm,
+__global__ __launch_bounds__((xmid_threads<G>())) void xmid_kernel(float2* __restrict__ ws, VecMap vm,
                                                         long long v0, int use_eps,
                                                         const float2* __restrict__ twid_g,
                                                         long long tiles) {
   using P = Plan<G>;
   constexpr int N1 = P::N1, N2 = P::N2;
   constexpr int ZT = xtile_for(G);
+  constexpr int NT = xmid_threads<G>();
   constexpr long long G2 = (long long)G * G, G3 = G2 * G;
   constexpr int nzt = (G + ZT - 1) / ZT;
   // forward-transform rows (register-resident variant): (N1+1)·N2 slots, odd stride
@@ -486,7 +492,7 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   constexpr long long kEpsX = G2;
 #endif
   constexpr int T3 = ZT * N1;
-  constexpr int TPT = (T3 + kThreads - 1) / kThreads;
+  constexpr int TPT = (T3 + NT - 1) / NT;
   // inverse step 1: HBM → registers → LDS
   auto active = [&](int t) { return t < ZT * N2 && z0 + t % ZT < G; };
   auto load1 = [&](int t, float2 (&vv)[N1]) {
@@ -503,7 +509,7 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   auto load_eps = [&]() {
 #pragma unroll
     for (int j = 0; j < TPT; ++j) {
-      const int t = threadIdx.x + j * kThreads;
+      const int t = threadIdx.x + j * NT;
       const int z = t % ZT, k1 = t / ZT;
       if (t < T3 && z0 + z < G) {
 #pragma unroll
@@ -519,9 +525,9 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
     __syncthreads();
     if constexpr (kEpsEarly) load_eps();
     if (active(t1)) step1_store<G, +1>(vv, t1 / ZT, buf + (t1 % ZT) * P::kStride, twid);
-    t1 += kThreads;
+    t1 += NT;
   }
-  for (; t1 < ZT * N2; t1 += kThreads) {
+  for (; t1 < ZT * N2; t1 += NT) {
     if (active(t1)) {
       float2 vv[N1];
       load1(t1, vv);
@@ -533,7 +539,7 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   float2 r[TPT][N2];
 #pragma unroll
   for (int j = 0; j < TPT; ++j) {
-    const int t = threadIdx.x + j * kThreads;
+    const int t = threadIdx.x + j * NT;
     const int z = t % ZT, k1 = t / ZT;
     if (t < T3 && z0 + z < G) {
       step3_load<G, +1>(r[j], k1, buf + z * P::kStride);
@@ -553,7 +559,7 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
   float2* fbuf = buf;  // reused: every inverse step-3 read has completed (barrier above)
 #pragma unroll
   for (int j = 0; j < TPT; ++j) {
-    const int t = threadIdx.x + j * kThreads;
+    const int t = threadIdx.x + j * NT;
     const int z = t % ZT, k1 = t / ZT;
     if (t < T3 && z0 + z < G) {
       DFT<N2, -1>::run(r[j]);
@@ -564,7 +570,7 @@ __global__ __launch_bounds__(kThreads) void xmid_kernel(float2* __restrict__ ws,
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < ZT * N2; t += kThreads) {
+  for (int t = threadIdx.x; t < ZT * N2; t += NT) {
     const int z = t % ZT, f1 = t / ZT;
     if (z0 + z < G) {
       float2 o2[N1];
