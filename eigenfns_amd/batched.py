@@ -1,6 +1,6 @@
 """Batched, continuously admitted deflated block LOBPCG for many small problems.
 
-Serving path of the north star: many (structure, Bloch k-point, band count)
+The serving path: many (structure, Bloch k-point, band count)
 solves of one grid size advance in lockstep, one LOBPCG iteration per engine
 step, through shared launches — multi-operator Θ / preconditioner
 (efn_theta_multi: each slot's vectors carry their own frame tables and ε),

@@ -209,7 +209,7 @@ class MaxwellOperator:
                                                        X.shape[0], scale, stream_ptr()))
         return out
 
-    # ---- fused recurrence steps (north-star fusions; no reference symbol) --
+    # ---- fused recurrence steps (fused into Θ's last pass; no reference symbol) --
     def cheb_step(self, T1, T0, Y, alpha: float, ck: float) -> None:
         """T0 ← 2(alpha·Θ·T1 − T1) − T0; Y ← Y + ck·T0, in place (interior.py:347-349)."""
         _lib.check(_lib.load().efn_cheb_step(self._ctx, T1.data_ptr(), T0.data_ptr(), Y.data_ptr(),

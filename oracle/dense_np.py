@@ -2,8 +2,8 @@
 
 Follows the reference's dense-matrix oracle, tests/test_operator.py:16-24
 (column i of M is Θ applied to the i-th unit vector), batched over columns.
-The small-grid eigenvalue reference of tests/test_solver.py:20-33 and of the
-north-star "FP64 dense solve of the 16³ grid" is built from it.
+The small-grid eigenvalue reference of tests/test_solver.py:20-33 and the
+FP64 dense solve of a 16³ grid are built from it.
 """
 from __future__ import annotations
 

@@ -1,6 +1,6 @@
 """GPU parity of the fused operator kernels against the FP64 oracle.
 
-Tolerance (north star): Θ-apply within 1e-5 relative in L2 per vector; the
+Tolerance: Θ-apply within 1e-5 relative in L2 per vector; the
 same bar for the preconditioner and the fused Chebyshev/KPM steps.
 """
 import numpy as np

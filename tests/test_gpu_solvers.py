@@ -1,6 +1,6 @@
 """GPU parity of the solvers (LOBPCG, Lanczos, bandpass ChebSI, RR, KPM) against FP64 oracles.
 
-Bar (north star): converged eigenvalues within 1e-6 relative of the FP64
+Bar: converged eigenvalues within 1e-6 relative of the FP64
 dense solve, with matching subspace overlap; KPM moments within 1e-5 of the
 FP64 recurrence on the same probes.
 """

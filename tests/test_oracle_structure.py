@@ -2,7 +2,7 @@
 
 Golden masks: tests/golden/raster_golden.npz, written by
 tests/golden/make_raster_golden.py with the reference's eigenfns/structure.py.
-Bit-exact equality is required (north star: voxel masks bit-exact).
+Bit-exact equality is required (the voxel masks are integer data).
 """
 import hashlib
 import json
