@@ -163,15 +163,18 @@ __device__ __forceinline__ Frame point_frame(const FrameTables& ft, const RowFra
 // z-tile width of the strided (y, x) passes: 16 complex = 128 B runs.
 __host__ __device__ constexpr int ztile_for(int G) { return G >= 16 ? 16 : G; }
 
-// The y pass (AXIS 1) at 128³: tiles of EFN_YTILE z on 512-thread blocks
-// (32 z: 256-B runs, 6 instead of 5 waves per SIMD) measured +1.4 % on Θ at
-// 128³ and −0.2 % at 192³, so 192³ keeps 16.
+// The y pass (AXIS 1) at 128³ and 192³: tiles of EFN_YTILE z on 512-thread
+// blocks (32 z: 256-B runs, 6 instead of 5 waves per SIMD) measured +1.4 % on
+// Θ at 128³ and +0.1 % on Θ, +0.4 % on the fused Chebyshev/KPM steps at 192³.
 #ifndef EFN_YTILE
 #define EFN_YTILE 32
 #endif
+#ifndef EFN_YTILE_MAXG
+#define EFN_YTILE_MAXG 224
+#endif
 template <int G, int AXIS>
 __host__ __device__ constexpr int strided_tile() {
-  return AXIS == 1 && G >= 128 && G < 160 ? EFN_YTILE : ztile_for(G);
+  return AXIS == 1 && G >= 128 && G < EFN_YTILE_MAXG ? EFN_YTILE : ztile_for(G);
 }
 template <int G, int AXIS>
 __host__ __device__ constexpr int strided_threads() { return strided_tile<G, AXIS>() > 16 ? 512 : 256; }
