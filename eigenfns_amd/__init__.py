@@ -7,6 +7,7 @@ Drop-in for the reference package ``eigenfns`` on its hot path:
   eigenfns_amd.interior   ← eigenfns.interior   (bandpass filter, RR extraction, filtered SI)
   eigenfns_amd.kpm        ← scripts/exp/exp_kpm_dos.py + exp_kpm_analyze.py (KPM DOS moments)
   eigenfns_amd.structure  ← eigenfns.structure  (rasterizer) + synthetic rod networks
+  eigenfns_amd.io         ← eigenfns.io         (block checkpoints, the reference's file format)
 All device arithmetic runs in the HIP library libeigenfns_hip.so (C ABI,
 include/eigenfns_hip.h); torch supplies device memory and streams.
 
@@ -18,9 +19,9 @@ from __future__ import annotations
 import importlib
 import sys
 
-__version__ = "0.1.0"
+__version__ = "0.3.0"
 
-_SUBMODULES = ("operator", "solver", "chebyshev", "interior", "kpm", "structure", "linalg")
+_SUBMODULES = ("operator", "solver", "chebyshev", "interior", "kpm", "structure", "linalg", "io")
 
 
 def install_alias(name: str = "eigenfns") -> None:

@@ -36,6 +36,7 @@ from .linalg import (COMBINE_IN_PLACE_MAX, RR_DEAD_PENALTY, SVQB_DROP, combine_m
                      gram_multi, rowreduce_strided, row_axpby_norm_strided, rowstats_strided,
                      MAX_PROBLEMS)
 from .operator import MaxwellOperator, precondition_multi, theta_multi
+from .solver import STAGNATION_WINDOW, StagnationLock
 
 
 @dataclass
@@ -53,6 +54,8 @@ class Request:
     vecs: torch.Tensor | None = None
     iterations: int = 0
     theta_applications: int = 0
+    converged: bool = True       # every locked block met the residual tolerance
+    stagnated_blocks: int = 0    # blocks locked at the FP32 residual floor above tol
     t_arrive: float = 0.0
     t_admit: float = 0.0
     t_done: float = 0.0
@@ -73,6 +76,7 @@ class _Slot:
         self.phase = "init"
         self.it = 0
         self.n_lock = 0
+        self.stag = None
 
 
 class BatchedLOBPCG:
@@ -86,7 +90,8 @@ class BatchedLOBPCG:
     def __init__(self, grid_size: int, m: int = 24, guard: int = 8, nslots: int = 8,
                  max_nev: int = 64, tol: float = 1e-4, maxit: int = 400,
                  keep_vectors: bool = False, normalized_rq: bool = True,
-                 verify_dead_rows: bool = False, small_dense: str = "host"):
+                 verify_dead_rows: bool = False, small_dense: str = "host",
+                 stagnation: int = STAGNATION_WINDOW):
         if not 0 < guard < m:
             raise ValueError("need 0 < guard < m")
         if not 1 <= nslots <= MAX_PROBLEMS:
@@ -97,6 +102,7 @@ class BatchedLOBPCG:
         self.nslots = int(nslots)
         self.tol = float(tol)
         self.maxit = int(maxit)
+        self.stagnation = int(stagnation)  # lobpcg_blocks' FP32-floor lock, per slot
         self.keep_vectors = keep_vectors
         self.normalized_rq = normalized_rq
         self.verify_dead_rows = verify_dead_rows  # also measure the row norms and compare (tests)
@@ -332,6 +338,7 @@ class BatchedLOBPCG:
             sl = self.slots[s]
             sl.phase = "iter"
             sl.it = 0
+            sl.stag = StagnationLock(self.tol, self.stagnation)
             sl.n_lock = min(m - self.guard, sl.nev - sl.locked_vals.size)
 
     def _iterate(self, idx, lam):
@@ -459,7 +466,11 @@ class BatchedLOBPCG:
             l = np.where(dead, np.inf, lam_h[s])
             r = np.where(dead, np.inf, rn_h[s])
             order = np.argsort(l)
-            if (r[order[:sl.n_lock]] < self.tol).all() or sl.it >= self.maxit:
+            worst = float(r[order[:sl.n_lock]].max())
+            if worst < self.tol or sl.it >= self.maxit or sl.stag.stalled(worst):
+                if not worst < self.tol:
+                    sl.req.converged = False
+                    sl.req.stagnated_blocks += sl.it < self.maxit
                 req = self._finish_block(s, l, r, order)
                 if req is not None:
                     finished.append(req)

@@ -830,13 +830,24 @@ int combine_impl(int nprob, const void* C, const void* const* S, int m_in, int m
   memset(&cp, 0, sizeof(cp));
   cp.split = split;
   auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
+  // [p, p + rows·n) of complex64 against S's [S, S + m_in·n)
+  auto overlaps_s = [&](const void* p, long long rows, const void* s) {
+    if (!p || !s || rows <= 0) return false;
+    const uintptr_t a0 = (uintptr_t)p, a1 = a0 + (uintptr_t)(rows * n) * sizeof(float2);
+    const uintptr_t b0 = (uintptr_t)s, b1 = b0 + (uintptr_t)((long long)m_in * n) * sizeof(float2);
+    return a0 < b1 && b0 < a1;
+  };
   int vec = (n & 1) == 0;
   for (int i = 0; i < nprob; ++i) {
     if ((split > 0 && !out[i]) || (split < m_out && !out2[i]) || (m_in > 0 && !S[i]))
       return fail(EFN_EINVAL, "NULL block pointer");
-    // in place needs one row tile: every reader of a column is its writer
-    if (m_in > 0 && (out[i] == S[i] || (out2 && out2[i] == S[i])) && m_out > kCombineInPlaceMax)
-      return fail(EFN_EINVAL, "outputs may lie in S only for m_out <= " + std::to_string(kCombineInPlaceMax));
+    // in place needs one row tile: every reader of a column is its writer, so
+    // an output block that overlaps S anywhere (not only at S itself) is
+    // limited to one row tile of outputs
+    if (m_in > 0 && m_out > kCombineInPlaceMax &&
+        (overlaps_s(split > 0 ? out[i] : nullptr, split, S[i]) ||
+         overlaps_s(out2 && split < m_out ? out2[i] : nullptr, m_out - split, S[i])))
+      return fail(EFN_EINVAL, "outputs may overlap S only for m_out <= " + std::to_string(kCombineInPlaceMax));
     cp.s[i] = m_in > 0 ? (const float2*)S[i] : nullptr;
     cp.x[i] = X ? (const float2*)X[i] : nullptr;
     cp.out[i] = (float2*)out[i];

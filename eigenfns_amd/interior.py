@@ -72,6 +72,66 @@ def _apply_cheb_poly(op, X, coef: np.ndarray, lam_max: float):
     return normalize_rows(Y, out=Y)
 
 
+# ------------------------------------------------- host-resident basis helpers
+def _to_device_rows(Xh, start: int, rows: int):
+    part = Xh[start:start + rows]
+    if isinstance(part, torch.Tensor):
+        return part.to(device="cuda", dtype=torch.complex64).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(part, np.complex64)).cuda()
+
+
+def _stream_rotate(C, Xh, rows: int = 8):
+    """out[i] = Σ_j C[j, i]·Xh[j] for a host block Xh (interior.py:93-109): row
+    chunks of `rows` vectors move to the device, each output chunk accumulates
+    there through the combine kernel (out = Cᵀ·S + out) and comes back once.
+    C: (m_in, m_out); returns a host complex64 array."""
+    C = np.asarray(C)
+    m_in, m_out = C.shape
+    out = np.empty((m_out,) + tuple(Xh.shape[1:]), np.complex64)
+    for so in range(0, m_out, rows):
+        ro = min(rows, m_out - so)
+        acc = torch.zeros((ro,) + tuple(Xh.shape[1:]), dtype=torch.complex64, device="cuda")
+        for si in range(0, m_in, rows):
+            Xc = _to_device_rows(Xh, si, rows)
+            Cb = torch.from_numpy(np.ascontiguousarray(C[si:si + Xc.shape[0], so:so + ro],
+                                                       np.complex64)).cuda()
+            combine_into(Cb, Xc, acc, X=acc, alpha=1.0)
+            del Xc
+        out[so:so + ro] = acc.cpu().numpy()
+        del acc
+    return out
+
+
+def gram_hosted(Ah, Bh=None, rows: int = 8):
+    """FP64 ⟨A_i, B_j⟩ of host-resident blocks (interior.py:112-126), chunk by
+    chunk through the device Gram; B defaults to A."""
+    if Bh is None:
+        Bh = Ah
+    ma, mb = int(Ah.shape[0]), int(Bh.shape[0])
+    Gm = np.zeros((ma, mb), np.complex128)
+    for si in range(0, ma, rows):
+        Ac = _to_device_rows(Ah, si, rows)
+        for sj in range(0, mb, rows):
+            Bc = _to_device_rows(Bh, sj, rows)
+            Gm[si:si + Ac.shape[0], sj:sj + Bc.shape[0]] = gram(Ac, Bc).cpu().numpy()
+            del Bc
+        del Ac
+    return Gm
+
+
+def svqb_hosted(Xh, passes: int = 2, rows: int = 8):
+    """SVQB of a host block (interior.py:129-138): FP64 Gram, host eigh, keep
+    w > 1e-5·max w, rotate with V·w^(−½); dropped directions become zero rows."""
+    for _ in range(passes):
+        Gm = gram_hosted(Xh, rows=rows)
+        Gm = (Gm + Gm.conj().T) / 2
+        w, V = np.linalg.eigh(Gm)
+        keep = w > SVQB_DROP * max(w.max(), 1e-300)
+        scale = np.where(keep, 1.0 / np.sqrt(np.where(keep, w, 1.0)), 0.0)
+        Xh = _stream_rotate(V * scale[None, :], Xh, rows)
+    return Xh
+
+
 # ---------------------------------------------------------------- RR extract
 def gram_chunked(A, B, rows: int = 8):
     """⟨A_i, B_j⟩ as host FP64 (interior.py:432-441).  One kernel call on the device."""

@@ -27,14 +27,27 @@ def _host(a):
     return np.asarray(a)
 
 
-def _atomic_save_npy(path: Path, arr) -> None:
-    tmp = path.with_suffix(".tmp.npy")
-    np.save(tmp, _host(arr))
-    tmp.replace(path)
+def _replace_atomically(final: Path, write) -> None:
+    """Run write(tmp) on a sibling temporary file, then rename it over `final`
+    (a crash leaves the old file or the new one, never a partial one).  The
+    temporary name keeps the final suffix, so numpy writes exactly that path."""
+    tmp = final.with_name(f"{final.stem}.tmp{final.suffix}")
+    write(tmp)
+    tmp.replace(final)
 
 
 class BlockCheckpointer:
-    """Persist locked eigenpairs block by block; resume a partial solve (io.py:19-67)."""
+    """Locked eigenpairs persisted block by block; a later run resumes from them.
+
+    Protocol of the reference's checkpointer (eigenfns/io.py:19-67), used by
+    ``lobpcg_blocks_resumable``: ``load() -> (vals, vecs, n_blocks, meta)``
+    and ``save_block(i, vals, vecs, extra=None)``.  On-disk format, shared
+    with the reference so either code resumes the other's runs:
+    ``<tag>_block<iii>_vals.npy`` / ``<tag>_block<iii>_vecs.npy`` per block
+    (never rewritten) and ``<tag>_meta.json`` whose ``n_blocks`` counts the
+    blocks that are complete on disk, plus the caller's metadata, the block's
+    ``extra`` entries and ``saved_at``.
+    """
 
     def __init__(self, directory, tag: str, meta: dict | None = None):
         self.dir = Path(directory)
@@ -42,55 +55,62 @@ class BlockCheckpointer:
         self.tag = tag
         self.meta = dict(meta or {})
 
-    def _meta_path(self) -> Path:
-        return self.dir / f"{self.tag}_meta.json"
+    # the format's file names
+    def _file(self, suffix: str) -> Path:
+        return self.dir / f"{self.tag}_{suffix}"
 
-    def _block_paths(self, i: int) -> tuple[Path, Path]:
-        return (self.dir / f"{self.tag}_block{i:03d}_vals.npy",
-                self.dir / f"{self.tag}_block{i:03d}_vecs.npy")
+    def _block_file(self, i: int, what: str) -> Path:
+        return self._file(f"block{i:03d}_{what}.npy")
+
+    def _read_meta(self) -> dict | None:
+        path = self._file("meta.json")
+        return json.loads(path.read_text()) if path.exists() else None
+
+    def _read_block(self, i: int):
+        files = [self._block_file(i, what) for what in ("vals", "vecs")]
+        absent = [f.name for f in files if not f.exists()]
+        if absent:
+            raise RuntimeError(f"checkpoint corrupt: the metadata counts block {i} but "
+                               f"{', '.join(absent)} is missing")
+        vals, vecs = (np.load(f, allow_pickle=False) for f in files)
+        if vals.shape[0] != vecs.shape[0]:
+            raise RuntimeError(f"checkpoint corrupt: block {i} holds {vals.shape[0]} eigenvalues "
+                               f"and {vecs.shape[0]} vectors")
+        return vals, vecs
 
     def load(self):
-        """(vals, vecs, n_blocks, meta); vals/vecs None when there is no checkpoint.
-        Raises RuntimeError if the metadata names a block whose files are missing."""
-        pm = self._meta_path()
-        if not pm.exists():
+        """(vals, vecs, n_blocks, meta) of every saved block in order; vals and vecs
+        are None (and n_blocks 0) when nothing has been saved.  RuntimeError when
+        a block the metadata counts is missing or inconsistent."""
+        meta = self._read_meta()
+        if meta is None:
             return None, None, 0, self.meta
-        meta = json.loads(pm.read_text())
-        n_blocks = int(meta.get("n_blocks", 0))
-        vals, vecs = [], []
-        for i in range(n_blocks):
-            pv, px = self._block_paths(i)
-            if not (pv.exists() and px.exists()):
-                raise RuntimeError(f"checkpoint corrupt: missing block {i} files")
-            vals.append(np.load(pv, allow_pickle=False))
-            vecs.append(np.load(px, allow_pickle=False))
-        if not vals:
+        count = int(meta.get("n_blocks", 0))
+        if count == 0:
             return None, None, 0, meta
-        return np.concatenate(vals), np.concatenate(vecs), n_blocks, meta
+        parts = [self._read_block(i) for i in range(count)]
+        return (np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts]),
+                count, meta)
 
     def save_block(self, i: int, vals, vecs, extra: dict | None = None) -> None:
-        """Append block i: the two arrays first, then the metadata that counts it."""
-        pv, px = self._block_paths(i)
-        _atomic_save_npy(pv, vals)
-        _atomic_save_npy(px, vecs)
-        meta = dict(self.meta)
-        meta.update(extra or {})
-        meta["n_blocks"] = i + 1
-        meta["saved_at"] = time.strftime("%Y-%m-%d %H:%M:%S")
-        tmp = self._meta_path().with_suffix(".tmp.json")
-        tmp.write_text(json.dumps(meta, indent=1))
-        tmp.replace(self._meta_path())
+        """Write block i's arrays, then the metadata that makes it count (so a crash
+        between the two leaves the previous checkpoint intact)."""
+        for what, arr in (("vals", vals), ("vecs", vecs)):
+            data = _host(arr)
+            _replace_atomically(self._block_file(i, what), lambda tmp, d=data: np.save(tmp, d))
+        record = {**self.meta, **(extra or {}), "n_blocks": i + 1,
+                  "saved_at": time.strftime("%Y-%m-%d %H:%M:%S")}
+        text = json.dumps(record, indent=1)
+        _replace_atomically(self._file("meta.json"), lambda tmp: tmp.write_text(text))
 
 
 def save_interior_state(path, X, stage: str, outer: int, n_theta: int, lam_max: float,
                         history: list) -> None:
     """The ChebSI basis and protocol position (run_interior.py:116-121)."""
-    path = Path(path)
-    tmp = path.with_name(path.stem + ".tmp.npz")
-    np.savez(tmp, X=_host(X), stage=np.array(stage), outer=np.array(int(outer)),
-             n_theta=np.array(int(n_theta)), lam_max=np.array(float(lam_max)),
-             history=np.array(json.dumps(history)))
-    tmp.replace(path)
+    fields = dict(X=_host(X), stage=np.array(stage), outer=np.array(int(outer)),
+                  n_theta=np.array(int(n_theta)), lam_max=np.array(float(lam_max)),
+                  history=np.array(json.dumps(history)))
+    _replace_atomically(Path(path), lambda tmp: np.savez(tmp, **fields))
 
 
 def load_interior_state(path):

@@ -12,6 +12,7 @@ solver.py:228-238 / interior.py:75-82.
 """
 from __future__ import annotations
 
+import contextlib
 import os
 import threading
 
@@ -397,22 +398,45 @@ _POOL_LOCK = threading.Lock()
 
 
 def _pool():
-    """Thread pool for the host's small FP64 eigensolves.  Creating it limits
-    host BLAS to one thread per call (process-wide): these matrices are ≤ 96 ×
-    96, where threaded BLAS inside a pool of threaded calls oversubscribes the
-    cores (measured 48 ms for 21 eigensolves of 72 × 72 with 8 BLAS threads
-    per call, 5–13 ms with one)."""
+    """Thread pool for the host's small FP64 eigensolves."""
     global _POOL
     with _POOL_LOCK:  # serving engines on several host threads may get here together
         if _POOL is None:
             from concurrent.futures import ThreadPoolExecutor
-            try:
-                from threadpoolctl import threadpool_limits
-                threadpool_limits(limits=1, user_api="blas")
-            except ImportError:  # without threadpoolctl the BLAS keeps its threads
-                pass
             _POOL = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
     return _POOL
+
+
+_BLAS_LOCK = threading.Lock()
+_BLAS_USERS = 0
+_BLAS_LIMITER = None
+
+
+@contextlib.contextmanager
+def _one_blas_thread():
+    """Host BLAS at one thread while a pooled batch of small eigensolves runs:
+    these matrices are ≤ 96 × 96, where threaded BLAS inside a pool of threaded
+    calls oversubscribes the cores (measured 48 ms for 21 eigensolves of
+    72 × 72 with 8 BLAS threads per call, 5–13 ms with one).  The BLAS thread
+    setting is process-wide, so concurrent batches share one limit, taken by
+    the first and restored to the caller's setting by the last."""
+    global _BLAS_USERS, _BLAS_LIMITER
+    with _BLAS_LOCK:
+        if _BLAS_USERS == 0:
+            try:
+                from threadpoolctl import threadpool_limits
+                _BLAS_LIMITER = threadpool_limits(limits=1, user_api="blas")
+            except ImportError:  # without threadpoolctl the BLAS keeps its threads
+                _BLAS_LIMITER = None
+        _BLAS_USERS += 1
+    try:
+        yield
+    finally:
+        with _BLAS_LOCK:
+            _BLAS_USERS -= 1
+            if _BLAS_USERS == 0 and _BLAS_LIMITER is not None:
+                _BLAS_LIMITER.restore_original_limits()
+                _BLAS_LIMITER = None
 
 
 def eigh_stack_host(a: np.ndarray, lowest: int | None = None):
@@ -423,7 +447,8 @@ def eigh_stack_host(a: np.ndarray, lowest: int | None = None):
     if a.shape[0] == 1:
         res = [np.linalg.eigh(a[0])]
     else:
-        res = list(_pool().map(np.linalg.eigh, list(a)))
+        with _one_blas_thread():
+            res = list(_pool().map(np.linalg.eigh, list(a)))
     w = np.stack([r[0] for r in res])
     V = np.stack([r[1] for r in res])
     if lowest is not None and lowest < a.shape[-1]:

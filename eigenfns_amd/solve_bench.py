@@ -65,6 +65,8 @@ def run_c1():
     return {"case": "C1", "grid": c["grid"], "nev": c["nev"], "m": c["m"], "guard": c["guard"],
             "tol": c["tol"], "gpu_seconds": round(t, 3), "iterations": st.iterations,
             "theta_applications": st.theta_applications,
+            "blocks": [{k: r[k] for k in ("iters", "max_res_locked", "converged", "stagnated")}
+                       for r in st.rounds],
             "max_rel_err_vs_fp64_oracle": _rel_err(vals, _golden("c1")), "vals": vals.tolist()}
 
 

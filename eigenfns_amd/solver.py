@@ -7,14 +7,26 @@ fresh Θ(W), Θ(P) for the Rayleigh–Ritz matrix, dead-row penalty, lock the
 lowest m−guard pairs, monotone final sort.  What changes is where things
 live: [X; W; P] and [ΘX; ΘW; ΘP] are two contiguous device blocks, so the
 Rayleigh–Ritz matrix is one Gram kernel and each update one combine kernel;
-the locked set stays in HBM (no host streaming); the small FP64 algebra runs
-on the device.  Rayleigh quotients are normalised by ‖x‖² (the reference's
-interior.py:69-74 fix) unless ``normalized_rq=False``.
+the locked set stays in HBM (no host streaming).  The small FP64 algebra
+(the SVQB and whitening eigensolves and the Rayleigh–Ritz eigenproblem, at
+most 3m × 3m) runs in host LAPACK as in the reference (solver.py:119-121,
+263-265, 301), on the Gram matrices the device sums in FP64.  Rayleigh
+quotients are normalised by ‖x‖² (the reference's interior.py:69-74 fix)
+unless ``normalized_rq=False``.
+
+One deliberate difference: a block whose lockable residuals have reached
+the FP32 floor just above `tol` (below STAGNATION_FACTOR·tol, no better by
+10 % over the last `stagnation` iterations) is locked there instead of
+iterating to `maxit`; the block's entry in ``SolveStats.rounds`` records
+``converged`` (worst locked residual < tol) and ``stagnated``, and a
+RuntimeWarning names the blocks that locked above tol.  ``stagnation=0``
+restores the reference's run-to-maxit behaviour.
 """
 from __future__ import annotations
 
 import gc
 import time
+import warnings
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -27,7 +39,7 @@ from .linalg import (COMBINE_IN_PLACE_MAX, DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_
                      svqb_coeffs)
 
 __all__ = ["gram", "combine", "deflate", "ortho_block", "apply_chunked", "deflate_chunk_rows",
-           "SolveStats", "lobpcg_blocks", "lobpcg_blocks_resumable", "_flat",
+           "SolveStats", "StagnationLock", "lobpcg_blocks", "lobpcg_blocks_resumable", "_flat",
            "_SVQB_DROP", "_RR_DEAD_PENALTY", "_DEFLATE_CHUNK"]
 
 _SVQB_DROP = SVQB_DROP
@@ -46,6 +58,40 @@ class SolveStats:
     wall_seconds: float = 0.0
     precond_applications: int = 0
     iterations: int = 0
+
+    @property
+    def converged(self) -> bool:
+        """Every locked block met the residual tolerance."""
+        return all(r.get("converged", True) for r in self.rounds)
+
+
+# A block stops at the FP32 residual floor when its worst lockable residual is
+# below STAGNATION_FACTOR·tol and the best of the last `stagnation` iterations
+# is not 10 % better than the best before them.
+STAGNATION_WINDOW = 20
+STAGNATION_FACTOR = 10.0
+
+
+class StagnationLock:
+    """Per-block record of the worst lockable residual; ``stalled`` says when
+    the block has stopped improving at the FP32 floor near `tol`."""
+
+    def __init__(self, tol: float, window: int = STAGNATION_WINDOW,
+                 factor: float = STAGNATION_FACTOR):
+        self.tol = float(tol)
+        self.window = int(window)
+        self.factor = float(factor)
+        self.hist: list = []
+
+    def reset(self) -> None:
+        self.hist = []
+
+    def stalled(self, worst: float) -> bool:
+        self.hist.append(float(worst))
+        w = self.window
+        if w <= 0 or len(self.hist) <= w or not worst < self.factor * self.tol:
+            return False
+        return min(self.hist[-w:]) > 0.9 * min(self.hist[:-w])
 
 
 def _deflate_inplace(Y, L, nl):
@@ -80,11 +126,14 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                   maxit: int = 400, seed: int = 0, precond_shift: str = "median",
                   verbose: bool = True, log_every: int = 0, initial_locked=None,
                   initial_vals=None, on_block=None, locked_storage: str = "auto",
-                  theta_chunk: int = 0, normalized_rq: bool = True):
+                  theta_chunk: int = 0, normalized_rq: bool = True,
+                  stagnation: int = STAGNATION_WINDOW):
     """Bottom-up deflated block LOBPCG.  Returns (eigenvalues, locked_vectors, stats).
 
     Locks the lowest (m − guard) Ritz pairs of each block once their relative
-    residuals pass `tol`; the guard Ritz vectors seed the next block.
+    residuals pass `tol` (or, with `stagnation` > 0, once they stall at the
+    FP32 floor below STAGNATION_FACTOR·tol); the guard Ritz vectors seed the
+    next block.
     `initial_locked`/`initial_vals` resume a partial solve;
     `on_block(n_locked_total, lam_block, X_block)` fires after each lock.
     `locked_storage` and `theta_chunk` are accepted for API compatibility: the
@@ -150,6 +199,8 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
         stats.theta_applications += m
         have_p = False
         n_lock = min(n_lock_per, nev - locked_vals.size)
+        stag = StagnationLock(tol, stagnation)
+        stalled = False
         it = 0
         for it in range(maxit):
             # one read of X and ΘX for ⟨x,Θx⟩, |x|², |Θx|²; R = ΘX − λX with |R|² in the
@@ -165,7 +216,11 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             lam_h = np.where(dead_rows, np.inf, lam_h)
             rn_h = np.where(dead_rows, np.inf, rn_h)
             order = np.argsort(lam_h)
-            if (rn_h[order[:n_lock]] < tol).all():
+            worst = float(rn_h[order[:n_lock]].max())
+            if worst < tol:
+                break
+            if stag.stalled(worst):
+                stalled = True
                 break
             if log_every and it % log_every == 0:
                 leak = float(gram(L[:nl], X).abs().max().item()) if nl else 0.0
@@ -247,8 +302,10 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
         carry = X[carry_idx].clone()
         n_locked_now += n_lock
         locked_vals = np.concatenate([locked_vals, lam[order[:n_lock]]])
+        worst = float(rn[order[:n_lock]].max())
         stats.rounds.append({"iters": it + 1, "locked": int(locked_vals.size),
-                             "max_res_locked": float(rn[order[:n_lock]].max())})
+                             "max_res_locked": worst, "converged": bool(worst < tol),
+                             "stagnated": stalled})
         if on_block is not None:
             on_block(int(locked_vals.size), lam[order[:n_lock]],
                      locked_buf[n_locked_now - n_lock:n_locked_now].cpu().numpy())
@@ -258,6 +315,12 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                   f"worst res {rn[order[:n_lock]].max():.1e}  "
                   f"elapsed {time.perf_counter() - t0:6.1f}s", flush=True)
     stats.wall_seconds = time.perf_counter() - t0
+    above = [i for i, r in enumerate(stats.rounds) if not r["converged"]]
+    if above:
+        warnings.warn(f"lobpcg_blocks: {len(above)} of {len(stats.rounds)} blocks locked with "
+                      f"residuals above tol={tol:g} (blocks {above}; worst "
+                      f"{max(stats.rounds[i]['max_res_locked'] for i in above):.2e})",
+                      RuntimeWarning, stacklevel=2)
     if (np.diff(locked_vals) < -1e-6 * np.abs(locked_vals[1:])).any():
         worst = float(np.min(np.diff(locked_vals)))
         print(f"WARNING: locked eigenvalues non-monotone (worst step {worst:.3e}) — a band was "

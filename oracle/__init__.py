@@ -12,10 +12,11 @@ reference file:line it follows (paths are relative to the reference checkout):
   operator_np   eigenfns/operator.py:44-184      make_basis, Θ, preconditioners
   linalg_np     eigenfns/solver.py:35-126        gram, combine, deflate, SVQB
   lobpcg_np     eigenfns/solver.py:136-382       deflated block LOBPCG
-  chebyshev_np  eigenfns/chebyshev.py:28-267     Lanczos λ_max, lowpass ChebSI, KPM count
+  chebyshev_np  eigenfns/chebyshev.py:28-64,213-267  Lanczos λ_max, KPM count
   interior_np   eigenfns/interior.py:37-88,312-427  bandpass coefficients, filter, RR
   kpm_np        scripts/exp/exp_kpm_dos.py:125-163, exp_kpm_analyze.py:22-70
   structure_np  eigenfns/structure.py:29-151     pen-like rasterizer
+  audit_np      scripts/exp/exp_i2_v2.py:129-152 deflated-probe completeness audit (I2)
   dense_np      tests/test_operator.py:16-24     dense Θ matrix by probing
   rng_np        (no reference counterpart)       counter-based RNG shared with the HIP kernels
 

@@ -6,10 +6,15 @@ Restates eigenfns/chebyshev.py:
 Random start vectors and probes come from ``rng_np`` (the reference's JAX
 threefry streams are not reproducible).
 
-Pinned by (tests/test_oracle_chebyshev.py): λ_max against the dense maximum
-and the rigorous bound max|k+G|²/ε_min at G=8; the KPM count against the exact
-count of dense eigenvalues below λ_b (within the stochastic error and the
-Jackson smearing), and against the direct moment sum Σ c_k·μ_k.
+Pinned by:
+  tests/test_oracle_chebyshev.py  Lanczos: θ_max + β_k equals the Rayleigh–Ritz
+      maximum of the Krylov space K_k(Θ, v₀) (Arnoldi with full
+      re-orthogonalisation on the dense FP64 matrix) plus the norm of Θq_k
+      outside it, for k = 3, 6, 10 and two seeds; the max over seeds; the
+      bracket of the dense λ_max at 24 steps.
+  tests/test_oracle_spectral.py   KPM count against the exact count of dense
+      eigenvalues below λ_b (within the stochastic error and the Jackson
+      smearing).
 """
 from __future__ import annotations
 

@@ -8,14 +8,18 @@ Restates eigenfns/interior.py:
   rr_extract         interior.py:37-88   (SVQB → A = XᴴΘX → dead penalty 1e8 →
                                            eigh → rotate → normalised RQ, relres)
   bandpass_subspace  interior.py:357-427
-and the two-stage protocol of scripts/run_interior.py:154-216 (``two_stage``).
+(the two-stage protocol of scripts/run_interior.py:154-216 is checked on the
+device against a dense FP64 eigendecomposition directly).
 
-Pinned by (tests/test_oracle_interior.py): Jackson kernel closed form
-(g₀ = 1, g_p → small, the kernel sums to a positive Fejér-like density);
-step coefficients against Gauss–Chebyshev quadrature of the indicator;
-apply_cheb_poly against V·diag(p(λ))·Vᴴ·X from a dense eigendecomposition at
-G=8; rr_extract on an invariant subspace (tests/test_interior.py:80-92 of
-the reference).
+Pinned by:
+  tests/test_oracle_spectral.py  Jackson kernel closed form (g₀ = 1, monotone,
+      the kernel is a non-negative density); step coefficients against
+      quadrature of the indicator; apply_cheb_poly against V·diag(p(λ))·Vᴴ·X
+      from a dense eigendecomposition at G=8; rr_extract on an invariant
+      subspace (tests/test_interior.py:80-92 of the reference).
+  tests/test_oracle_interior.py  bandpass_subspace at G=8 returns exactly the
+      dense window states (eigenvalues to 1e-9, subspace angles to 1e-9) and
+      honours the target and stagnation stop rules.
 """
 from __future__ import annotations
 

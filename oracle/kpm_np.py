@@ -8,10 +8,16 @@ Chebyshev–Gauss grid, smearing width).
   μ_{2k} = 2⟨T_k,T_k⟩ − μ₀,   μ_{2k+1} = 2⟨T_{k+1},T_k⟩ − μ₁,
   B = (2Θ − λ_max)/λ_max.
 
-Pinned by (tests/test_oracle_kpm.py): the doubling identities against the
-direct recurrence μ_k = ⟨z, T_k(B) z⟩; μ_k = Σ_i |⟨v_i, z⟩|²·T_k(x_i) from a
-dense eigendecomposition at G=8; μ₀ = 2G³ − 2 for Rademacher probes at Γ; the
-counting function at the top of the spectrum equals 2G³ − 2.
+Pinned by:
+  tests/test_oracle_spectral.py  the doubling identities against the direct
+      recurrence μ_k = ⟨z, T_k(B) z⟩; μ_k = Σ_i |⟨v_i, z⟩|²·T_k(x_i) from a
+      dense eigendecomposition at G=8; μ₀ = 2G³ − 2 for Rademacher probes at
+      Γ; the counting function at the top of the spectrum equals 2G³ − 2.
+  tests/test_oracle_kpm.py       dos_curve: its quadrature over λ is exactly
+      μ₀ (Chebyshev orthogonality on the midpoint grid); one state's curve
+      has the Jackson kernel's exact mean g₁·a and second moment
+      (1 + g₂T₂(a))/2; its integral below λ_b is the counting function.
+      smearing_width: the kernel's exact standard deviation to O(1/p).
 """
 from __future__ import annotations
 

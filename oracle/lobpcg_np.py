@@ -9,8 +9,9 @@ final monotone sort (:371-381).  Random rows come from ``rng_np`` instead of
 JAX threefry (row offset = block index × m), so iterates are not the
 reference's, only the converged eigenpairs are comparable.
 
-Pinned by (tests/test_oracle_lobpcg.py): the dense FP64 spectrum at G=8
-(tests/test_solver.py:36-48 of the reference, tightened to 1e-6), the
+Pinned by (tests/test_oracle_spectral.py: test_lobpcg_matches_dense,
+test_lobpcg_homogeneous_analytic): the dense FP64 spectrum at G=8
+(tests/test_solver.py:36-48 of the reference, tightened to 1e-10), the
 homogeneous analytic spectrum, monotonicity and orthonormality of the locked
 set.
 """
