@@ -410,6 +410,7 @@ def _pool():
 _BLAS_LOCK = threading.Lock()
 _BLAS_USERS = 0
 _BLAS_LIMITER = None
+_BLAS_CONTROLLER = None   # threadpoolctl's scan of the loaded BLAS libraries, made once
 
 
 @contextlib.contextmanager
@@ -420,12 +421,16 @@ def _one_blas_thread():
     72 × 72 with 8 BLAS threads per call, 5–13 ms with one).  The BLAS thread
     setting is process-wide, so concurrent batches share one limit, taken by
     the first and restored to the caller's setting by the last."""
-    global _BLAS_USERS, _BLAS_LIMITER
+    global _BLAS_USERS, _BLAS_LIMITER, _BLAS_CONTROLLER
     with _BLAS_LOCK:
         if _BLAS_USERS == 0:
             try:
-                from threadpoolctl import threadpool_limits
-                _BLAS_LIMITER = threadpool_limits(limits=1, user_api="blas")
+                if _BLAS_CONTROLLER is None:
+                    # scanning the process's libraries costs ms with torch loaded;
+                    # the controller keeps the scan, a limit from it costs µs
+                    from threadpoolctl import ThreadpoolController
+                    _BLAS_CONTROLLER = ThreadpoolController()
+                _BLAS_LIMITER = _BLAS_CONTROLLER.limit(limits=1, user_api="blas")
             except ImportError:  # without threadpoolctl the BLAS keeps its threads
                 _BLAS_LIMITER = None
         _BLAS_USERS += 1
