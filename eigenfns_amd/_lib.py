@@ -49,6 +49,7 @@ _SIGNATURES = {
     "efn_gram_engine": (_i, [_i]),
     "efn_combine_engine": (_i, [_i]),
     "efn_gram": (_i, [_vp, _i, _vp, _i, _i64, _vp, _vp, _sz, _vp]),
+    "efn_gram_noalloc": (_i, [_vp, _i, _vp, _i, _i64, _vp, _vp]),
     "efn_combine": (_i, [_vp, _i, _i, _vp, _i64, _vp, _f, _vp, _vp]),
     "efn_gram_multi_work_bytes": (_sz, [_i, _i, _i, _i64]),
     "efn_gram_multi": (_i, [_i, _vp, _vp, _i, _i, _i64, _vp, _vp, _sz, _vp]),

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -813,6 +814,56 @@ int efn_gram_multi_herm(int nprob, const void* const* A, const void* const* B, i
 int efn_gram(const void* A, int ma, const void* B, int mb, int64_t n, void* out, void* work,
              size_t work_bytes, void* stream) {
   return gram_multi_impl(1, &A, &B, ma, mb, n, out, work, work_bytes, false, stream);
+}
+
+}  // extern "C"
+
+namespace {
+// The split partials of efn_gram_noalloc live in this file-scope array: each
+// device that loads the library holds its own copy (EFN_GRAM_NOALLOC_MAX_BYTES)
+// until the process ends.  Host code never uses the symbol itself as an address
+// (on the host it is only a stand-in): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) unsigned char scratch[EFN_GRAM_NOALLOC_MAX_BYTES];
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_addr;  // indexed by device ordinal
+
+int scratch_address(void** out) {
+  int dev = 0;
+  EFN_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_addr.size() <= dev) g_scratch_addr.resize(dev + 1, nullptr);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(EFN_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return EFN_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int efn_gram_noalloc(const void* A, int ma, const void* B, int mb, int64_t n, void* out,
+                     void* stream) {
+  if (ma < 0 || mb < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
+  const size_t need = efn_gram_work_bytes(ma, mb, n);
+  if (need > (size_t)EFN_GRAM_NOALLOC_MAX_BYTES)
+    return fail(EFN_EINVAL, "efn_gram_noalloc: " + std::to_string(ma) + "x" + std::to_string(mb) +
+                                " over n = " + std::to_string((long long)n) + " needs " +
+                                std::to_string(need) + " bytes of split partials; the static scratch holds " +
+                                std::to_string((size_t)EFN_GRAM_NOALLOC_MAX_BYTES) +
+                                " (use efn_gram with a work buffer)");
+  void* work = nullptr;
+  int rc;
+  if ((rc = scratch_address(&work))) return rc;
+  // every partial the sum kernel reads is written by the partial kernels of the
+  // same call, so the scratch needs no clearing between calls
+  return gram_multi_impl(1, &A, &B, ma, mb, n, out, work, (size_t)EFN_GRAM_NOALLOC_MAX_BYTES, false,
+                         stream);
 }
 
 namespace {

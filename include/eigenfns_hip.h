@@ -144,6 +144,20 @@ int efn_gram_engine(int engine);
 int efn_gram(const void* A_c64, int ma, const void* B_c64, int mb, int64_t n, void* out_c128,
              void* work, size_t work_bytes, void* stream);
 
+/* efn_gram without a work buffer and without any allocation, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The split partials go to a static device array of EFN_GRAM_NOALLOC_MAX_BYTES
+ * that each device holds for the life of the process; calls needing more
+ * (efn_gram_work_bytes(ma, mb, n) > the maximum, e.g. beyond 96 × 96 at 192³)
+ * return EFN_EINVAL.  Results are bitwise those of efn_gram (same kernels, same
+ * splits).  Every call on one device shares that device's array, so calls that
+ * overlap on one device (different streams, host threads) race: order them on
+ * one stream. */
+#define EFN_GRAM_NOALLOC_MAX_BYTES (40u << 20)
+int efn_gram_noalloc(const void* A_c64, int ma, const void* B_c64, int mb, int64_t n,
+                     void* out_c128, void* stream);
+
 /* Gram blocks of `nprob` (≤ 32) independent problems in one launch:
  * out[p] = A_pᴴ·B_p (ma×mb complex128 each, contiguous), A_p / B_p device
  * pointers given in host arrays.  solver.py:35-43 per problem. */
