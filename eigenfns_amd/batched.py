@@ -33,8 +33,8 @@ import torch
 
 from .linalg import (COMBINE_IN_PLACE_MAX, RR_DEAD_PENALTY, SVQB_DROP, combine_multi,
                      combine_multi_split, copy_multi, eigh_batched_device, eigh_stack_host,
-                     gram_multi, rowreduce_strided, row_axpby_norm_strided, rowstats_strided,
-                     MAX_PROBLEMS)
+                     gram_multi, one_blas_thread, rowreduce_strided, row_axpby_norm_strided,
+                     rowstats_strided, MAX_PROBLEMS)
 from .operator import MaxwellOperator, precondition_multi, theta_multi
 from .solver import STAGNATION_WINDOW, StagnationLock
 
@@ -490,8 +490,9 @@ def solve_batched(requests, grid_size, m=24, guard=8, nslots=8, max_nev=None, to
                         verify_dead_rows=verify_dead_rows, small_dense=small_dense)
     pending = list(requests)
     done = []
-    while pending or eng.active():
-        while pending and eng.free_slots():
-            eng.admit(pending.pop(0))
-        done.extend(eng.step())
+    with one_blas_thread():
+        while pending or eng.active():
+            while pending and eng.free_slots():
+                eng.admit(pending.pop(0))
+            done.extend(eng.step())
     return done

@@ -414,13 +414,16 @@ _BLAS_CONTROLLER = None   # threadpoolctl's scan of the loaded BLAS libraries, m
 
 
 @contextlib.contextmanager
-def _one_blas_thread():
+def one_blas_thread():
     """Host BLAS at one thread while a pooled batch of small eigensolves runs:
     these matrices are ≤ 96 × 96, where threaded BLAS inside a pool of threaded
     calls oversubscribes the cores (measured 48 ms for 21 eigensolves of
     72 × 72 with 8 BLAS threads per call, 5–13 ms with one).  The BLAS thread
     setting is process-wide, so concurrent batches share one limit, taken by
-    the first and restored to the caller's setting by the last."""
+    the first and restored to the caller's setting by the last.  A serving
+    loop holds it for its whole run: restoring a larger thread count makes
+    OpenBLAS start its threads again, which cost the engines' host phases
+    30 % when the limit came and went with each batch."""
     global _BLAS_USERS, _BLAS_LIMITER, _BLAS_CONTROLLER
     with _BLAS_LOCK:
         if _BLAS_USERS == 0:
@@ -452,7 +455,7 @@ def eigh_stack_host(a: np.ndarray, lowest: int | None = None):
     if a.shape[0] == 1:
         res = [np.linalg.eigh(a[0])]
     else:
-        with _one_blas_thread():
+        with one_blas_thread():
             res = list(_pool().map(np.linalg.eigh, list(a)))
     w = np.stack([r[0] for r in res])
     V = np.stack([r[1] for r in res])

@@ -21,6 +21,7 @@ import time
 import numpy as np
 
 from .batched import BatchedLOBPCG, Request
+from .linalg import one_blas_thread
 from .operator import MaxwellOperator
 from .workloads import NET_SEEDS, Spec, network, request_stream  # noqa: F401 (re-exported)
 
@@ -117,8 +118,9 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     max_nev = max(sp.nev for sp in specs)
     plan = plan_slots(grids, m, guard, max_nev, max_slots, hbm_budget, engine_voxels)
     if not threads:
-        return _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose,
-                                 small_dense)
+        with one_blas_thread():
+            return _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose,
+                                     small_dense)
     import threading
     import torch
     queues = {g: sorted([sp for sp in specs if sp.grid == g], key=lambda s: s.t_arrive)
@@ -174,10 +176,11 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
 
     workers = [threading.Thread(target=worker, args=(g, e), name=f"engine-{g}.{e}")
                for g in grids for e in range(k)]
-    for w in workers:
-        w.start()
-    for w in workers:
-        w.join()
+    with one_blas_thread():   # host BLAS at one thread while the engines' pools run
+        for w in workers:
+            w.start()
+        for w in workers:
+            w.join()
     if errors:
         raise errors[0]
     return done, _summary(done, time.perf_counter() - t0, turns[0], engines)

@@ -137,11 +137,13 @@ def run_c3(n_k: int | None = None, plain: bool = True, bucket_step: int = 4):
             except BaseException as exc:
                 errors.append(exc)
 
+        from .linalg import one_blas_thread
         ths = [threading.Thread(target=worker, args=(b, it)) for b, it in buckets.items()]
-        for th in ths:
-            th.start()
-        for th in ths:
-            th.join()
+        with one_blas_thread():
+            for th in ths:
+                th.start()
+            for th in ths:
+                th.join()
         if errors:
             raise errors[0]
         return done

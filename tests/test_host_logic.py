@@ -1,4 +1,5 @@
 """Host-side bookkeeping of the batched engine (no GPU needed)."""
+import pytest
 
 
 def test_plan_slots_budget_and_work_cap():
@@ -13,3 +14,31 @@ def test_plan_slots_budget_and_work_cap():
     assert all(capped[g] * g ** 3 <= 8 * 64 ** 3 or capped[g] == 1 for g in grids)
     # a cap below one slot still leaves one slot per grid
     assert plan_slots(grids, 24, 8, 64, 32, 200e9, engine_voxels=1)[32] == 1
+
+
+def test_blas_thread_limit_is_scoped_and_restored():
+    """The host eigensolve pool limits BLAS to one thread only while it runs
+    (nested and concurrent holders share one limit); afterwards the caller's
+    setting is back."""
+    import threading
+    import numpy as np
+    pytest.importorskip("threadpoolctl")
+    from threadpoolctl import threadpool_info
+    from eigenfns_amd import linalg
+
+    def blas_threads():
+        return sorted(i["num_threads"] for i in threadpool_info() if i["user_api"] == "blas")
+
+    before = blas_threads()
+    a = np.random.default_rng(0).standard_normal((6, 16, 16)) + 0j
+    linalg.eigh_stack_host(a)
+    assert blas_threads() == before
+    with linalg.one_blas_thread():
+        assert all(n == 1 for n in blas_threads())
+        ths = [threading.Thread(target=linalg.eigh_stack_host, args=(a,)) for _ in range(4)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        assert all(n == 1 for n in blas_threads())
+    assert blas_threads() == before
