@@ -47,6 +47,8 @@ _SIGNATURES = {
     "efn_kpm_count_step": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _f, _i, _vp]),
     "efn_gram_work_bytes": (_sz, [_i, _i, _i64]),
     "efn_gram_engine": (_i, [_i]),
+    "efn_theta_engine": (_i, [_i]),
+    "efn_theta_passes": (_i, [_i]),
     "efn_combine_engine": (_i, [_i]),
     "efn_gram": (_i, [_vp, _i, _vp, _i, _i64, _vp, _vp, _sz, _vp]),
     "efn_gram_noalloc": (_i, [_vp, _i, _vp, _i, _i64, _vp, _vp]),

@@ -2,6 +2,7 @@
 // Declarations and reference citations: include/eigenfns_hip.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -15,6 +16,7 @@
 #include "eig_kernels.hpp"
 #include "operator_kernels.hpp"
 #include "raster_kernels.hpp"
+#include "theta3_kernels.hpp"
 
 
 namespace {
@@ -98,6 +100,31 @@ struct efn_ctx {
 
 namespace {
 
+// Process-wide Θ pass structure (efn_theta_engine): EFN_THETA_AUTO takes the
+// per-grid choice of theta_passes_auto, 3 and 5 force one form where compiled.
+std::atomic<int> g_theta_engine{EFN_THETA_AUTO};
+
+// The plane pass holds one G×G plane of a component on chip: it fits one CU's
+// registers and LDS without spilling up to 128³ (at 192³ the 48 resident
+// complex values per thread of a 768-thread block left the compiler ~90
+// VGPRs short: spilled to memory, it measured slower than five passes).
+template <int G>
+constexpr bool has_three_pass() { return G == 64 || G == 96 || G == 128; }
+
+// The pass structure used for grid G when the engine is EFN_THETA_AUTO: three
+// passes wherever they are compiled (Θ on a 32-vector block, 10,000-point
+// network, default chunk: 64³ 128,026 vs 105,982 vector-applies/s, 96³ 29,107
+// vs 28,800, 128³ 14,413 vs 13,342).
+template <int G>
+constexpr int theta_passes_auto() { return has_three_pass<G>() ? 3 : 5; }
+
+template <int G>
+int theta_passes() {
+  const int e = g_theta_engine.load();
+  const int want = e == EFN_THETA_AUTO ? theta_passes_auto<G>() : e;
+  return (want == 3 && has_three_pass<G>()) ? 3 : 5;
+}
+
 // reduce per-block partials of one chunk and apply the KPM formulas
 __global__ void reduce_partials_kernel(const double* __restrict__ partial, int nb, int nvec,
                                        int mode, double* __restrict__ mom, int ld, int k,
@@ -132,9 +159,73 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
   }
 }
 
+// xout with the request's operator kind and epilogue
+template <int G>
+void launch_xout(const OpRequest& rq, const EpiArgs& ea, dim3 grid, long long v0, const float2* ws,
+                 const float2* twid, hipStream_t s) {
+  using namespace efn;
+  const dim3 blk(xout_threads<G>());
+  if (rq.kind == kPrecond) {
+    hipLaunchKernelGGL((xout_kernel<G, kPrecond, kStore>), grid, blk, 0, s, ws, rq.vm, v0, twid, ea);
+    return;
+  }
+  switch (rq.epi) {
+    case kStore:
+      hipLaunchKernelGGL((xout_kernel<G, kTheta, kStore>), grid, blk, 0, s, ws, rq.vm, v0, twid, ea);
+      break;
+    case kCheb:
+      hipLaunchKernelGGL((xout_kernel<G, kTheta, kCheb>), grid, blk, 0, s, ws, rq.vm, v0, twid, ea);
+      break;
+    case kKpm:
+      hipLaunchKernelGGL((xout_kernel<G, kTheta, kKpm>), grid, blk, 0, s, ws, rq.vm, v0, twid, ea);
+      break;
+    default:
+      hipLaunchKernelGGL((xout_kernel<G, kTheta, kDot>), grid, blk, 0, s, ws, rq.vm, v0, twid, ea);
+      break;
+  }
+}
+
+// Three passes per chunk: xin | yzmid | xout (theta3_kernels.hpp).
+template <int G>
+int run_op3(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
+  using namespace efn;
+  constexpr int nb = G * x3_tiles<G>();  // xin/xout blocks per vector
+  const int use_eps = rq.kind == kTheta ? 0 : 1;
+  for (int v0 = 0; v0 < rq.m; v0 += c->chunk) {
+    const int nv = rq.m - v0 < c->chunk ? rq.m - v0 : c->chunk;
+    float2* const ws = c->d_ws;
+    const dim3 gx((unsigned)(nv * nb)), gm((unsigned)(3 * nv * G));
+    if (rq.kind == kTheta)
+      hipLaunchKernelGGL((xin_kernel<G, kTheta>), gx, dim3(xin_threads<G>()), 0, s, rq.in, ws, rq.vm,
+                         (long long)v0, c->d_twid);
+    else
+      hipLaunchKernelGGL((xin_kernel<G, kPrecond>), gx, dim3(xin_threads<G>()), 0, s, rq.in, ws, rq.vm,
+                         (long long)v0, c->d_twid);
+    EFN_LAUNCH_CHECK();
+    hipLaunchKernelGGL((yzmid_kernel<G>), gm, dim3(mid_threads<G>()), 0, s, ws, rq.vm, (long long)v0,
+                       use_eps, c->d_twid, (long long)nv * G);
+    EFN_LAUNCH_CHECK();
+    EpiArgs ea = rq.ea;
+    ea.x = rq.in;
+    ea.partial = c->d_partial;
+    launch_xout<G>(rq, ea, gx, (long long)v0, ws, c->d_twid, s);
+    EFN_LAUNCH_CHECK();
+    if (rq.reduce_mode) {
+      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nv), dim3(256), 0, s, c->d_partial, nb, nv,
+                         rq.reduce_mode, rq.mom ? rq.mom + (long long)v0 * rq.ld : nullptr, rq.ld,
+                         rq.k, rq.est ? rq.est + v0 : nullptr, rq.ck64);
+      EFN_LAUNCH_CHECK();
+    }
+  }
+  return EFN_OK;
+}
+
 template <int G>
 int run_op(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
   using namespace efn;
+  if constexpr (has_three_pass<G>()) {
+    if (theta_passes<G>() == 3) return run_op3<G>(c, rq, s);
+  }
   constexpr int R = zrows3<G>();
   constexpr int nyt = (G + strided_tile<G, 1>() - 1) / strided_tile<G, 1>();
   constexpr int nxt = (G + xtile_for(G) - 1) / xtile_for(G);
@@ -194,6 +285,7 @@ struct GridEntry {
   int (*run)(efn_ctx*, const OpRequest&, hipStream_t);
   int (*real)(efn_ctx*, const OpRequest&, int, const float*, void*, hipStream_t);
   int zrows;  // z-pencils per zin/zout block
+  int (*passes)();  // pass structure Θ uses now (3 or 5)
 };
 
 // Real-space fields (modes: efn::RealMode): zin forms the Cartesian field
@@ -232,7 +324,7 @@ int run_real(efn_ctx* c, const OpRequest& rq, int mode, const float* lam, void* 
   return EFN_OK;
 }
 
-#define EFN_GRID(g) {g, &run_op<g>, &run_real<g>, efn::zrows3<g>()}
+#define EFN_GRID(g) {g, &run_op<g>, &run_real<g>, efn::zrows3<g>(), &theta_passes<g>}
 const GridEntry kGrids[] = {EFN_GRID(4),   EFN_GRID(6),   EFN_GRID(8),   EFN_GRID(12),
                             EFN_GRID(16),  EFN_GRID(24),  EFN_GRID(32),  EFN_GRID(48),
                             EFN_GRID(64),  EFN_GRID(96),  EFN_GRID(128), EFN_GRID(192),
@@ -293,7 +385,7 @@ int rowreduce_nsplit(long long n) {
 
 extern "C" {
 
-int efn_version(void) { return 200; }  // 0.2.0: adds efn_combine_multi_split
+int efn_version(void) { return 300; }  // 0.3.0: three-pass Θ, efn_theta_engine / efn_theta_passes, efn_gram_noalloc
 
 const char* efn_last_error(void) { return g_last_error.c_str(); }
 
@@ -393,7 +485,8 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   }
   c->chunk = chunk;
   c->ws_bytes = per_vec * chunk;
-  const int nb = G * G / ge->zrows;
+  // per-block partials: zout blocks (G²/rows) or xout blocks (G·G/16), whichever is more
+  const int nb = std::max(G * G / ge->zrows, G * std::max(1, G / efn::kX3Tile));
 #define EFN_ALLOC(ptr, bytes)                                                         \
   do {                                                                                \
     if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess) {                           \
@@ -725,6 +818,19 @@ int efn_combine_engine(int engine) {
   if (engine < 0) return g_combine_mfma.load();
   if (engine > 1) return fail(EFN_EINVAL, "unknown combine engine");
   return g_combine_mfma.exchange(engine);
+}
+
+int efn_theta_engine(int engine) {
+  if (engine < 0) return g_theta_engine.load();
+  if (engine != EFN_THETA_AUTO && engine != 3 && engine != 5)
+    return fail(EFN_EINVAL, "theta engine must be EFN_THETA_AUTO (0), 3 or 5");
+  return g_theta_engine.exchange(engine);
+}
+
+int efn_theta_passes(int grid_size) {
+  const GridEntry* ge = find_grid(grid_size);
+  if (!ge) return fail(EFN_EUNSUPPORTED, "grid size " + std::to_string(grid_size) + " has no compiled kernels");
+  return ge->passes();
 }
 
 int efn_gram_engine(int engine) {

@@ -237,6 +237,30 @@ __device__ __forceinline__ void step1_store(float2 (&v)[P::N1], int n2, float2* 
   for (int k1 = 0; k1 < N1; ++k1) col[(N2 + 1) * k1] = v[k1];
 }
 
+// step1_store for register-bound kernels: the twiddles are read from LDS
+// four at a time, each group consumed before the next is fetched, so at
+// most four are held in registers (the plain version lets the compiler fetch
+// all N1 − 1 of them ahead of the DFT).
+template <int N, int S, class P = Plan<N>>
+__device__ __forceinline__ void step1_store_lean(float2 (&v)[P::N1], int n2, float2* __restrict__ pencil,
+                                                 const float2* __restrict__ twid) {
+  constexpr int N1 = P::N1, N2 = P::N2;
+  DFT<N1, S>::run(v);
+#pragma unroll
+  for (int k0 = 1; k0 < N1; k0 += 4) {
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k1 = k0; k1 < k0 + 4 && k1 < N1; ++k1) {
+      float2 w = twid[n2 * k1];
+      if (S > 0) w.y = -w.y;
+      v[k1] = cmul(v[k1], w);
+    }
+  }
+  float2* col = pencil + n2;
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) col[(N2 + 1) * k1] = v[k1];
+}
+
 template <int N, int S, class P = Plan<N>>
 __device__ __forceinline__ void step3_load(float2 (&r)[P::N2], int k1, const float2* __restrict__ pencil) {
   constexpr int N2 = P::N2;

@@ -280,6 +280,28 @@ class MaxwellOperator:
         return self._realspace(Hs, 2, eigvals, (G, G, G), torch.float32)
 
 
+def theta_engine(engine: int | None = None) -> int:
+    """Process-wide pass structure of Θ, the preconditioner and the fused steps:
+    0 = the library's per-grid choice, 3 = three passes (xin | yzmid | xout,
+    grids 128 and 192), 5 = five passes.  Returns the previous setting (None:
+    query only)."""
+    lib = _lib.load()
+    if engine is None:
+        return int(lib.efn_theta_engine(-1))
+    prev = int(lib.efn_theta_engine(int(engine)))
+    if prev < 0:
+        _lib.check(prev, "theta_engine")
+    return prev
+
+
+def theta_passes(grid_size: int) -> int:
+    """The pass structure (3 or 5) Θ uses now at this grid size."""
+    rc = int(_lib.load().efn_theta_passes(int(grid_size)))
+    if rc < 0:
+        _lib.check(rc, "theta_passes")
+    return rc
+
+
 def _multi(kind, ws_op, ops, in_base, in_off, out_base, out_off, nrow, alpha=1.0, beta=0.0):
     import ctypes as C
     n = len(ops)

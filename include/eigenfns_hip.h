@@ -118,6 +118,19 @@ int efn_kpm_step(efn_ctx* ctx, const void* T1_c64, void* T0_c64, double* mom_dev
 int efn_kpm_count_step(efn_ctx* ctx, const void* T1_c64, void* T0_c64, const void* Z_c64,
                        double* est_dev, double ck, float alpha, int m, void* stream);
 
+/* Process-wide pass structure of Θ and the preconditioner (and their fused
+ * Chebyshev / KPM epilogues); returns the previous engine (engine < 0: query).
+ *   EFN_THETA_AUTO  the per-grid choice compiled into the library (default)
+ *   3               xin | yzmid | xout: 132 B of compulsory traffic per voxel
+ *                   and vector, on grids that have the plane pass (64, 96, 128)
+ *   5               zin | y | x·ε·x | y | zout: 228 B per voxel and vector
+ * Either form computes Θ of operator.py:82-95 (the two differ only by FP32
+ * rounding).  No reference counterpart (XLA schedules the reference's FFTs). */
+#define EFN_THETA_AUTO 0
+int efn_theta_engine(int engine);
+/* The pass structure (3 or 5) Θ uses now at this grid size. */
+int efn_theta_passes(int grid_size);
+
 /* Gram block out[i][j] = Σ_k conj(A[i][k])·B[j][k] (solver.py:35-43), products
  * of the FP32 inputs summed in FP64; out is ma×mb complex128 on the device.
  * n = complex entries per row (2·G³ for a block).  work is a caller-provided
