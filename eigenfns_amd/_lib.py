@@ -49,6 +49,8 @@ _SIGNATURES = {
     "efn_gram_engine": (_i, [_i]),
     "efn_theta_engine": (_i, [_i]),
     "efn_theta_passes": (_i, [_i]),
+    "efn_ctx_set_passes": (_i, [_vp, _i]),
+    "efn_theta_three_pass": (_i, [_i]),
     "efn_combine_engine": (_i, [_i]),
     "efn_gram": (_i, [_vp, _i, _vp, _i, _i64, _vp, _vp, _sz, _vp]),
     "efn_gram_noalloc": (_i, [_vp, _i, _vp, _i, _i64, _vp, _vp]),

@@ -64,6 +64,7 @@ struct OpRequest {
 
 struct efn_ctx {
   int G = 0;
+  int passes = 0;                // 0: the engine's choice; 3 or 5: this context's (efn_ctx_set_passes)
   double L = 0.0;
   double k[3] = {0, 0, 0};
   int chunk = 0;
@@ -118,12 +119,20 @@ constexpr bool has_three_pass() { return G == 64 || G == 96 || G == 128; }
 template <int G>
 constexpr int theta_passes_auto() { return has_three_pass<G>() ? 3 : 5; }
 
+// A forced engine (3 or 5) wins; under EFN_THETA_AUTO a context's own choice
+// (ctx_passes, e.g. from the tuning table) wins over the compiled default.
 template <int G>
-int theta_passes() {
+int theta_passes(int ctx_passes = 0) {
   const int e = g_theta_engine.load();
-  const int want = e == EFN_THETA_AUTO ? theta_passes_auto<G>() : e;
+  const int want = e != EFN_THETA_AUTO ? e : (ctx_passes ? ctx_passes : theta_passes_auto<G>());
   return (want == 3 && has_three_pass<G>()) ? 3 : 5;
 }
+
+template <int G>
+int theta_passes_default() { return theta_passes<G>(0); }
+
+template <int G>
+bool three_pass_compiled() { return has_three_pass<G>(); }
 
 // reduce per-block partials of one chunk and apply the KPM formulas
 __global__ void reduce_partials_kernel(const double* __restrict__ partial, int nb, int nvec,
@@ -224,7 +233,7 @@ template <int G>
 int run_op(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
   using namespace efn;
   if constexpr (has_three_pass<G>()) {
-    if (theta_passes<G>() == 3) return run_op3<G>(c, rq, s);
+    if (theta_passes<G>(c->passes) == 3) return run_op3<G>(c, rq, s);
   }
   constexpr int R = zrows3<G>();
   constexpr int nyt = (G + strided_tile<G, 1>() - 1) / strided_tile<G, 1>();
@@ -285,7 +294,8 @@ struct GridEntry {
   int (*run)(efn_ctx*, const OpRequest&, hipStream_t);
   int (*real)(efn_ctx*, const OpRequest&, int, const float*, void*, hipStream_t);
   int zrows;  // z-pencils per zin/zout block
-  int (*passes)();  // pass structure Θ uses now (3 or 5)
+  int (*passes)(int);     // pass structure Θ uses now (3 or 5) for a context's choice
+  bool (*has3)();         // the plane pass is compiled for this grid
 };
 
 // Real-space fields (modes: efn::RealMode): zin forms the Cartesian field
@@ -324,7 +334,7 @@ int run_real(efn_ctx* c, const OpRequest& rq, int mode, const float* lam, void* 
   return EFN_OK;
 }
 
-#define EFN_GRID(g) {g, &run_op<g>, &run_real<g>, efn::zrows3<g>(), &theta_passes<g>}
+#define EFN_GRID(g) {g, &run_op<g>, &run_real<g>, efn::zrows3<g>(), &theta_passes<g>, &three_pass_compiled<g>}
 const GridEntry kGrids[] = {EFN_GRID(4),   EFN_GRID(6),   EFN_GRID(8),   EFN_GRID(12),
                             EFN_GRID(16),  EFN_GRID(24),  EFN_GRID(32),  EFN_GRID(48),
                             EFN_GRID(64),  EFN_GRID(96),  EFN_GRID(128), EFN_GRID(192),
@@ -830,7 +840,24 @@ int efn_theta_engine(int engine) {
 int efn_theta_passes(int grid_size) {
   const GridEntry* ge = find_grid(grid_size);
   if (!ge) return fail(EFN_EUNSUPPORTED, "grid size " + std::to_string(grid_size) + " has no compiled kernels");
-  return ge->passes();
+  return ge->passes(0);
+}
+
+int efn_theta_three_pass(int grid_size) {
+  const GridEntry* ge = find_grid(grid_size);
+  if (!ge) return fail(EFN_EUNSUPPORTED, "grid size " + std::to_string(grid_size) + " has no compiled kernels");
+  return ge->has3() ? 1 : 0;
+}
+
+int efn_ctx_set_passes(efn_ctx* ctx, int passes) {
+  if (!ctx) return fail(EFN_EINVAL, "ctx is NULL");
+  const GridEntry* ge = find_grid(ctx->G);
+  if (passes < 0) return ge->passes(ctx->passes);
+  if (passes != 0 && passes != 3 && passes != 5) return fail(EFN_EINVAL, "passes must be 0, 3 or 5");
+  if (passes == 3 && !ge->has3())
+    return fail(EFN_EUNSUPPORTED, "grid size " + std::to_string(ctx->G) + " has no three-pass kernels");
+  ctx->passes = passes;
+  return ge->passes(passes);
 }
 
 int efn_gram_engine(int engine) {

@@ -74,11 +74,15 @@ class MaxwellOperator:
 
     operator.py:130-184.  `dtype` must be complex64 (the GPU path computes in
     complex FP32; FP64 lives in the small dense algebra and the oracle).
-    `max_chunk` bounds the workspace (vectors per pass; 0 = ~1 GiB).
+    `max_chunk` bounds the workspace (vectors per launch set; 0 = the tuning
+    table's choice for this grid, else the library default; −1 = no workspace,
+    for operators that only lend their tables to multi-operator launches).
+    `passes` picks Θ's pass structure for this operator (0 = the tuning table's
+    choice, else the library's; 3 or 5 to force one).
     """
 
     def __init__(self, eps, box_size: float, k_frac=(0.0, 0.0, 0.0), dtype=torch.complex64,
-                 max_chunk: int = 0):
+                 max_chunk: int = 0, passes: int = 0, tuned: bool = True):
         eps_np = eps.detach().cpu().numpy() if isinstance(eps, torch.Tensor) else np.asarray(eps)
         if eps_np.ndim != 3 or len(set(eps_np.shape)) != 1:
             raise ValueError(f"eps must be a cube, got {eps_np.shape}")
@@ -92,6 +96,12 @@ class MaxwellOperator:
         if not lib.efn_grid_supported(self.grid_size):
             raise ValueError(f"grid size {self.grid_size} has no compiled kernels")
         self._dev = device()
+        if tuned and max_chunk >= 0 and (max_chunk == 0 or passes == 0):
+            from .autotune import choice
+            entry = choice(self.grid_size)
+            if entry:
+                max_chunk = max_chunk or int(entry["chunk"])
+                passes = passes or int(entry["passes"])
         eps32 = np.ascontiguousarray(eps_np, dtype=np.float32)
         handle = C.c_void_p()
         with torch.cuda.device(self._dev):
@@ -99,6 +109,12 @@ class MaxwellOperator:
                                           eps32.ctypes.data, int(max_chunk), C.byref(handle)),
                        "efn_ctx_create")
         self._ctx = handle
+        if passes:
+            rc = int(lib.efn_ctx_set_passes(self._ctx, int(passes)))
+            if rc < 0:
+                lib.efn_ctx_destroy(self._ctx)
+                self._ctx = None
+                _lib.check(rc, "passes")
         kf, emi, ch, wsb = C.c_double(), C.c_double(), C.c_int(), C.c_size_t()
         _lib.check(lib.efn_ctx_query(self._ctx, C.byref(kf), C.byref(emi), C.byref(ch),
                                      C.byref(wsb)))
@@ -106,11 +122,17 @@ class MaxwellOperator:
         self.eps_mean_inv = float(np.mean(1.0 / eps32))
         self.chunk = int(ch.value)
         self.workspace_bytes = int(wsb.value)
+        self._passes_req = int(passes)
         self._eps32 = eps32
         self._basis = None
         self._inv_eps = None
         self._eps_grid = None
         self._mask = None
+
+    @property
+    def passes(self) -> int:
+        """The pass structure (3 or 5) this operator's launches use now."""
+        return int(_lib.load().efn_ctx_set_passes(self._ctx, -1))
 
     def __del__(self):
         ctx = getattr(self, "_ctx", None)

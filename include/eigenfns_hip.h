@@ -130,6 +130,16 @@ int efn_kpm_count_step(efn_ctx* ctx, const void* T1_c64, void* T0_c64, const voi
 int efn_theta_engine(int engine);
 /* The pass structure (3 or 5) Θ uses now at this grid size. */
 int efn_theta_passes(int grid_size);
+/* 1 if the three-pass form is compiled for this grid size, else 0 (host
+ * only, no device needed). */
+int efn_theta_three_pass(int grid_size);
+/* This context's pass structure: 0 follows the engine, 3 or 5 applies while
+ * the engine is EFN_THETA_AUTO (a forced engine still wins); passes < 0 only
+ * queries.  Returns the form the context's launches use now (3 or 5), or
+ * EFN_EUNSUPPORTED for 3 on a grid without the plane pass.  Set it before the
+ * context is used on a stream; the tuning table (eigenfns_amd.autotune) sets
+ * it at creation.  No reference counterpart. */
+int efn_ctx_set_passes(efn_ctx* ctx, int passes);
 
 /* Gram block out[i][j] = Σ_k conj(A[i][k])·B[j][k] (solver.py:35-43), products
  * of the FP32 inputs summed in FP64; out is ma×mb complex128 on the device.

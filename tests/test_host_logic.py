@@ -42,3 +42,35 @@ def test_blas_thread_limit_is_scoped_and_restored():
             t.join()
         assert all(n == 1 for n in blas_threads())
     assert blas_threads() == before
+
+
+def test_tuning_table_lookup_and_user_override(tmp_path, monkeypatch):
+    """The committed tuning table is read for a grid, the user table wins over
+    it, and grids in neither table give no entry unless first-use tuning is on."""
+    import json
+    from eigenfns_amd import autotune
+    user = tmp_path / "t.json"
+    monkeypatch.setenv("EFN_TUNING_CACHE", str(user))
+    monkeypatch.delenv("EFN_AUTOTUNE", raising=False)
+    autotune.reload()
+    pkg = json.loads(autotune.PACKAGE_TABLE.read_text())[autotune.ARCH] if autotune.PACKAGE_TABLE.exists() else {}
+    for G, e in pkg.items():
+        assert e["passes"] in (3, 5) and e["chunk"] >= 1
+        assert autotune.choice(int(G)) == e
+    user.write_text(json.dumps({autotune.ARCH: {"64": {"passes": 5, "chunk": 7}}}))
+    autotune.reload()
+    assert autotune.choice(64) == {"passes": 5, "chunk": 7}
+    assert autotune.choice(4) is None or "4" in pkg
+    autotune.reload()
+    monkeypatch.delenv("EFN_TUNING_CACHE")
+    autotune.reload()
+
+
+def test_tuning_candidates_respect_the_workspace_cap():
+    from eigenfns_amd import autotune
+    for G in (32, 64, 128, 192, 256):
+        c = autotune.candidates(G)
+        assert all(ch * 3 * G ** 3 * 8 <= 1.1e9 for _, ch in c)
+        assert (5, autotune.default_chunk(G)) in c
+    assert {p for p, _ in autotune.candidates(128)} == {3, 5}
+    assert {p for p, _ in autotune.candidates(192)} == {5}
