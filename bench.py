@@ -42,6 +42,7 @@ def algorithmic_bytes_per_vector(G, m):
     return 32 * G ** 3 + 4 * G ** 3 / m
 
 
+MFMA_F32_PEAK_TFS = 157.3   # dense FP32 matrix cores, MI355X
 MFMA_F64_PEAK_TFS = 78.6
 TRAFFIC_PROFILE = "profiles/theta_apply_192_m32_pmc.md"
 
@@ -108,6 +109,22 @@ def path_rooflines(op, X, Y, G, stream, reps=5):
                          "mfma_TFLOPs": round(0.75 * flops / t / 1e12, 2),
                          "frac_mfma_f64": round(0.75 * flops / t / 1e12 / MFMA_F64_PEAK_TFS, 4),
                          "engine": "mfma_f64"}
+    # the opt-in FP32 matrix-core engine (four real products per complex entry,
+    # FP32 partial sums of at most 64 products added into FP64) on the same shape
+    from eigenfns_amd.linalg import gram_engine
+    prev = gram_engine("mfma_f32")
+    try:
+        t32 = _launch_seconds(lambda: gram(S, S), 2, stream)
+        g32 = gram(S, S)
+    finally:
+        gram_engine(prev)
+    g64 = gram(S, S)
+    out["gram_96x96_mfma_f32"] = {
+        "launch_ms": round(t32 * 1e3, 3), "TFLOPs": round(flops / t32 / 1e12, 2),
+        "mfma_TFLOPs": round(flops / t32 / 1e12, 2),
+        "frac_mfma_f32": round(flops / t32 / 1e12 / MFMA_F32_PEAK_TFS, 4),
+        "max_abs_diff_vs_f64_engine_rel": float((g32 - g64).abs().max() / g64.abs().max())}
+    del g32, g64
     C = torch.randn(ms, m, dtype=torch.complex64, device=X.device)
     t = _launch_seconds(lambda: combine_into(C, S, Y), reps, stream)
     byts = 16.0 * G3 * (ms + m)

@@ -61,6 +61,8 @@ _SIGNATURES = {
     "efn_combine_multi": (_i, [_i, _vp, _vp, _i, _i, _i64, _vp, _f, _vp, _vp]),
     "efn_combine_multi_split": (_i, [_i, _vp, _vp, _i, _i, _i64, _vp, _vp, _i, _vp]),
     "efn_heev_batched": (_i, [_i, _i, _vp, _vp, _vp, _i, _vp]),
+    "efn_heevd_work_bytes": (_sz, [_i, _i]),
+    "efn_heevd_batched": (_i, [_i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
     "efn_rowreduce_work_bytes": (_sz, [_i, _i64]),
     "efn_rowreduce": (_i, [_i, _vp, _vp, _i, _i64, _vp, _vp, _sz, _vp]),
     "efn_row_axpby": (_i, [_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp]),

@@ -31,8 +31,8 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .linalg import (COMBINE_IN_PLACE_MAX, RR_DEAD_PENALTY, SVQB_DROP, combine_multi,
-                     combine_multi_split, copy_multi, eigh_batched_device, eigh_stack_host,
+from .linalg import (COMBINE_IN_PLACE_MAX, DEAD_SPREAD, RR_DEAD_PENALTY, SVQB_DROP, combine_multi,
+                     combine_multi_split, copy_multi, dead_spread, eigh_batched_device, eigh_stack_host,
                      gram_multi, one_blas_thread, rowreduce_strided, row_axpby_norm_strided,
                      rowstats_strided, MAX_PROBLEMS)
 from .operator import MaxwellOperator, precondition_multi, theta_multi
@@ -229,7 +229,8 @@ class BatchedLOBPCG:
         keep = (w > SVQB_DROP * torch.clamp(wmax, min=1e-300)) & (wmax > 0)
         scale = torch.where(keep, 1.0 / torch.sqrt(torch.where(keep, w, torch.ones_like(w))),
                             torch.zeros_like(w))
-        return V * scale[:, None, :], keep
+        # select rather than scale by zero: a dropped direction's vector may be non-finite
+        return torch.where(keep[:, None, :], V * scale[:, None, :], torch.zeros_like(V)), keep
 
     def _whiten_small(self, G):
         """(V·w^{-1/2} with dropped directions zeroed, keep mask) of a Gram stack:
@@ -280,7 +281,8 @@ class BatchedLOBPCG:
         for j in range(len(idx)):   # symmetrised, dead rows penalised (solver.py:294-297)
             A[j] = (A[j] + A[j].conj().T) / 2
             penalty = max(RR_DEAD_PENALTY, 100.0 * float(np.abs(np.diag(A[j])).max()))
-            A[j][np.diag_indices(3 * m)] += np.where(dead[j], penalty, 0.0)
+            A[j][np.diag_indices(3 * m)] += np.where(dead[j], penalty * (1.0 + DEAD_SPREAD * np.arange(3 * m)),
+                                                     0.0)
         _, C = eigh_stack_host(A, lowest=m)
         Cp = C.copy()
         Cp[:, :m] = 0
@@ -299,7 +301,8 @@ class BatchedLOBPCG:
         A = (A + A.conj().transpose(1, 2)) / 2
         penalty = torch.clamp(100.0 * torch.diagonal(A, dim1=1, dim2=2).abs().max(dim=1).values,
                               min=RR_DEAD_PENALTY)
-        A = A + torch.diag_embed(torch.where(dead, penalty[:, None], 0.0).to(A.dtype))
+        A = A + torch.diag_embed(torch.where(dead, penalty[:, None] * dead_spread(3 * m, A.device)[None, :],
+                                             0.0).to(A.dtype))
         _, C = eigh_batched_device(A, lowest=m)
         Cp = C.clone()
         Cp[:, :m] = 0

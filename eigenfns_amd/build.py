@@ -28,6 +28,9 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+# rocSOLVER's batched Hermitian eigensolver (small dense FP64 algebra) and the
+# rocBLAS handle it runs on
+LINK_FLAGS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def hipcc() -> str:
@@ -40,7 +43,7 @@ def hipcc() -> str:
 def build(verbose: bool = True, extra_flags=()) -> Path:
     """Compile the library (always recompiles); returns its path."""
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, *extra_flags, "-o", str(tmp), *map(str, SOURCES)]
+    cmd = [hipcc(), *HIPCC_FLAGS, *extra_flags, "-o", str(tmp), *map(str, SOURCES), *LINK_FLAGS]
     if verbose:
         print("[eigenfns_amd.build]", " ".join(cmd), flush=True)
     res = subprocess.run(cmd, cwd=str(ROOT), capture_output=True, text=True)

@@ -1,12 +1,14 @@
 // C ABI of libeigenfns_hip.so: operator contexts and kernel dispatch.
 // Declarations and reference citations: include/eigenfns_hip.h.
 #include <hip/hip_runtime.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -1105,8 +1107,247 @@ int efn_heev_batched(int nprob, int n, const void* A, double* w, void* V, int ma
     if (e != hipSuccess) return fail(EFN_EHIP, std::string("heev LDS size: ") + hipGetErrorString(e));
   }
   hipLaunchKernelGGL(efn::heev_jacobi_kernel, dim3((unsigned)nprob), dim3(efn::kEigThreads), lds,
-                     (hipStream_t)stream, (const double2*)A, n, max_sweeps, w, (double2*)V);
+                     (hipStream_t)stream, (const double2*)A, n, max_sweeps, w, (double2*)V, (const int*)nullptr);
   EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// In place: each n×n block M ← Mᴴ (rocSOLVER's column-major eigenvectors of
+// conj(A) read row-major are Vᵀ with V the eigenvectors of A; conjugating the
+// transpose gives V with column j the eigenvector of w[j]).
+__global__ void conj_transpose_kernel(double2* __restrict__ M, int n) {
+  double2* a = M + (long long)blockIdx.x * n * n;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - i * n;
+    if (i < j) {
+      const double2 x = a[i * n + j], y = a[j * n + i];
+      a[i * n + j] = make_double2(y.x, -y.y);
+      a[j * n + i] = make_double2(x.x, -x.y);
+    } else if (i == j) {
+      a[i * n + i].y = -a[i * n + i].y;
+    }
+  }
+}
+
+// Per matrix: 1 if every entry is exactly zero (a block without P rows gives
+// an all-zero Gram matrix, which the divide and conquer does not return as
+// w = 0, V = I).
+__global__ void zero_matrix_flags_kernel(const double2* __restrict__ A, int n, int* __restrict__ flag) {
+  __shared__ int nz;
+  if (threadIdx.x == 0) nz = 0;
+  __syncthreads();
+  const double2* a = A + (long long)blockIdx.x * n * n;
+  int any = 0;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) any |= (a[e].x != 0.0 || a[e].y != 0.0);
+  if (any) atomicOr(&nz, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) flag[blockIdx.x] = nz ? 0 : 1;
+}
+
+// Per matrix: 1 if any of its eigenvalues or eigenvector entries is not finite.
+__global__ void nonfinite_flags_kernel(const double* __restrict__ w, const double2* __restrict__ V, int n,
+                                       int* __restrict__ flag) {
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const double2* v = V + (long long)blockIdx.x * n * n;
+  int any = 0;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) any |= !(isfinite(v[e].x) && isfinite(v[e].y));
+  for (int i = threadIdx.x; i < n; i += blockDim.x) any |= !isfinite(w[(long long)blockIdx.x * n + i]);
+  if (any) atomicOr(&bad, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) flag[blockIdx.x] = bad;
+}
+
+__global__ void zero_matrix_fix_kernel(const int* __restrict__ flag, int n, double* __restrict__ w,
+                                       double2* __restrict__ V) {
+  if (!flag[blockIdx.x]) return;
+  double2* v = V + (long long)blockIdx.x * n * n;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - i * n;
+    v[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) w[(long long)blockIdx.x * n + i] = 0.0;
+}
+
+// Per matrix C = op(X)·Y, n × n complex128, row-major.  mode 0: X is a
+// Hermitian matrix of which only the upper triangle (and the real diagonal)
+// is read; 1: X as stored; 2: Xᴴ.  strides in elements (0: one matrix for
+// the whole batch).
+__global__ void small_zgemm_kernel(const double2* __restrict__ X, long long sx, int mode,
+                                   const double2* __restrict__ Y, long long sy,
+                                   double2* __restrict__ C, long long sc, int n) {
+  const double2* x = X + blockIdx.x * sx;
+  const double2* y = Y + blockIdx.x * sy;
+  double2* c = C + blockIdx.x * sc;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - i * n;
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < n; ++k) {
+      double2 a;
+      if (mode == 0) {
+        a = i < k ? x[i * n + k] : (i == k ? make_double2(x[i * n + i].x, 0.0)
+                                           : make_double2(x[k * n + i].x, -x[k * n + i].y));
+      } else if (mode == 1) {
+        a = x[i * n + k];
+      } else {
+        a = make_double2(x[k * n + i].x, -x[k * n + i].y);
+      }
+      const double2 b = y[k * n + j];
+      re = fma(a.x, b.x, fma(-a.y, b.y, re));
+      im = fma(a.x, b.y, fma(a.y, b.x, im));
+    }
+    c[e] = make_double2(re, im);
+  }
+}
+
+// A fixed unitary n × n matrix per (device, n): the orthonormalised columns
+// of a seeded complex Gaussian matrix (modified Gram–Schmidt in FP64, twice),
+// uploaded on first use and kept for the life of the process.
+int fixed_unitary(int n, const double2** out) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, double2*> cache;
+  int dev = 0;
+  EFN_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto key = std::make_pair(dev, n);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    std::vector<double2> q((size_t)n * n);
+    uint64_t st = 0x5EEDull + (uint64_t)n;
+    auto next = [&st]() {  // splitmix64 → uniform (0, 1)
+      uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    for (auto& v : q) {
+      const double r = std::sqrt(-2.0 * std::log(next())), t = 2.0 * M_PI * next();
+      v = make_double2(r * std::cos(t), r * std::sin(t));
+    }
+    for (int pass = 0; pass < 2; ++pass)
+      for (int j = 0; j < n; ++j) {  // column j against columns < j, then normalised
+        for (int k = 0; k < j; ++k) {
+          double pr = 0.0, pi = 0.0;  // ⟨q_k, q_j⟩
+          for (int i = 0; i < n; ++i) {
+            const double2 a = q[(size_t)i * n + k], b = q[(size_t)i * n + j];
+            pr += a.x * b.x + a.y * b.y;
+            pi += a.x * b.y - a.y * b.x;
+          }
+          for (int i = 0; i < n; ++i) {
+            const double2 a = q[(size_t)i * n + k];
+            q[(size_t)i * n + j].x -= pr * a.x - pi * a.y;
+            q[(size_t)i * n + j].y -= pr * a.y + pi * a.x;
+          }
+        }
+        double nr = 0.0;
+        for (int i = 0; i < n; ++i) nr += q[(size_t)i * n + j].x * q[(size_t)i * n + j].x + q[(size_t)i * n + j].y * q[(size_t)i * n + j].y;
+        nr = 1.0 / std::sqrt(nr);
+        for (int i = 0; i < n; ++i) { q[(size_t)i * n + j].x *= nr; q[(size_t)i * n + j].y *= nr; }
+      }
+    double2* d = nullptr;
+    EFN_HIP_CHECK(hipMalloc((void**)&d, q.size() * sizeof(double2)));
+    EFN_HIP_CHECK(hipMemcpy(d, q.data(), q.size() * sizeof(double2), hipMemcpyHostToDevice));
+    it = cache.emplace(key, d).first;
+  }
+  *out = it->second;
+  return EFN_OK;
+}
+
+// One rocBLAS handle per (host thread, device): creating one costs
+// milliseconds; the stream is set on every call.
+int rocblas_handle_for(rocblas_handle* out) {
+  thread_local std::map<int, rocblas_handle> handles;
+  int dev = 0;
+  EFN_HIP_CHECK(hipGetDevice(&dev));
+  auto it = handles.find(dev);
+  if (it == handles.end()) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success || !h)
+      return fail(EFN_EHIP, "rocblas_create_handle failed");
+    it = handles.emplace(dev, h).first;
+  }
+  *out = it->second;
+  return EFN_OK;
+}
+}  // namespace
+
+extern "C" {
+
+size_t efn_heevd_work_bytes(int nprob, int n) {
+  if (nprob <= 0 || n <= 0) return 0;
+  const size_t e = (size_t)nprob * n * sizeof(double);
+  const size_t t = (size_t)nprob * n * n * sizeof(double2);
+  return ((e + 255) / 256) * 256 + ((3 * (size_t)nprob * sizeof(int) + 255) / 256) * 256 + t;
+}
+
+int efn_heevd_batched(int nprob, int n, const void* A, double* w, void* V, void* work, size_t work_bytes,
+                      void* stream) {
+  if (nprob < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
+  if (nprob == 0 || n == 0) return EFN_OK;
+  if (!A || !w || !V) return fail(EFN_EINVAL, "NULL pointer");
+  if (A == V) return fail(EFN_EINVAL, "A and V must not alias");
+  if (!work || work_bytes < efn_heevd_work_bytes(nprob, n)) return fail(EFN_EINVAL, "heevd work buffer too small");
+  hipStream_t s = (hipStream_t)stream;
+  rocblas_handle h;
+  int rc;
+  if ((rc = rocblas_handle_for(&h))) return rc;
+  if (rocblas_set_stream(h, s) != rocblas_status_success) return fail(EFN_EHIP, "rocblas_set_stream failed");
+  const size_t e_bytes = (((size_t)nprob * n * sizeof(double) + 255) / 256) * 256;
+  const size_t i_bytes = ((3 * (size_t)nprob * sizeof(int) + 255) / 256) * 256;
+  double* E = (double*)work;
+  rocblas_int* info = (rocblas_int*)((char*)work + e_bytes);
+  int* zero = (int*)(info + nprob);
+  int* redo = zero + nprob;
+  double2* T = (double2*)((char*)work + e_bytes + i_bytes);
+  hipLaunchKernelGGL(zero_matrix_flags_kernel, dim3((unsigned)nprob), dim3(256), 0, s, (const double2*)A, n, zero);
+  EFN_LAUNCH_CHECK();
+  // B = Qᴴ·A·Q with a fixed unitary Q: the same eigenvalues, and no exactly
+  // zero column (the dead rows of a Rayleigh–Ritz matrix are decoupled from
+  // the rest, which made the divide and conquer return non-finite vectors)
+  const double2* Q;
+  if ((rc = fixed_unitary(n, &Q))) return rc;
+  const long long nn = (long long)n * n;
+  hipLaunchKernelGGL(small_zgemm_kernel, dim3((unsigned)nprob), dim3(256), 0, s, (const double2*)A, nn, 0, Q,
+                     0LL, T, nn, n);
+  hipLaunchKernelGGL(small_zgemm_kernel, dim3((unsigned)nprob), dim3(256), 0, s, Q, 0LL, 2, (const double2*)T,
+                     nn, (double2*)V, nn, n);
+  EFN_LAUNCH_CHECK();
+  // the row-major B read column-major is Bᵀ = conj(B); its lower triangle is B's upper one
+  const rocblas_status st = rocsolver_zheevd_strided_batched(
+      h, rocblas_evect_original, rocblas_fill_lower, n, (rocblas_double_complex*)V, n,
+      (rocblas_stride)n * n, w, (rocblas_stride)n, E, (rocblas_stride)n, info, nprob);
+  if (st != rocblas_status_success)
+    return fail(EFN_EHIP, "rocsolver_zheevd_strided_batched: status " + std::to_string((int)st));
+  hipLaunchKernelGGL(conj_transpose_kernel, dim3((unsigned)nprob), dim3(256), 0, s, (double2*)V, n);
+  // eigenvectors of A: Q·U
+  hipLaunchKernelGGL(small_zgemm_kernel, dim3((unsigned)nprob), dim3(256), 0, s, Q, 0LL, 1, (const double2*)V,
+                     nn, T, nn, n);
+  EFN_HIP_CHECK(hipMemcpyAsync(V, T, (size_t)nprob * nn * sizeof(double2), hipMemcpyDeviceToDevice, s));
+  EFN_LAUNCH_CHECK();
+  hipLaunchKernelGGL(zero_matrix_fix_kernel, dim3((unsigned)nprob), dim3(256), 0, s, zero, n, w, (double2*)V);
+  EFN_LAUNCH_CHECK();
+  // safety net: a matrix whose decomposition came back non-finite (the divide
+  // and conquer failed on exactly degenerate clusters) is solved again by the
+  // Jacobi kernel, which skips every other matrix
+  if (n <= efn::kEigMaxN) {
+    hipLaunchKernelGGL(nonfinite_flags_kernel, dim3((unsigned)nprob), dim3(256), 0, s, w, (const double2*)V, n, redo);
+    const int P = (n + (n & 1)) / 2;
+    const size_t lds = (size_t)n * (n + 1) * sizeof(double2) + 3 * (size_t)P * sizeof(double2) +
+                       (size_t)n * sizeof(double);
+    if (lds > 65536) {
+      const hipError_t e = hipFuncSetAttribute((const void*)efn::heev_jacobi_kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return fail(EFN_EHIP, std::string("heev LDS size: ") + hipGetErrorString(e));
+    }
+    hipLaunchKernelGGL(efn::heev_jacobi_kernel, dim3((unsigned)nprob), dim3(efn::kEigThreads), lds, s,
+                       (const double2*)A, n, 30, w, (double2*)V, (const int*)redo);
+    EFN_LAUNCH_CHECK();
+  }
   return EFN_OK;
 }
 

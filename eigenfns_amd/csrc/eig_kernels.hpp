@@ -45,10 +45,13 @@ __device__ __forceinline__ int circle_at(int r, int j, int N) { return j == 0 ? 
 
 // grid: (nprob); block: kEigThreads.  A_in: nprob × n × n, w: nprob × n,
 // V: nprob × n × n.  Dynamic LDS: n·(n + 1) double2 + 3·(N/2) double2-sized
-// rotation slots + n doubles.
+// rotation slots + n doubles.  redo (may be NULL): only the matrices with
+// redo[p] != 0 are solved, the others left as they are.
 __global__ __launch_bounds__(kEigThreads) void heev_jacobi_kernel(const double2* __restrict__ A_in, int n,
                                                                   int max_sweeps, double* __restrict__ w_out,
-                                                                  double2* __restrict__ V_out) {
+                                                                  double2* __restrict__ V_out,
+                                                                  const int* __restrict__ redo) {
+  if (redo && !redo[blockIdx.x]) return;  // uniform per block
   extern __shared__ double2 lds[];
   const int ld = n + 1;
   double2* A = lds;                          // n × ld

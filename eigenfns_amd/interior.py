@@ -16,7 +16,7 @@ import time
 import numpy as np
 import torch
 
-from .linalg import (SVQB_DROP, combine, combine_into, eigh, gram, gram_herm, normalize_rows,
+from .linalg import (SVQB_DROP, combine, combine_into, dead_spread, eigh, gram, gram_herm, normalize_rows,
                      ortho_block, row_axpby, rowdot, rownorm, rownorm2, axpby)
 
 
@@ -155,7 +155,8 @@ def rr_extract(op, X, theta_chunk: int = 8):
     HX = op.theta(X)
     A = gram_herm(X, HX)
     dead = rownorm(X) < 0.5
-    A = (A + A.conj().T) / 2 + torch.diag(torch.where(dead, 1e8, 0.0).to(A.dtype))
+    A = (A + A.conj().T) / 2 + torch.diag(torch.where(dead, 1e8 * dead_spread(A.shape[0], A.device),
+                                                      0.0).to(A.dtype))
     wA, VA = eigh(A)
     Xr = torch.empty_like(X)
     combine_into(VA, X, Xr)

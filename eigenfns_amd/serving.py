@@ -101,7 +101,7 @@ def _summary(done, wall, turns, engines):
 
 def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slots=32,
                   hbm_budget=200e9, verbose=False, threads=True, progress=False,
-                  engines_per_grid=2, engine_voxels=None, small_dense="host"):
+                  engines_per_grid=2, engine_voxels=None, small_dense="host", max_nev=None):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
 
     threads=True: each grid's engine runs in its own host thread on its own HIP
@@ -112,10 +112,19 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     (threads, streams) pulling from the same queue, so one half's host work
     (five small eigensolves and syncs per iteration) overlaps the other
     half's kernels: 2 measured +16 % solves/s over 1 (48 requests, 32³–64³),
-    3 no further gain."""
+    3 no further gain.
+
+    Slots are sized for the service's band limit `max_nev` (default: the
+    request stream configuration's upper bound, 64), not for the requests
+    seen so far; a request asking for more is rejected (ValueError) before
+    anything runs."""
     cache = cache or StructureCache()
     grids = sorted({sp.grid for sp in specs})
-    max_nev = max(sp.nev for sp in specs)
+    from .workloads import C5
+    max_nev = int(max_nev or C5["nev_range"][1])
+    too_many = [sp.rid for sp in specs if sp.nev > max_nev]
+    if too_many:
+        raise ValueError(f"requests {too_many[:8]} ask for more than max_nev={max_nev} bands")
     plan = plan_slots(grids, m, guard, max_nev, max_slots, hbm_budget, engine_voxels)
     if not threads:
         with one_blas_thread():

@@ -33,7 +33,8 @@ import numpy as np
 import torch
 
 from .linalg import (COMBINE_IN_PLACE_MAX, DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_DROP, apply_chunked,
-                     combine, combine_into, combine_multi_split, deflate, deflate_chunk_rows, eigh,
+                     combine, combine_into, combine_multi_split, dead_spread, deflate, deflate_chunk_rows,
+                     eigh,
                      gram, gram_herm, normalize_rows, ortho_block, row_axpby,
                      row_axpby_norm_strided, rowdot, rownorm, rownorm2, rowstats_strided,
                      svqb_coeffs)
@@ -258,7 +259,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             A = (A + A.conj().T) / 2
             dead = rownorm(Sb) < 0.5
             penalty = max(RR_DEAD_PENALTY, 100.0 * float(torch.diagonal(A).abs().max().item()))
-            A = A + torch.diag(torch.where(dead, penalty, 0.0).to(A.dtype))
+            A = A + torch.diag(torch.where(dead, penalty * dead_spread(nb, A.device), 0.0).to(A.dtype))
             wA, VA = eigh(A)
             # Xn = S·c/‖S·c‖, Pn = S·c'/‖S·c'‖ (c' = c with its X rows zeroed;
             # solver.py:298-307).  S is orthonormal by construction and dead rows

@@ -12,6 +12,9 @@ Cases (all from eigenfns_amd.workloads):
             nev = width − 4, guard 4, tol 1e-8.
   srv_<rid> the serving sample's requests at 32³ with nev ≤ 32: oracle
             LOBPCG, block 32, guard 8, tol 1e-8.
+  c5_<rid>  one request per grid 48³, 64³, 96³, 128³ of the C5 stream
+            (request_stream(2000, seed 0); the smallest nev, first by id):
+            oracle LOBPCG, block 16, guard 8, tol 1e-8.
 
 Output: tests/golden/solve_golden.npz (eigenvalues and the case parameters
 as JSON).  Run:  python tests/golden/make_solve_golden.py [case ...]
@@ -91,7 +94,31 @@ def case_srv():
     return out
 
 
-CASES = {"c1": case_c1, "c2dense": case_c2dense, "c3": case_c3, "srv": case_srv}
+def c5_golden_specs():
+    """One request per grid 48³-128³ from the C5 stream (2000 requests, seed 0):
+    the smallest nev, first by request id."""
+    specs = wl.request_stream(2000, 0, rate=None)
+    out = []
+    for G in (48, 64, 96, 128):
+        c = sorted((s for s in specs if s.grid == G), key=lambda s: (s.nev, s.rid))
+        out.append(c[0])
+    return out
+
+
+def case_c5():
+    out = {}
+    for sp in c5_golden_specs():
+        eps, L = wl.network(sp.n_points, sp.grid)
+        t0 = time.perf_counter()
+        vals, st = _lobpcg(eps, L, sp.k_frac, sp.nev, 16, 8, seed=sp.seed)
+        out[f"c5_{sp.rid}"] = (vals, dict(rid=sp.rid, n_points=sp.n_points, grid=sp.grid, k=sp.k_frac,
+                                          nev=sp.nev, solver="oracle LOBPCG", tol=TOL, block=16,
+                                          guard=8, stream="request_stream(2000, seed=0)"))
+        print(f"c5_{sp.rid}: {sp.grid}^3 nev {sp.nev} in {time.perf_counter() - t0:.0f} s", flush=True)
+    return out
+
+
+CASES = {"c1": case_c1, "c2dense": case_c2dense, "c3": case_c3, "srv": case_srv, "c5": case_c5}
 
 
 def main(names):

@@ -1,5 +1,7 @@
-"""GPU tests of the batched Hermitian eigensolver (efn_heev_batched, FP64
-Jacobi) against LAPACK (numpy.linalg.eigh) on the same FP64 matrices.
+"""GPU tests of the batched Hermitian eigensolvers — efn_heev_batched (the
+library's FP64 Jacobi kernel) and efn_heevd_batched (rocSOLVER's batched
+divide and conquer) — against LAPACK (numpy.linalg.eigh) on the same FP64
+matrices.
 
 Bars: eigenvalues within 1e-13·max|λ| of LAPACK's; residual ‖AV − VΛ‖ and
 ‖VᴴV − I‖ below 1e-12 (relative to max|λ|); the smallest eigenvalues of a
@@ -39,19 +41,24 @@ def _check(A, w, V):
     return wr
 
 
+METHODS = ["dc", "jacobi"]
+
+
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("n", [1, 2, 7, 24, 48, 72, 96])
-def test_heev_matches_lapack(n):
+def test_heev_matches_lapack(n, method):
     import torch
     from eigenfns_amd.linalg import eigh_batched_device
     rng = np.random.default_rng(n)
     mats = np.stack([_herm(rng, n) for _ in range(5)])
-    w, V = eigh_batched_device(torch.from_numpy(mats).cuda())
+    w, V = eigh_batched_device(torch.from_numpy(mats).cuda(), method=method)
     w, V = w.cpu().numpy(), V.cpu().numpy()
     for j in range(len(mats)):
         _check(mats[j], w[j], V[j])
 
 
-def test_heev_gram_and_rayleigh_ritz_shapes():
+@pytest.mark.parametrize("method", METHODS)
+def test_heev_gram_and_rayleigh_ritz_shapes(method):
     """Near-dependent Gram matrix with a zero row (SVQB) and a 72×72 PSD matrix
     with eigenvalues 3e-2..1e3 plus three dead rows at 1e4 (Rayleigh–Ritz)."""
     import torch
@@ -61,27 +68,67 @@ def test_heev_gram_and_rayleigh_ritz_shapes():
     Y[5] = Y[3] + 1e-6 * Y[7]
     Y[9] = 0
     G = Y.conj() @ Y.T
-    w, V = eigh_batched_device(torch.from_numpy(G[None]).cuda())
+    w, V = eigh_batched_device(torch.from_numpy(G[None]).cuda(), method=method)
     _check(G, w.cpu().numpy()[0], V.cpu().numpy()[0])
     A = _rr_like(rng, 72, dead=(50, 60, 70))
-    w, V = eigh_batched_device(torch.from_numpy(A[None]).cuda())
+    w, V = eigh_batched_device(torch.from_numpy(A[None]).cuda(), method=method)
     w0 = w.cpu().numpy()[0]
     wr = _check(A, w0, V.cpu().numpy()[0])
     assert np.abs(w0[:24] - wr[:24]).max() < 1e-10 * np.abs(wr[:24]).min()
     # lowest subset
-    wl, Vl = eigh_batched_device(torch.from_numpy(A[None]).cuda(), lowest=24)
+    wl, Vl = eigh_batched_device(torch.from_numpy(A[None]).cuda(), lowest=24, method=method)
     assert wl.shape == (1, 24) and Vl.shape == (1, 72, 24)
     assert torch.equal(wl, w[:, :24]) and torch.equal(Vl, V[:, :, :24])
 
 
-def test_heev_batch_invariant_and_limits():
+@pytest.mark.parametrize("method", METHODS)
+def test_heev_batch_invariant_and_limits(method):
     import torch
     from eigenfns_amd.linalg import eigh_batched_device
     rng = np.random.default_rng(9)
     mats = torch.from_numpy(np.stack([_rr_like(rng, 48) for _ in range(6)])).cuda()
-    wb, Vb = eigh_batched_device(mats)
+    wb, Vb = eigh_batched_device(mats, method=method)
     for j in range(6):
-        w1, V1 = eigh_batched_device(mats[j:j + 1])
+        w1, V1 = eigh_batched_device(mats[j:j + 1], method=method)
         assert torch.equal(w1[0], wb[j]) and torch.equal(V1[0], Vb[j])
-    with pytest.raises(ValueError):
-        eigh_batched_device(torch.zeros((1, 97, 97), dtype=torch.complex128, device="cuda"))
+    if method == "jacobi":
+        with pytest.raises(ValueError):
+            eigh_batched_device(torch.zeros((1, 97, 97), dtype=torch.complex128, device="cuda"),
+                                method=method)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_heev_zero_and_diagonal_matrices(method):
+    """An all-zero matrix (the Gram matrix of a block without P rows) gives
+    w = 0 and V = I; a diagonal one its sorted diagonal; both finite."""
+    import torch
+    from eigenfns_amd.linalg import eigh_batched_device
+    mats = np.zeros((3, 24, 24), np.complex128)
+    mats[1] = np.diag(np.arange(24, 0, -1.0))
+    mats[2] = _herm(np.random.default_rng(2), 24)
+    w, V = eigh_batched_device(torch.from_numpy(mats).cuda(), method=method)
+    w, V = w.cpu().numpy(), V.cpu().numpy()
+    assert np.isfinite(w).all() and np.isfinite(V).all()
+    assert np.array_equal(w[0], np.zeros(24))
+    assert np.abs(np.abs(V[0]) - np.eye(24)).max() < 1e-14 or np.abs(V[0].conj().T @ V[0] - np.eye(24)).max() < 1e-12
+    assert np.allclose(w[1], np.arange(1, 25.0), rtol=0, atol=1e-12)
+    _check(mats[2], w[2], V[2])
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_heev_decoupled_dead_block(method):
+    """A Rayleigh–Ritz matrix whose whole P block (rows 48-71) is dead: exactly
+    zero coupling to the rest and a 24-fold penalty eigenvalue.  Eigenpairs
+    finite and to the usual bars (the divide and conquer alone returned
+    non-finite vectors for this structure)."""
+    import torch
+    from eigenfns_amd.linalg import eigh_batched_device
+    rng = np.random.default_rng(11)
+    mats = np.stack([_rr_like(rng, 72, dead=range(48, 72)), _rr_like(rng, 72, dead=range(60, 72)),
+                     _rr_like(rng, 72)])
+    w, V = eigh_batched_device(torch.from_numpy(mats).cuda(), method=method)
+    w, V = w.cpu().numpy(), V.cpu().numpy()
+    assert np.isfinite(w).all() and np.isfinite(V).all()
+    for j in range(3):
+        wr = _check(mats[j], w[j], V[j])
+        assert np.abs(w[j][:24] - wr[:24]).max() < 1e-10 * np.abs(wr[:24]).min()

@@ -103,3 +103,26 @@ def test_serving_sample_matches_fp64_oracle(golden):
         err = _rel(req.vals, z[f"srv_{req.rid}"])
         print(f"request {req.rid}: nev {req.nev}, max rel err {err:.2e}")
         assert err < TOL
+
+
+def test_c5_requests_at_48_to_128_match_fp64_oracle(golden):
+    """One request per grid 48³-128³ of the C5 stream, served together (one
+    engine per grid, as the serving loop runs them), against the oracle's
+    FP64 LOBPCG eigenvalues (tests/golden/make_solve_golden.py, case c5)."""
+    from eigenfns_amd.serving import StructureCache, serve_batched
+    z, meta = golden
+    keys = sorted(k for k in meta if k.startswith("c5_"))
+    if not keys:
+        pytest.skip("no C5 requests in solve_golden.npz")
+    specs = wl.request_stream(2000, 0, rate=None)
+    want = {meta[k]["rid"]: k for k in keys}
+    subset = [sp for sp in specs if sp.rid in want]
+    assert len(subset) == len(keys)
+    done, _ = serve_batched(subset, StructureCache(), **wl.SERVING_SOLVER)
+    grids = set()
+    for req in done:
+        err = _rel(req.vals, z[want[req.rid]])
+        grids.add(req.meta["grid"])
+        print(f"C5 request {req.rid} at {req.meta['grid']}³: nev {req.nev}, max rel err {err:.2e}")
+        assert err < TOL
+    assert grids == {48, 64, 96, 128}
