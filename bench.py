@@ -402,7 +402,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": committed_traffic(G, m, op.chunk),
-                     "kernel": "fused Θ-apply (zin, ypass, xmid, ypass, zout)",
+                     "kernel": ("fused Θ-apply (xin, yzmid, xout)" if op.passes == 3 else "fused Θ-apply (zin, ypass, xmid, ypass, zout)"),
                      "alg_bytes_per_launch": int(alg_bytes),
                      "launch_ms": round(launch_s * 1e3, 4)},
         "step_ms_p10_p50_p90": [round(float(np.percentile(step_ms, q)), 4) for q in (10, 50, 90)],
