@@ -228,16 +228,24 @@ def run_c4cheb(outers: int | None = None, degree: int | None = None):
 
 
 def run_c5(n: int, rate: float | None, seed: int = 0, plain: bool = True, grids=None,
-           engine_voxels=None):
+           engine_voxels=None, start: int = 0, total: int | None = None):
+    """`n` requests of the seeded stream of `total` (default n) requests,
+    starting at request `start`; arrival times keep their spacing, shifted so
+    the slice's first request arrives at t = 0."""
     from .serving import StructureCache, serve_batched, serve_plain
-    specs = wl.request_stream(n, seed, rate=rate, grids=grids or wl.C5["grids"])
+    specs = wl.request_stream(max(total or n, start + n), seed, rate=rate,
+                              grids=grids or wl.C5["grids"])[start:start + n]
+    t_first = specs[0].t_arrive
+    for sp in specs:
+        sp.t_arrive -= t_first
     cache = StructureCache()
     for sp in specs:
         cache.get(sp.n_points, sp.grid)
     res, summ = serve_batched(specs, cache, max_slots=32, progress=True,
                               engine_voxels=engine_voxels)
-    out = {"case": "C5", "requests": n, "rate": rate, "grids": list(grids or wl.C5["grids"]),
-           "engine_voxels": engine_voxels, "batched": summ}
+    out = {"case": "C5", "requests": n, "first_request": start, "stream_length": max(total or n, start + n),
+           "rate": rate, "grids": list(grids or wl.C5["grids"]), "engine_voxels": engine_voxels,
+           "batched": summ}
     if plain:
         pres, psumm = serve_plain(specs, cache, progress=True)
         by = {r.rid: r.vals for r in res}
@@ -258,6 +266,8 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=200)
     ap.add_argument("--rate", type=float, default=None)
     ap.add_argument("--grids", default=None)
+    ap.add_argument("--start", type=int, default=0, help="C5: first request of the slice")
+    ap.add_argument("--total", type=int, default=None, help="C5: length of the seeded stream")
     ap.add_argument("--engine-voxels", type=float, default=None,
                     help="C5: cap slots x G^3 per engine (latency setting)")
     args = ap.parse_args(argv)
@@ -274,7 +284,7 @@ def main(argv=None):
     else:
         grids = tuple(int(g) for g in args.grids.split(",")) if args.grids else None
         out = run_c5(args.n, args.rate, plain=not args.no_plain, grids=grids,
-                     engine_voxels=args.engine_voxels)
+                     engine_voxels=args.engine_voxels, start=args.start, total=args.total)
     print(json.dumps(out, default=float), flush=True)
 
 
