@@ -221,5 +221,7 @@ def test_combine_split_into_own_rows(engine):
         C = torch.zeros((1, 96, 96), dtype=torch.complex64, device="cuda")
         with pytest.raises(ValueError):
             combine_multi_split(C, [S], [S[:48]], [S], 48)
+        with pytest.raises(ValueError):       # an output starting inside S, not at S
+            combine_multi_split(C, [S], [S[:48]], [S[48:]], 48)
     finally:
         combine_engine(prev)
