@@ -324,11 +324,17 @@ def two_stage(op, lam_lo: float, lam_hi: float, lam_max: float, m: int = 104,
             inwin = np.isfinite(lam) & (lam >= lam_lo) & (lam <= lam_hi)
             conv = inwin & (rn < tol)
             rec = {"stage": stage, "outer": outer, "in_window": int(inwin.sum()),
-                   "converged": int(conv.sum()), "outer_seconds": time.perf_counter() - t0}
+                   "converged": int(conv.sum()),
+                   "med_res_inwin": float(np.median(rn[inwin])) if inwin.any() else None,
+                   "max_res_inwin": float(np.max(rn[inwin])) if inwin.any() else None,
+                   "outer_seconds": time.perf_counter() - t0}
             history.append(rec)
             if verbose:
+                med = rec["med_res_inwin"]
                 print(f"  {stage} outer {outer}: in-window {rec['in_window']:3d} "
-                      f"converged {rec['converged']:3d} [{rec['outer_seconds']:.1f}s]", flush=True)
+                      f"converged {rec['converged']:3d} med-res "
+                      f"{med if med is not None else float('nan'):.1e} "
+                      f"[{rec['outer_seconds']:.1f}s]", flush=True)
             if state_path is not None:
                 save_interior_state(state_path, X, stage, outer, n_theta, lam_max, history)
             if stage == "polish" and inwin.any() and conv.sum() == inwin.sum():
@@ -341,5 +347,7 @@ def two_stage(op, lam_lo: float, lam_hi: float, lam_max: float, m: int = 104,
     inwin = np.isfinite(lam) & (lam >= lam_lo) & (lam <= lam_hi)
     conv = inwin & (rn < tol)
     idx = np.where(conv)[0][np.argsort(lam[conv], kind="stable")]
+    win = np.where(inwin)[0][np.argsort(lam[inwin], kind="stable")]
     return lam[idx], X[torch.as_tensor(idx, device=X.device)], rn[idx], {
-        "history": history, "theta_applications": n_theta, "lam_max": lam_max}
+        "history": history, "theta_applications": n_theta, "lam_max": lam_max,
+        "window_vals": lam[win], "window_res": rn[win]}

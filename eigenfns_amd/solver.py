@@ -242,15 +242,16 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                 XW = S[:2 * m]
                 for _ in range(2):
                     combine_into(-gram(XW, P), XW, P, X=P, alpha=1.0)
+                # a P with no direction above the drop threshold becomes zero
+                # rows, which the dead-row penalty below removes from the
+                # Rayleigh–Ritz: the reference's [X, W] basis without a host
+                # read of the kept count
                 wp, Vp = eigh(gram_herm(P))
                 keepp = wp > SVQB_DROP * torch.clamp(wp.max(), min=1e-300)
-                if not bool(keepp.any()):
-                    have_p = False
-                else:
-                    scale = torch.where(keepp, 1.0 / torch.sqrt(torch.where(keepp, wp,
-                                                                            torch.ones_like(wp))),
-                                        torch.zeros_like(wp))
-                    _combine_back(Vp * scale[None, :], P, T1)
+                scale = torch.where(keepp, 1.0 / torch.sqrt(torch.where(keepp, wp,
+                                                                        torch.ones_like(wp))),
+                                    torch.zeros_like(wp))
+                _combine_back(Vp * scale[None, :], P, T1)
             nb = (3 if have_p else 2) * m
             Sb, HSb = S[:nb], HS[:nb]
             op.theta_into(Sb[m:], HSb[m:])
@@ -258,7 +259,8 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             A = gram_herm(Sb, HSb)
             A = (A + A.conj().T) / 2
             dead = rownorm(Sb) < 0.5
-            penalty = max(RR_DEAD_PENALTY, 100.0 * float(torch.diagonal(A).abs().max().item()))
+            # max(1e4, 100·max|diag A|) (solver.py:299), kept on the device
+            penalty = torch.clamp(100.0 * torch.diagonal(A).abs().max(), min=RR_DEAD_PENALTY)
             A = A + torch.diag(torch.where(dead, penalty * dead_spread(nb, A.device), 0.0).to(A.dtype))
             wA, VA = eigh(A)
             # Xn = S·c/‖S·c‖, Pn = S·c'/‖S·c'‖ (c' = c with its X rows zeroed;

@@ -60,6 +60,8 @@ def test_c4_two_stage_protocol_with_fp64_residuals():
                "history": info["history"], "in_window_converged": len(vals),
                "vals": [float(v) for v in vals], "rn_device": [float(r) for r in rn],
                "fp64_residual": res64, "fp64_rel_lambda_diff": dlam,
+               "window_vals": [float(v) for v in info["window_vals"]],
+               "window_res_device": [float(r) for r in info["window_res"]],
                "theta_applications": info.get("theta_applications")}
     line = json.dumps(summary, default=float)
     print(line, flush=True)
