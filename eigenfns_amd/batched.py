@@ -31,7 +31,8 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .linalg import (COMBINE_IN_PLACE_MAX, DEAD_SPREAD, RR_DEAD_PENALTY, SVQB_DROP, combine_multi,
+from .linalg import (COMBINE_IN_PLACE_MAX, DEAD_SPREAD, ORTHO_METHODS, RR_DEAD_PENALTY, SVQB_DROP,
+                     cholqr_coeffs_batched, combine_multi,
                      combine_multi_split, copy_multi, dead_spread, eigh_batched_device, eigh_stack_host,
                      gram_multi, one_blas_thread, rowreduce_strided, row_axpby_norm_strided,
                      rowstats_strided, MAX_PROBLEMS)
@@ -91,7 +92,7 @@ class BatchedLOBPCG:
                  max_nev: int = 64, tol: float = 1e-4, maxit: int = 400,
                  keep_vectors: bool = False, normalized_rq: bool = True,
                  verify_dead_rows: bool = False, small_dense: str = "host",
-                 stagnation: int = STAGNATION_WINDOW):
+                 stagnation: int = STAGNATION_WINDOW, ortho: str = "svqb"):
         if not 0 < guard < m:
             raise ValueError("need 0 < guard < m")
         if not 1 <= nslots <= MAX_PROBLEMS:
@@ -117,6 +118,12 @@ class BatchedLOBPCG:
         if small_dense not in ("device", "host", "hybrid"):
             raise ValueError(f"small_dense must be 'device', 'host' or 'hybrid', got {small_dense!r}")
         self.small_dense = small_dense
+        # the SVQB steps and the P whitening by eigendecomposition ("svqb", the
+        # reference's) or by pivoted Cholesky-QR on the device ("cholqr": no
+        # host round trip for them in any small_dense mode)
+        if ortho not in ORTHO_METHODS:
+            raise ValueError(f"ortho must be one of {ORTHO_METHODS}, got {ortho!r}")
+        self.ortho = ortho
         per = self.m - self.guard
         self.cap = int(np.ceil(max_nev / per)) * per + self.m
         self.max_nev = int(max_nev)
@@ -235,7 +242,12 @@ class BatchedLOBPCG:
     def _whiten_small(self, G):
         """(V·w^{-1/2} with dropped directions zeroed, keep mask) of a Gram stack:
         on the host in NumPy when the eigensolve is there (one copy of the
-        coefficients back; masks as NumPy), on the device otherwise."""
+        coefficients back; masks as NumPy), on the device otherwise.  With
+        Cholesky-QR the coefficients and mask (rows below the rank) come from
+        the device kernel in every mode."""
+        if self.ortho == "cholqr":
+            C, rank = cholqr_coeffs_batched(G)
+            return C, torch.arange(G.shape[1], device=G.device)[None, :] < rank[:, None]
         if self.small_dense == "host":
             w, V = eigh_stack_host(G.cpu().numpy())
             wmax = w.max(axis=1, keepdims=True)
@@ -486,11 +498,12 @@ class BatchedLOBPCG:
 
 
 def solve_batched(requests, grid_size, m=24, guard=8, nslots=8, max_nev=None, tol=1e-4,
-                  maxit=400, keep_vectors=False, verify_dead_rows=False, small_dense="host"):
+                  maxit=400, keep_vectors=False, verify_dead_rows=False, small_dense="host",
+                  ortho="svqb"):
     """Solve a list of same-grid requests through one engine (admission as slots free)."""
     max_nev = max_nev or max(r.nev for r in requests)
     eng = BatchedLOBPCG(grid_size, m, guard, nslots, max_nev, tol, maxit, keep_vectors,
-                        verify_dead_rows=verify_dead_rows, small_dense=small_dense)
+                        verify_dead_rows=verify_dead_rows, small_dense=small_dense, ortho=ortho)
     pending = list(requests)
     done = []
     with one_blas_thread():

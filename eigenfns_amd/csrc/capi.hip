@@ -16,6 +16,7 @@
 #include "../../include/eigenfns_hip.h"
 #include "blas_kernels.hpp"
 #include "eig_kernels.hpp"
+#include "cholqr_kernels.hpp"
 #include "operator_kernels.hpp"
 #include "raster_kernels.hpp"
 #include "theta3_kernels.hpp"
@@ -1108,6 +1109,25 @@ int efn_heev_batched(int nprob, int n, const void* A, double* w, void* V, int ma
   }
   hipLaunchKernelGGL(efn::heev_jacobi_kernel, dim3((unsigned)nprob), dim3(efn::kEigThreads), lds,
                      (hipStream_t)stream, (const double2*)A, n, max_sweeps, w, (double2*)V, (const int*)nullptr);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+int efn_cholqr_batched(int nprob, int n, const void* G, void* C, double drop, int* rank, void* stream) {
+  if (nprob < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
+  if (n > efn::kCholMaxN) return fail(EFN_EINVAL, "n must be <= " + std::to_string(efn::kCholMaxN));
+  if (!(drop >= 0.0 && drop < 1.0)) return fail(EFN_EINVAL, "drop must be in [0, 1)");
+  if (nprob == 0 || n == 0) return EFN_OK;
+  if (!G || !C || !rank) return fail(EFN_EINVAL, "NULL pointer");
+  if (G == C) return fail(EFN_EINVAL, "G and C must not alias");
+  const size_t lds = (size_t)n * (n + 1) * sizeof(double2) + (size_t)(n + 2) * sizeof(int);
+  if (lds > 65536) {
+    const hipError_t e = hipFuncSetAttribute((const void*)efn::cholqr_coeffs_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return fail(EFN_EHIP, std::string("cholqr LDS size: ") + hipGetErrorString(e));
+  }
+  hipLaunchKernelGGL(efn::cholqr_coeffs_kernel, dim3((unsigned)nprob), dim3(efn::kCholThreads), lds,
+                     (hipStream_t)stream, (const double2*)G, n, drop, (double2*)C, rank);
   EFN_LAUNCH_CHECK();
   return EFN_OK;
 }

@@ -252,13 +252,49 @@ def deflate(X, locked, chunk_rows: int = DEFLATE_CHUNK, n_locked: int | None = N
     return out
 
 
-def ortho_block(X, passes: int = 2, drop: float = SVQB_DROP):
-    """SVQB orthonormalisation, FP64 Gram algebra, `passes` sweeps (solver.py:113-126).
+ORTHO_METHODS = ("svqb", "cholqr")
+
+
+def cholqr_coeffs_batched(Gs: torch.Tensor, drop: float = SVQB_DROP):
+    """(C, rank) for a (nprob, k, k) stack of Hermitian PSD Gram matrices, on
+    the device (efn_cholqr_batched, FP64): C = Π·L⁻ᴴ of a pivoted Cholesky
+    stopped at pivots ≤ drop·max(diag G), zero columns past the rank, so
+    Cᴴ·G·C = diag(I_rank, 0); rank stays on the device (no host sync)."""
+    Gs = Gs.to(torch.complex128).contiguous()
+    nprob, k = int(Gs.shape[0]), int(Gs.shape[1])
+    C = torch.empty_like(Gs)
+    rank = torch.empty(nprob, dtype=torch.int32, device=Gs.device)
+    _lib.check(_lib.load().efn_cholqr_batched(nprob, k, Gs.data_ptr(), C.data_ptr(), float(drop),
+                                              rank.data_ptr(), stream_ptr()), "cholqr_batched")
+    return C, rank
+
+
+def cholqr_coeffs(Gm: torch.Tensor, drop: float = SVQB_DROP) -> torch.Tensor:
+    """Cholesky-QR coefficients of one Gram matrix: the SVQB step's C
+    (Cᴴ·G·C = I on the kept rows, dropped rows zero) from a pivoted FP64
+    Cholesky on the device instead of an eigendecomposition."""
+    Gm = Gm.to(torch.complex128)
+    return cholqr_coeffs_batched(((Gm + Gm.conj().T) / 2)[None], drop)[0][0]
+
+
+def ortho_coeffs(Gm: torch.Tensor, drop: float = SVQB_DROP, method: str = "svqb") -> torch.Tensor:
+    """Orthonormalising coefficients of a Gram matrix by `method` ("svqb":
+    solver.py:119-124; "cholqr": pivoted Cholesky)."""
+    if method == "svqb":
+        return svqb_coeffs(Gm, drop)
+    if method == "cholqr":
+        return cholqr_coeffs(Gm, drop)
+    raise ValueError(f"ortho method must be one of {ORTHO_METHODS}, got {method!r}")
+
+
+def ortho_block(X, passes: int = 2, drop: float = SVQB_DROP, method: str = "svqb"):
+    """SVQB orthonormalisation, FP64 Gram algebra, `passes` sweeps (solver.py:113-126);
+    method="cholqr" uses Cholesky-QR coefficients instead (same passes and drop).
 
     Rank-dropped directions come back as zero rows."""
     X = as_block(X)
     for _ in range(passes):
-        Cw = svqb_coeffs(gram_herm(X), drop)
+        Cw = ortho_coeffs(gram_herm(X), drop, method)
         Y = torch.empty_like(X)
         combine_into(Cw, X, Y)
         X = Y

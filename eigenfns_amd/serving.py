@@ -101,7 +101,8 @@ def _summary(done, wall, turns, engines):
 
 def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slots=32,
                   hbm_budget=200e9, verbose=False, threads=True, progress=False,
-                  engines_per_grid=2, engine_voxels=None, small_dense="host", max_nev=None):
+                  engines_per_grid=2, engine_voxels=None, small_dense="host", max_nev=None,
+                  ortho="svqb"):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
 
     threads=True: each grid's engine runs in its own host thread on its own HIP
@@ -117,7 +118,8 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     Slots are sized for the service's band limit `max_nev` (default: the
     request stream configuration's upper bound, 64), not for the requests
     seen so far; a request asking for more is rejected (ValueError) before
-    anything runs."""
+    anything runs.  `small_dense` and `ortho` choose where the engines' small
+    FP64 eigensolves run and how blocks are orthonormalised (BatchedLOBPCG)."""
     cache = cache or StructureCache()
     grids = sorted({sp.grid for sp in specs})
     from .workloads import C5
@@ -129,7 +131,7 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     if not threads:
         with one_blas_thread():
             return _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose,
-                                     small_dense)
+                                     small_dense, ortho)
     import threading
     import torch
     queues = {g: sorted([sp for sp in specs if sp.grid == g], key=lambda s: s.t_arrive)
@@ -147,7 +149,7 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
                 pending = queues[g]
                 nslots = max(1, min(-(-plan[g] // k), -(-len(pending) // k)))
                 eng = BatchedLOBPCG(g, m, guard, nslots, max_nev, tol, maxit,
-                                    small_dense=small_dense)
+                                    small_dense=small_dense, ortho=ortho)
                 with lock:
                     engines[str(g) if k == 1 else f"{g}.{e}"] = eng
                 while True:
@@ -196,7 +198,7 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
 
 
 def _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose,
-                      small_dense="host"):
+                      small_dense="host", ortho="svqb"):
     """One host thread, one stream: every loop turn steps each engine in turn."""
     engines = {}
     pending = sorted(specs, key=lambda s: s.t_arrive)
@@ -213,7 +215,8 @@ def _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose
             eng = engines.get(sp.grid)
             if eng is None:
                 eng = engines[sp.grid] = BatchedLOBPCG(sp.grid, m, guard, plan[sp.grid], max_nev,
-                                                       tol, maxit, small_dense=small_dense)
+                                                       tol, maxit, small_dense=small_dense,
+                                                       ortho=ortho)
             if eng.free_slots():
                 req = Request(_make_op(cache, sp, workspace=False), sp.nev, seed=sp.seed,
                               rid=sp.rid, meta={"grid": sp.grid, "n_points": sp.n_points})
@@ -277,13 +280,14 @@ if __name__ == "__main__":
     ap.add_argument("--sequential", action="store_true", help="one host thread, one stream")
     ap.add_argument("--engines-per-grid", type=int, default=2)
     ap.add_argument("--small-dense", choices=["device", "host", "hybrid"], default="host")
+    ap.add_argument("--ortho", choices=["svqb", "cholqr"], default="svqb")
     args = ap.parse_args()
     grids = tuple(int(g) for g in args.grids.split(","))
     specs = request_stream(args.n, args.seed, grids=grids, nev_range=(8, args.nev_max))
     cache = StructureCache()
     res, summ = serve_batched(specs, cache, max_slots=args.slots, verbose=True,
                               threads=not args.sequential, engines_per_grid=args.engines_per_grid,
-                              small_dense=args.small_dense)
+                              small_dense=args.small_dense, ortho=args.ortho)
     print(json.dumps({"batched": summ}), flush=True)
     if args.plain:
         pres, psumm = serve_plain(specs, cache)

@@ -15,6 +15,7 @@ from __future__ import annotations
 import argparse
 import contextlib
 import json
+import os
 import sys
 import time
 from pathlib import Path
@@ -53,41 +54,44 @@ def _sync_time(fn):
     return out, time.perf_counter() - t0
 
 
-def run_c1():
+def run_c1(ortho: str = "svqb"):
     from .operator import MaxwellOperator
     from .solver import lobpcg_blocks
     c = wl.C1
     eps, L = wl.network(c["n_points"], c["grid"])
     op = MaxwellOperator(eps, L)
-    lobpcg_blocks(op, 4, m=8, guard=4, tol=1e-3, verbose=False)  # warm (library, allocator)
+    lobpcg_blocks(op, 4, m=8, guard=4, tol=1e-3, verbose=False, ortho=ortho)  # warm
     (vals, _, st), t = _sync_time(lambda: lobpcg_blocks(op, c["nev"], m=c["m"], guard=c["guard"],
-                                                        tol=c["tol"], seed=0, verbose=False))
+                                                        tol=c["tol"], seed=0, verbose=False,
+                                                        ortho=ortho))
     return {"case": "C1", "grid": c["grid"], "nev": c["nev"], "m": c["m"], "guard": c["guard"],
-            "tol": c["tol"], "gpu_seconds": round(t, 3), "iterations": st.iterations,
+            "tol": c["tol"], "ortho": ortho, "small_dense": os.environ.get("EFN_SMALL_DENSE", "host"),
+            "gpu_seconds": round(t, 3), "iterations": st.iterations,
             "theta_applications": st.theta_applications,
             "blocks": [{k: r[k] for k in ("iters", "max_res_locked", "converged", "stagnated")}
                        for r in st.rounds],
             "max_rel_err_vs_fp64_oracle": _rel_err(vals, _golden("c1")), "vals": vals.tolist()}
 
 
-def run_c2():
+def run_c2(ortho: str = "svqb"):
     from .operator import MaxwellOperator
     from .solver import lobpcg_blocks
     c = wl.C2
     res = {"case": "C2", "grid": c["grid"], "nev": c["nev"], "m": c["m"], "guard": c["guard"],
-           "tol": c["tol"]}
+           "tol": c["tol"], "ortho": ortho, "small_dense": os.environ.get("EFN_SMALL_DENSE", "host")}
     # parity: the same network at 16³ against the dense FP64 solve
     eps16, L = wl.network(c["n_points"], c["dense_grid"])
     op16 = MaxwellOperator(eps16, L)
     vals16, _, _ = lobpcg_blocks(op16, c["nev"], m=c["m"], guard=c["guard"], tol=c["tol"],
-                                 verbose=False)
+                                 verbose=False, ortho=ortho)
     res["dense16_max_rel_err"] = _rel_err(vals16, _golden("c2dense"))
     del op16
     eps, L = wl.network(c["n_points"], c["grid"])
     op = MaxwellOperator(eps, L)
     with contextlib.redirect_stdout(sys.stderr):    # per-block progress lines
         (vals, _, st), t = _sync_time(lambda: lobpcg_blocks(
-            op, c["nev"], m=c["m"], guard=c["guard"], tol=c["tol"], seed=0, verbose=True))
+            op, c["nev"], m=c["m"], guard=c["guard"], tol=c["tol"], seed=0, verbose=True,
+            ortho=ortho))
     res.update({"gpu_seconds": round(t, 2), "iterations": st.iterations,
                 "theta_applications": st.theta_applications,
                 "lambda_1_128": [float(vals[0]), float(vals[-1])]})
@@ -267,14 +271,16 @@ def main(argv=None):
     ap.add_argument("--rate", type=float, default=None)
     ap.add_argument("--grids", default=None)
     ap.add_argument("--start", type=int, default=0, help="C5: first request of the slice")
+    ap.add_argument("--ortho", default="svqb", choices=["svqb", "cholqr"],
+                    help="C1/C2: orthonormalisation of the LOBPCG blocks")
     ap.add_argument("--total", type=int, default=None, help="C5: length of the seeded stream")
     ap.add_argument("--engine-voxels", type=float, default=None,
                     help="C5: cap slots x G^3 per engine (latency setting)")
     args = ap.parse_args(argv)
     if args.case == "c1":
-        out = run_c1()
+        out = run_c1(args.ortho)
     elif args.case == "c2":
-        out = run_c2()
+        out = run_c2(args.ortho)
     elif args.case == "c3":
         out = run_c3(args.n_k, plain=not args.no_plain, bucket_step=args.bucket_step)
     elif args.case == "c4kpm":

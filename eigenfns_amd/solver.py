@@ -35,9 +35,9 @@ import torch
 from .linalg import (COMBINE_IN_PLACE_MAX, DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_DROP, apply_chunked,
                      combine, combine_into, combine_multi_split, dead_spread, deflate, deflate_chunk_rows,
                      eigh,
-                     gram, gram_herm, normalize_rows, ortho_block, row_axpby,
+                     gram, gram_herm, normalize_rows, ortho_block, ortho_coeffs, row_axpby,
                      row_axpby_norm_strided, rowdot, rownorm, rownorm2, rowstats_strided,
-                     svqb_coeffs)
+                     svqb_coeffs, ORTHO_METHODS)
 
 __all__ = ["gram", "combine", "deflate", "ortho_block", "apply_chunked", "deflate_chunk_rows",
            "SolveStats", "StagnationLock", "lobpcg_blocks", "lobpcg_blocks_resumable", "_flat",
@@ -105,11 +105,12 @@ def _deflate_inplace(Y, L, nl):
     return Y
 
 
-def _svqb_into(Y, tmp, drop=SVQB_DROP, passes=2):
+def _svqb_into(Y, tmp, drop=SVQB_DROP, passes=2, method="svqb"):
     """SVQB of Y (two sweeps), result written back into Y; tmp has Y's shape
-    (unused when the combine can run in place)."""
+    (unused when the combine can run in place).  method="cholqr": Cholesky-QR
+    coefficients in place of the eigendecomposition."""
     for _ in range(passes):
-        Cw = svqb_coeffs(gram_herm(Y), drop)
+        Cw = ortho_coeffs(gram_herm(Y), drop, method)
         _combine_back(Cw, Y, tmp)
     return Y
 
@@ -128,7 +129,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                   verbose: bool = True, log_every: int = 0, initial_locked=None,
                   initial_vals=None, on_block=None, locked_storage: str = "auto",
                   theta_chunk: int = 0, normalized_rq: bool = True,
-                  stagnation: int = STAGNATION_WINDOW):
+                  stagnation: int = STAGNATION_WINDOW, ortho: str = "svqb"):
     """Bottom-up deflated block LOBPCG.  Returns (eigenvalues, locked_vectors, stats).
 
     Locks the lowest (m − guard) Ritz pairs of each block once their relative
@@ -138,9 +139,14 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
     `initial_locked`/`initial_vals` resume a partial solve;
     `on_block(n_locked_total, lam_block, X_block)` fires after each lock.
     `locked_storage` and `theta_chunk` are accepted for API compatibility: the
-    locked set is always HBM-resident and Θ is chunked by the operator."""
+    locked set is always HBM-resident and Θ is chunked by the operator.
+    `ortho`: "svqb" (the reference's orthonormalisation) or "cholqr" (pivoted
+    FP64 Cholesky-QR on the device for the SVQB steps and the P whitening, no
+    host round trip; the Rayleigh–Ritz eigensolve is unchanged)."""
     if m <= guard:
         raise ValueError("m must exceed guard")
+    if ortho not in ORTHO_METHODS:
+        raise ValueError(f"ortho must be one of {ORTHO_METHODS}, got {ortho!r}")
     G = op.grid_size
     shape = (2, G, G, G)
     dev = op.random_block(0, seed).device
@@ -195,7 +201,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             normalize_rows(Z, out=X[nc:])
         block_idx += 1
         _deflate_inplace(X, L, nl)
-        _svqb_into(X, T1)
+        _svqb_into(X, T1, method=ortho)
         op.theta_into(X, HX)
         stats.theta_applications += m
         have_p = False
@@ -236,7 +242,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             _deflate_inplace(W, L, nl)
             for _ in range(2):
                 combine_into(-gram(X, W), X, W, X=W, alpha=1.0)
-            _svqb_into(W, T1)
+            _svqb_into(W, T1, method=ortho)
             # P projected out of [X, W] twice, fixed-shape whitening (solver.py:258-274)
             if have_p:
                 XW = S[:2 * m]
@@ -246,12 +252,15 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                 # rows, which the dead-row penalty below removes from the
                 # Rayleigh–Ritz: the reference's [X, W] basis without a host
                 # read of the kept count
-                wp, Vp = eigh(gram_herm(P))
-                keepp = wp > SVQB_DROP * torch.clamp(wp.max(), min=1e-300)
-                scale = torch.where(keepp, 1.0 / torch.sqrt(torch.where(keepp, wp,
-                                                                        torch.ones_like(wp))),
-                                    torch.zeros_like(wp))
-                _combine_back(Vp * scale[None, :], P, T1)
+                if ortho == "cholqr":
+                    _combine_back(ortho_coeffs(gram_herm(P), SVQB_DROP, ortho), P, T1)
+                else:
+                    wp, Vp = eigh(gram_herm(P))
+                    keepp = wp > SVQB_DROP * torch.clamp(wp.max(), min=1e-300)
+                    scale = torch.where(keepp, 1.0 / torch.sqrt(torch.where(keepp, wp,
+                                                                            torch.ones_like(wp))),
+                                        torch.zeros_like(wp))
+                    _combine_back(Vp * scale[None, :], P, T1)
             nb = (3 if have_p else 2) * m
             Sb, HSb = S[:nb], HS[:nb]
             op.theta_into(Sb[m:], HSb[m:])

@@ -219,6 +219,16 @@ int efn_combine(const void* C_c64, int m_in, int m_out, const void* S_c64, int64
 int efn_heev_batched(int nprob, int n, const void* A_c128, double* w, void* V_c128, int max_sweeps,
                      void* stream);
 
+/* Cholesky-QR coefficients of nprob Hermitian positive semidefinite Gram
+ * matrices G (n × n complex128, row-major, n ≤ 96): pivoted Cholesky of G in
+ * FP64, stopped where the largest remaining pivot is ≤ drop·max(diag G)
+ * (rank[p] = pivots taken); C (n × n complex128) = Π·L⁻ᴴ on those pivots and
+ * zero elsewhere, so Cᴴ·G·C = diag(I_rank, 0) and the rows Σ_j C[j][i]·y_j
+ * are orthonormal.  The orthonormalisation the reference does by SVQB
+ * (solver.py:113-126, :258-274) with a Cholesky factor in place of the
+ * eigendecomposition.  G and C must not alias. */
+int efn_cholqr_batched(int nprob, int n, const void* G, void* C, double drop, int* rank, void* stream);
+
 /* The same decomposition with rocSOLVER's batched divide and conquer
  * (rocsolver_zheevd_strided_batched, FP64): any n, same layouts (only A's
  * upper triangle is read), results to LAPACK accuracy; measured 1.55 ms for

@@ -1,0 +1,156 @@
+"""GPU tests of the Cholesky-QR coefficients (efn_cholqr_batched): for Gram
+matrices G = Yᴴ·Y of FP64 block rows, C must satisfy Cᴴ·G·C = diag(I_r, 0)
+with r the numerical rank at the drop threshold, so that the rows of Cᵀ·Y are
+orthonormal and span the rows of Y; and LOBPCG with ortho="cholqr" must give
+the same eigenvalues as with SVQB (dense FP64 spectrum at 8³, the FP64 oracle
+LOBPCG values of C1 at 32³).
+
+Bars: ‖CᴴGC − diag(I_r, 0)‖_max < 1e-9 for G with condition number ≤ 1e4
+(FP64 Cholesky: error ≈ κ·ε); eigenvalues within 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(rng, k, n, scales=None):
+    Y = rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
+    if scales is not None:
+        Y *= np.asarray(scales)[:, None]
+    return Y
+
+
+def _gram(Y):
+    return np.conj(Y) @ Y.T          # G[i][j] = ⟨y_i, y_j⟩
+
+
+def _run(Gs, drop=1e-5):
+    import torch
+    from eigenfns_amd.linalg import cholqr_coeffs_batched
+    C, r = cholqr_coeffs_batched(torch.from_numpy(np.ascontiguousarray(Gs)).cuda(), drop)
+    return C.cpu().numpy(), r.cpu().numpy()
+
+
+def _check(G, C, r, tol=1e-9):
+    k = G.shape[0]
+    want = np.zeros((k, k))
+    want[:r, :r] = np.eye(r)
+    scale = max(1.0, np.abs(np.diag(G)).max())
+    assert np.abs(C.conj().T @ G @ C - want).max() < tol, (k, r)
+    if r < k:
+        assert np.abs(C[:, r:]).max() == 0.0
+    # at most r rows of C are nonzero: the pivots taken
+    assert int((np.abs(C).max(axis=1) > 0).sum()) == r
+    return scale
+
+
+@pytest.mark.parametrize("k", [1, 2, 7, 24, 48, 72, 96])
+def test_cholqr_orthonormalises_full_rank_blocks(k):
+    """Rows of very different norms (1e-2 … 1): every row kept, Cᴴ·G·C = I,
+    and Q = Cᵀ·Y has orthonormal rows spanning Y's (Y = (QQᴴ)·Y)."""
+    rng = np.random.default_rng(k)
+    Ys = [_rows(rng, k, 4 * k + 8, np.logspace(-2, 0, k)[rng.permutation(k)]) for _ in range(4)]
+    Gs = np.stack([_gram(Y) for Y in Ys])
+    C, r = _run(Gs)
+    for j, Y in enumerate(Ys):
+        assert r[j] == k
+        _check(Gs[j], C[j], k)
+        Q = C[j].T @ Y
+        assert np.abs(np.conj(Q) @ Q.T - np.eye(k)).max() < 1e-9
+        proj = Q.T @ (np.conj(Q) @ Y.T)          # columns of Yᵀ projected on span(Q rows)
+        assert np.abs(proj - Y.T).max() < 1e-9 * np.abs(Y).max()
+
+
+def test_cholqr_drops_dependent_and_tiny_rows():
+    """Rows that are combinations of others, and rows below the drop threshold
+    (norm² 1e-8 of the largest against drop 1e-5), come back as zero columns
+    of C; rows at norm² 1e-4 are kept."""
+    rng = np.random.default_rng(5)
+    k, n = 32, 200
+    Y = _rows(rng, k, n)
+    Y[5] = 0.5 * Y[1] - 2.0 * Y[3]                 # dependent
+    Y[17] = Y[9]                                    # duplicate
+    Y[20] *= 1e-4                                   # below the threshold
+    Y[21] *= 1e-2                                   # above it
+    G = _gram(Y)
+    C, r = _run(G[None])
+    assert r[0] == k - 3
+    _check(G, C[0], k - 3)
+    kept_rows = np.abs(C[0]).max(axis=1) > 0
+    assert kept_rows[21] and not kept_rows[20]
+    # the kept rows' span covers every row of Y up to the dropped magnitudes
+    Q = C[0].T @ Y
+    resid = Y.T - Q.T @ (np.conj(Q) @ Y.T)
+    assert np.abs(resid).max() < 1e-3 * np.abs(Y).max()
+
+
+def test_cholqr_zero_nan_and_batch_invariance():
+    """A zero Gram matrix has rank 0 and C = 0; a NaN input gives rank 0
+    (no NaN in C); a problem's C is bitwise the same alone and in a batch."""
+    rng = np.random.default_rng(9)
+    k = 24
+    G1 = _gram(_rows(rng, k, 80))
+    Z = np.zeros((k, k), np.complex128)
+    N = G1.copy()
+    N[3, 3] = np.nan
+    C, r = _run(np.stack([G1, Z, N]))
+    assert list(r) == [k, 0, 0]
+    assert not np.isnan(C).any()
+    assert np.abs(C[1]).max() == 0.0 and np.abs(C[2]).max() == 0.0
+    C1, _ = _run(G1[None])
+    assert np.array_equal(C1[0], C[0])
+
+
+def test_cholqr_limits():
+    import torch
+    from eigenfns_amd.linalg import cholqr_coeffs_batched
+    G = torch.zeros((1, 97, 97), dtype=torch.complex128, device="cuda")
+    with pytest.raises(ValueError):
+        cholqr_coeffs_batched(G)
+    with pytest.raises(ValueError):
+        cholqr_coeffs_batched(G[:, :8, :8].contiguous(), drop=1.5)
+
+
+def test_lobpcg_cholqr_matches_dense_fp64_at_8():
+    """Deflated block LOBPCG with Cholesky-QR orthonormalisation: the lowest
+    eigenvalues of the reference's 8³ test cell (tests/test_solver.py:22-24)
+    at Γ and of a Bloch point within 1e-6 of the dense FP64 spectrum — the
+    same bars as with SVQB."""
+    from eigenfns_amd.operator import MaxwellOperator
+    from eigenfns_amd.solver import lobpcg_blocks
+    from oracle.dense_np import dense_eigh
+    from oracle.operator_np import MaxwellOperatorNP
+    rng = np.random.default_rng(3)
+    eps = np.asarray(1.0 + 7.57 * (rng.random((8, 8, 8)) > 0.7), np.float32)
+    w, _ = dense_eigh(MaxwellOperatorNP(eps, 1.0))
+    vals, vecs, _ = lobpcg_blocks(MaxwellOperator(eps, 1.0), 48, m=24, guard=8, tol=1e-6,
+                                  maxit=400, ortho="cholqr", verbose=False)
+    assert (np.abs(vals - w[:48]) / w[:48]).max() < 1e-6
+    Vg = vecs.reshape(48, -1).cpu().numpy().astype(np.complex128)
+    assert np.abs(Vg.conj() @ Vg.T - np.eye(48)).max() < 5e-5
+    rng = np.random.default_rng(11)
+    eps = np.where(rng.random((8, 8, 8)) < 0.3, 8.41, 1.0).astype(np.float32)
+    k = (0.25, 0.1, 0.0)
+    w, _ = dense_eigh(MaxwellOperatorNP(eps, 1.3, k), drop_gamma=False)
+    vals, _, _ = lobpcg_blocks(MaxwellOperator(eps, 1.3, k), 20, m=16, guard=6, tol=1e-6,
+                               ortho="cholqr", verbose=False)
+    assert (np.abs(vals - w[:20]) / w[:20]).max() < 1e-6
+
+
+def test_lobpcg_cholqr_c1_matches_fp64_oracle():
+    """C1 (64 points, 32³, 24 bands) with ortho="cholqr" against the FP64
+    oracle LOBPCG eigenvalues in tests/golden/solve_golden.npz."""
+    import json
+    from pathlib import Path
+    from eigenfns_amd import workloads as wl
+    from eigenfns_amd.operator import MaxwellOperator
+    from eigenfns_amd.solver import lobpcg_blocks
+    z = np.load(Path(__file__).parent / "golden" / "solve_golden.npz")
+    c = wl.C1
+    eps, L = wl.network(c["n_points"], c["grid"])
+    op = MaxwellOperator(eps, L)
+    vals, _, _ = lobpcg_blocks(op, c["nev"], m=c["m"], guard=c["guard"], tol=c["tol"], seed=0,
+                               ortho="cholqr", verbose=False)
+    ref = z["c1"]
+    assert (np.abs(vals[:len(ref)] - ref) / ref).max() < 1e-6, json.dumps(list(vals[:4]))
