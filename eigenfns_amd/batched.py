@@ -92,7 +92,7 @@ class BatchedLOBPCG:
                  max_nev: int = 64, tol: float = 1e-4, maxit: int = 400,
                  keep_vectors: bool = False, normalized_rq: bool = True,
                  verify_dead_rows: bool = False, small_dense: str = "host",
-                 stagnation: int = STAGNATION_WINDOW, ortho: str = "svqb"):
+                 stagnation: int = STAGNATION_WINDOW, ortho: str = "cholqr"):
         if not 0 < guard < m:
             raise ValueError("need 0 < guard < m")
         if not 1 <= nslots <= MAX_PROBLEMS:
@@ -118,9 +118,9 @@ class BatchedLOBPCG:
         if small_dense not in ("device", "host", "hybrid"):
             raise ValueError(f"small_dense must be 'device', 'host' or 'hybrid', got {small_dense!r}")
         self.small_dense = small_dense
-        # the SVQB steps and the P whitening by eigendecomposition ("svqb", the
-        # reference's) or by pivoted Cholesky-QR on the device ("cholqr": no
-        # host round trip for them in any small_dense mode)
+        # the SVQB steps and the P whitening by pivoted Cholesky-QR on the
+        # device ("cholqr", the default: no host round trip for them in any
+        # small_dense mode) or by eigendecomposition ("svqb", the reference's)
         if ortho not in ORTHO_METHODS:
             raise ValueError(f"ortho must be one of {ORTHO_METHODS}, got {ortho!r}")
         self.ortho = ortho
@@ -499,7 +499,7 @@ class BatchedLOBPCG:
 
 def solve_batched(requests, grid_size, m=24, guard=8, nslots=8, max_nev=None, tol=1e-4,
                   maxit=400, keep_vectors=False, verify_dead_rows=False, small_dense="host",
-                  ortho="svqb"):
+                  ortho="cholqr"):
     """Solve a list of same-grid requests through one engine (admission as slots free)."""
     max_nev = max_nev or max(r.nev for r in requests)
     eng = BatchedLOBPCG(grid_size, m, guard, nslots, max_nev, tol, maxit, keep_vectors,

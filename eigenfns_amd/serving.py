@@ -102,7 +102,7 @@ def _summary(done, wall, turns, engines):
 def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slots=32,
                   hbm_budget=200e9, verbose=False, threads=True, progress=False,
                   engines_per_grid=2, engine_voxels=None, small_dense="host", max_nev=None,
-                  ortho="svqb"):
+                  ortho="cholqr"):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
 
     threads=True: each grid's engine runs in its own host thread on its own HIP
@@ -198,7 +198,7 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
 
 
 def _serve_sequential(specs, cache, m, guard, tol, maxit, plan, max_nev, verbose,
-                      small_dense="host", ortho="svqb"):
+                      small_dense="host", ortho="cholqr"):
     """One host thread, one stream: every loop turn steps each engine in turn."""
     engines = {}
     pending = sorted(specs, key=lambda s: s.t_arrive)
@@ -280,7 +280,7 @@ if __name__ == "__main__":
     ap.add_argument("--sequential", action="store_true", help="one host thread, one stream")
     ap.add_argument("--engines-per-grid", type=int, default=2)
     ap.add_argument("--small-dense", choices=["device", "host", "hybrid"], default="host")
-    ap.add_argument("--ortho", choices=["svqb", "cholqr"], default="svqb")
+    ap.add_argument("--ortho", choices=["svqb", "cholqr"], default="cholqr")
     args = ap.parse_args()
     grids = tuple(int(g) for g in args.grids.split(","))
     specs = request_stream(args.n, args.seed, grids=grids, nev_range=(8, args.nev_max))

@@ -54,7 +54,7 @@ def _sync_time(fn):
     return out, time.perf_counter() - t0
 
 
-def run_c1(ortho: str = "svqb"):
+def run_c1(ortho: str = "cholqr"):
     from .operator import MaxwellOperator
     from .solver import lobpcg_blocks
     c = wl.C1
@@ -73,7 +73,7 @@ def run_c1(ortho: str = "svqb"):
             "max_rel_err_vs_fp64_oracle": _rel_err(vals, _golden("c1")), "vals": vals.tolist()}
 
 
-def run_c2(ortho: str = "svqb"):
+def run_c2(ortho: str = "cholqr"):
     from .operator import MaxwellOperator
     from .solver import lobpcg_blocks
     c = wl.C2
@@ -271,7 +271,7 @@ def main(argv=None):
     ap.add_argument("--rate", type=float, default=None)
     ap.add_argument("--grids", default=None)
     ap.add_argument("--start", type=int, default=0, help="C5: first request of the slice")
-    ap.add_argument("--ortho", default="svqb", choices=["svqb", "cholqr"],
+    ap.add_argument("--ortho", default="cholqr", choices=["svqb", "cholqr"],
                     help="C1/C2: orthonormalisation of the LOBPCG blocks")
     ap.add_argument("--total", type=int, default=None, help="C5: length of the seeded stream")
     ap.add_argument("--engine-voxels", type=float, default=None,

@@ -129,7 +129,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                   verbose: bool = True, log_every: int = 0, initial_locked=None,
                   initial_vals=None, on_block=None, locked_storage: str = "auto",
                   theta_chunk: int = 0, normalized_rq: bool = True,
-                  stagnation: int = STAGNATION_WINDOW, ortho: str = "svqb"):
+                  stagnation: int = STAGNATION_WINDOW, ortho: str = "cholqr"):
     """Bottom-up deflated block LOBPCG.  Returns (eigenvalues, locked_vectors, stats).
 
     Locks the lowest (m − guard) Ritz pairs of each block once their relative
@@ -140,9 +140,10 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
     `on_block(n_locked_total, lam_block, X_block)` fires after each lock.
     `locked_storage` and `theta_chunk` are accepted for API compatibility: the
     locked set is always HBM-resident and Θ is chunked by the operator.
-    `ortho`: "svqb" (the reference's orthonormalisation) or "cholqr" (pivoted
-    FP64 Cholesky-QR on the device for the SVQB steps and the P whitening, no
-    host round trip; the Rayleigh–Ritz eigensolve is unchanged)."""
+    `ortho`: "cholqr" (default: pivoted FP64 Cholesky-QR on the device for the
+    SVQB steps and the P whitening, no host round trip; the Rayleigh–Ritz
+    eigensolve is unchanged) or "svqb" (the reference's eigendecomposition-based
+    orthonormalisation, solver.py:113-126)."""
     if m <= guard:
         raise ValueError("m must exceed guard")
     if ortho not in ORTHO_METHODS:

@@ -67,10 +67,11 @@ def test_theta_multi_equals_per_operator():
         assert torch.equal(Y[4 * i:4 * i + 4], op.precondition(X[4 * i:4 * i + 4]))
 
 
-def test_batched_mix_matches_dense(dense_refs):
+@pytest.mark.parametrize("ortho", ["cholqr", "svqb"])
+def test_batched_mix_matches_dense(dense_refs, ortho):
     from eigenfns_amd.batched import solve_batched
     done = solve_batched(_requests(), 8, m=16, guard=6, nslots=3, tol=1e-6, maxit=300,
-                         verify_dead_rows=True)
+                         verify_dead_rows=True, ortho=ortho)
     assert len(done) == len(PROBLEMS)
     for req in done:
         w = dense_refs[req.rid]
@@ -147,15 +148,15 @@ def test_device_small_eigensolves_match_dense(dense_refs, small_dense):
     batch-invariant too."""
     from eigenfns_amd.batched import solve_batched
     done = solve_batched(_requests(), 8, m=16, guard=6, nslots=3, tol=1e-6, maxit=300,
-                         verify_dead_rows=True, small_dense=small_dense)
+                         verify_dead_rows=True, small_dense=small_dense, ortho="svqb")
     assert len(done) == len(PROBLEMS)
     for req in done:
         w = dense_refs[req.rid]
         assert (np.abs(req.vals - w[:req.nev]) / w[:req.nev]).max() < 1e-6
     alone = solve_batched(_requests([2]), 8, m=16, guard=6, nslots=1, max_nev=24, tol=1e-5,
-                          small_dense=small_dense)
+                          small_dense=small_dense, ortho="svqb")
     mixed = solve_batched(_requests(), 8, m=16, guard=6, nslots=4, max_nev=24, tol=1e-5,
-                          small_dense=small_dense)
+                          small_dense=small_dense, ortho="svqb")
     assert np.array_equal(alone[0].vals, next(r for r in mixed if r.rid == 2).vals)
 
 

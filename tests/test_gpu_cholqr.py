@@ -138,9 +138,10 @@ def test_lobpcg_cholqr_matches_dense_fp64_at_8():
     assert (np.abs(vals - w[:20]) / w[:20]).max() < 1e-6
 
 
-def test_lobpcg_cholqr_c1_matches_fp64_oracle():
-    """C1 (64 points, 32³, 24 bands) with ortho="cholqr" against the FP64
-    oracle LOBPCG eigenvalues in tests/golden/solve_golden.npz."""
+@pytest.mark.parametrize("ortho", ["cholqr", "svqb"])
+def test_lobpcg_c1_both_orthonormalisations_match_fp64_oracle(ortho):
+    """C1 (64 points, 32³, 24 bands) with either orthonormalisation against
+    the FP64 oracle LOBPCG eigenvalues in tests/golden/solve_golden.npz."""
     import json
     from pathlib import Path
     from eigenfns_amd import workloads as wl
@@ -151,6 +152,6 @@ def test_lobpcg_cholqr_c1_matches_fp64_oracle():
     eps, L = wl.network(c["n_points"], c["grid"])
     op = MaxwellOperator(eps, L)
     vals, _, _ = lobpcg_blocks(op, c["nev"], m=c["m"], guard=c["guard"], tol=c["tol"], seed=0,
-                               ortho="cholqr", verbose=False)
+                               ortho=ortho, verbose=False)
     ref = z["c1"]
     assert (np.abs(vals[:len(ref)] - ref) / ref).max() < 1e-6, json.dumps(list(vals[:4]))

@@ -30,11 +30,13 @@ def g8():
     return MaxwellOperator(eps, 1.0), ref, w, V
 
 
-def test_lobpcg_matches_dense_fp64(g8):
+@pytest.mark.parametrize("ortho", ["cholqr", "svqb"])
+def test_lobpcg_matches_dense_fp64(g8, ortho):
     # reference tests/test_solver.py:36-48, tightened to the 1e-6 eigenvalue bar
     from eigenfns_amd.solver import lobpcg_blocks
     op, ref, w, V = g8
-    vals, vecs, st = lobpcg_blocks(op, 48, m=24, guard=8, tol=1e-6, maxit=400, verbose=False)
+    vals, vecs, st = lobpcg_blocks(op, 48, m=24, guard=8, tol=1e-6, maxit=400, verbose=False,
+                                   ortho=ortho)
     rel = np.abs(vals - w[:48]) / w[:48]
     print(f"max rel eigenvalue error {rel.max():.2e}, iterations {st.iterations}")
     assert rel.max() < 1e-6
