@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -19,7 +20,7 @@
 #include "cholqr_kernels.hpp"
 #include "operator_kernels.hpp"
 #include "raster_kernels.hpp"
-#include "theta3_kernels.hpp"
+#include "theta3s_kernels.hpp"
 
 
 namespace {
@@ -80,6 +81,7 @@ struct efn_ctx {
   float* d_eps = nullptr;        // ε/G³
   float* d_kn2 = nullptr;        // |k+G|² (for the diagonal preconditioner)
   float* d_xeps = nullptr;       // ε⁻¹/G³ then ε/G³ in the x pass's tile order
+  float* d_heps = nullptr;       // ε⁻¹/G³ then ε/G³ de-interleaved in z (half-plane grids)
   float2* d_twid = nullptr;      // exp(−2πi e/G)
   float2* d_ws = nullptr;        // chunk·3·G³ complex64
   double* d_partial = nullptr;   // chunk·(G²/R)·2
@@ -109,18 +111,19 @@ namespace {
 std::atomic<int> g_theta_engine{EFN_THETA_AUTO};
 
 // The plane pass holds one G×G plane of a component on chip: it fits one CU's
-// registers and LDS without spilling up to 128³ (at 192³ the 48 resident
-// complex values per thread of a 768-thread block left the compiler ~90
-// VGPRs short: spilled to memory, it measured slower than five passes).
+// registers and LDS up to 128³.  At 192³ the plane (295 KB) does not, and the
+// three passes run on half-planes (theta3s_kernels.hpp); 256³ keeps five.
 template <int G>
-constexpr bool has_three_pass() { return G == 64 || G == 96 || G == 128; }
+constexpr bool has_three_pass() { return G == 64 || G == 96 || G == 128 || efn::split_pass_grid(G); }
 
 // The pass structure used for grid G when the engine is EFN_THETA_AUTO: three
-// passes wherever they are compiled (Θ on a 32-vector block, 10,000-point
+// passes wherever whole planes fit (Θ on a 32-vector block, 10,000-point
 // network, default chunk: 64³ 128,026 vs 105,982 vector-applies/s, 96³ 29,107
-// vs 28,800, 128³ 14,413 vs 13,342).
+// vs 28,800, 128³ 14,413 vs 13,342), five at 192³, where the half-plane three
+// passes move 137 instead of 229 B per voxel but issue-bound measured 3,758
+// against 3,860 vector-applies/s.
 template <int G>
-constexpr int theta_passes_auto() { return has_three_pass<G>() ? 3 : 5; }
+constexpr int theta_passes_auto() { return has_three_pass<G>() && !efn::split_pass_grid(G) ? 3 : 5; }
 
 // A forced engine (3 or 5) wins; under EFN_THETA_AUTO a context's own choice
 // (ctx_passes, e.g. from the tuning table) wins over the compiled default.
@@ -197,10 +200,86 @@ void launch_xout(const OpRequest& rq, const EpiArgs& ea, dim3 grid, long long v0
   }
 }
 
+// compute units of the current device (persistent launches size their grid by it)
+long long cu_count() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+    n = 256;
+  return n;
+}
+
+// xout_split with the request's operator kind and epilogue
+template <int G>
+void launch_xout_split(const OpRequest& rq, const EpiArgs& ea, dim3 grid, long long v0, const float2* ws,
+                       const float2* twid, hipStream_t s) {
+  using namespace efn;
+  const dim3 blk(xout_split_threads<G>());
+  if (rq.kind == kPrecond) {
+    hipLaunchKernelGGL((xout_split_kernel<G, kPrecond, kStore>), grid, blk, 0, s, ws, rq.vm, v0, twid, ea);
+    return;
+  }
+  switch (rq.epi) {
+    case kStore:
+      hipLaunchKernelGGL((xout_split_kernel<G, kTheta, kStore>), grid, blk, 0, s, ws, rq.vm, v0, twid, ea);
+      break;
+    case kCheb:
+      hipLaunchKernelGGL((xout_split_kernel<G, kTheta, kCheb>), grid, blk, 0, s, ws, rq.vm, v0, twid, ea);
+      break;
+    case kKpm:
+      hipLaunchKernelGGL((xout_split_kernel<G, kTheta, kKpm>), grid, blk, 0, s, ws, rq.vm, v0, twid, ea);
+      break;
+    default:
+      hipLaunchKernelGGL((xout_split_kernel<G, kTheta, kDot>), grid, blk, 0, s, ws, rq.vm, v0, twid, ea);
+      break;
+  }
+}
+
+// Split three passes per chunk: xin_split | yzlds | xout_split (theta3s_kernels.hpp).
+template <int G>
+int run_op3s(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
+  using namespace efn;
+  constexpr int nb = G * split_tiles<G>();  // xin/xout blocks per vector
+  const int use_eps = rq.kind == kTheta ? 0 : 1;
+  for (int v0 = 0; v0 < rq.m; v0 += c->chunk) {
+    const int nv = rq.m - v0 < c->chunk ? rq.m - v0 : c->chunk;
+    float2* const ws = c->d_ws;
+    const long long halves = (long long)nv * G * 2;
+    const dim3 gx((unsigned)(nv * nb));
+    if (rq.kind == kTheta)
+      hipLaunchKernelGGL((xin_split_kernel<G, kTheta>), gx, dim3(xin_split_threads<G>()), 0, s, rq.in, ws,
+                         rq.vm, (long long)v0, c->d_twid);
+    else
+      hipLaunchKernelGGL((xin_split_kernel<G, kPrecond>), gx, dim3(xin_split_threads<G>()), 0, s, rq.in, ws,
+                         rq.vm, (long long)v0, c->d_twid);
+    EFN_LAUNCH_CHECK();
+    // persistent: one block per CU walks its range of half-planes
+    const long long items = 3 * halves;
+    const long long nblk = items < cu_count() ? items : cu_count();
+    hipLaunchKernelGGL((yzlds_kernel<G>), dim3((unsigned)nblk), dim3(LdsHalf<G>::NT), 0, s, ws, rq.vm,
+                       (long long)v0, use_eps, c->d_twid, items);
+    EFN_LAUNCH_CHECK();
+    EpiArgs ea = rq.ea;
+    ea.x = rq.in;
+    ea.partial = c->d_partial;
+    launch_xout_split<G>(rq, ea, gx, (long long)v0, ws, c->d_twid, s);
+    EFN_LAUNCH_CHECK();
+    if (rq.reduce_mode) {
+      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nv), dim3(256), 0, s, c->d_partial, nb, nv,
+                         rq.reduce_mode, rq.mom ? rq.mom + (long long)v0 * rq.ld : nullptr, rq.ld,
+                         rq.k, rq.est ? rq.est + v0 : nullptr, rq.ck64);
+      EFN_LAUNCH_CHECK();
+    }
+  }
+  return EFN_OK;
+}
+
 // Three passes per chunk: xin | yzmid | xout (theta3_kernels.hpp).
 template <int G>
 int run_op3(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
   using namespace efn;
+  if constexpr (split_pass_grid(G)) return run_op3s<G>(c, rq, s);
+  else {
   constexpr int nb = G * x3_tiles<G>();  // xin/xout blocks per vector
   const int use_eps = rq.kind == kTheta ? 0 : 1;
   for (int v0 = 0; v0 < rq.m; v0 += c->chunk) {
@@ -230,6 +309,7 @@ int run_op3(efn_ctx* c, const OpRequest& rq, hipStream_t s) {
     }
   }
   return EFN_OK;
+  }
 }
 
 template <int G>
@@ -365,6 +445,7 @@ void destroy(efn_ctx* c) {
   free_dev(c->d_eps);
   free_dev(c->d_kn2);
   free_dev(c->d_xeps);
+  free_dev(c->d_heps);
   free_dev(c->d_twid);
   free_dev(c->d_ws);
   free_dev(c->d_partial);
@@ -481,6 +562,21 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
         xeps[t] = inv_eps[v];
         xeps[nx + t] = epsv[v];
       }
+  // and de-interleaved in z for the half-plane pass [x][h][y][m], z = 2m + h
+  std::vector<float> heps;
+  if (efn::split_pass_grid(G)) {
+    const int H = G / 2;
+    heps.assign(2 * (size_t)G3, 0.f);
+    for (int x = 0; x < G; ++x)
+      for (int h = 0; h < 2; ++h)
+        for (int y = 0; y < G; ++y)
+          for (int m = 0; m < H; ++m) {
+            const size_t v = ((size_t)x * G + y) * G + 2 * m + h;
+            const size_t t = (((size_t)x * 2 + h) * G + y) * H + m;
+            heps[t] = inv_eps[v];
+            heps[(size_t)G3 + t] = epsv[v];
+          }
+  }
   std::vector<float2> twid(G);
   for (int e = 0; e < G; ++e)
     twid[e] = make_float2((float)std::cos(2 * M_PI * e / G), (float)-std::sin(2 * M_PI * e / G));
@@ -514,6 +610,7 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   EFN_ALLOC(c->d_eps, G3 * sizeof(float));
   EFN_ALLOC(c->d_kn2, G3 * sizeof(float));
   EFN_ALLOC(c->d_xeps, xeps.size() * sizeof(float));
+  if (!heps.empty()) EFN_ALLOC(c->d_heps, heps.size() * sizeof(float));
   EFN_ALLOC(c->d_twid, G * sizeof(float2));
   if (chunk > 0) {
     EFN_ALLOC(c->d_ws, c->ws_bytes);
@@ -529,6 +626,8 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
   if (e == hipSuccess) e = hipMemcpy(c->d_eps, epsv.data(), G3 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_kn2, kn2f.data(), G3 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_xeps, xeps.data(), xeps.size() * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !heps.empty())
+    e = hipMemcpy(c->d_heps, heps.data(), heps.size() * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->d_twid, twid.data(), G * sizeof(float2), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     efn::OpDesc d;
@@ -537,6 +636,8 @@ int efn_ctx_create(int G, double L, double kx, double ky, double kz, const float
     d.eps = c->d_eps;
     d.inv_eps_x = c->d_xeps;
     d.eps_x = c->d_xeps + nx;
+    d.inv_eps_h = c->d_heps;
+    d.eps_h = c->d_heps ? c->d_heps + G3 : nullptr;
     e = hipMemcpy(c->d_desc, &d, sizeof(d), hipMemcpyHostToDevice);
   }
   if (e != hipSuccess) {

@@ -64,6 +64,8 @@ struct OpDesc {
   const float* eps;         // ε/G³ per voxel
   const float* inv_eps_x;   // the same two tables in the x pass's tile order:
   const float* eps_x;       // [y][z / ZT][x][z mod ZT], ZT = xtile_for(G)
+  const float* inv_eps_h;   // the same two tables de-interleaved in z for the
+  const float* eps_h;       // half-plane pass: [x][h][y][m] = (x, y, z = 2m + h)
 };
 
 // Vector map of one launch: launch-local vector v belongs to slot v / nrow and

@@ -1,9 +1,11 @@
-"""The three-pass Θ (xin | yzmid | xout) against the FP64 oracle, with the
-pass structure forced to 3 on the grids that have it (64³, 96³, 128³): Θ, the
-preconditioner and the fused Chebyshev / KPM / KPM-count steps at rel L2
-< 1e-5 per vector (moments and estimates at 1e-5 of their dot scale), the
-three-pass and five-pass forms within 1e-5 of each other, and chunking and
-multi-slot launches bitwise equal to one launch."""
+"""The three-pass Θ against the FP64 oracle, with the pass structure forced to
+3 on the grids that have it: whole planes (xin | yzmid | xout) at 64³, 96³ and
+128³, half-planes with the z-FFT's radix-2 stages in the x passes
+(xin_split | yzlds | xout_split) at 192³.  Θ, the preconditioner and the fused
+Chebyshev / KPM / KPM-count steps at rel L2 < 1e-5 per vector (moments and
+estimates at 1e-5 of their dot scale), the three-pass and five-pass forms
+within 1e-5 of each other, and chunking and multi-slot launches bitwise equal
+to one launch."""
 import numpy as np
 import pytest
 
@@ -34,7 +36,8 @@ def _eps(G, seed):
 
 
 @pytest.mark.parametrize("G,k", [(64, (0, 0, 0)), (64, (0.5, 0.5, 0.5)), (96, (0.1, 0.0, -0.3)),
-                                 (96, (0, 0, 0)), (128, (0, 0, 0)), (128, (0.25, -0.1, 0.4))])
+                                 (96, (0, 0, 0)), (128, (0, 0, 0)), (128, (0.25, -0.1, 0.4)),
+                                 (192, (0, 0, 0)), (192, (0.25, -0.1, 0.4))])
 def test_three_pass_theta_and_precondition_match_oracle(three, G, k):
     from eigenfns_amd.operator import MaxwellOperator, theta_passes
     assert theta_passes(G) == 3
@@ -49,10 +52,10 @@ def test_three_pass_theta_and_precondition_match_oracle(three, G, k):
     assert r1.max() < TOL and r2.max() < TOL
 
 
-def test_three_and_five_pass_agree_and_chunking_is_bitwise(three):
+@pytest.mark.parametrize("G", [128, 192])
+def test_three_and_five_pass_agree_and_chunking_is_bitwise(three, G):
     import torch
     from eigenfns_amd.operator import MaxwellOperator, theta_engine
-    G = 128
     eps = _eps(G, 3)
     one = MaxwellOperator(eps, 9.0, (0.1, 0.2, 0.3), max_chunk=1)
     many = MaxwellOperator(eps, 9.0, (0.1, 0.2, 0.3), max_chunk=3)
@@ -66,10 +69,10 @@ def test_three_and_five_pass_agree_and_chunking_is_bitwise(three):
     assert _rel(a.cpu().numpy(), c.cpu().numpy()).max() < TOL
 
 
-def test_three_pass_multi_slot_equals_single(three):
+@pytest.mark.parametrize("G", [128, 192])
+def test_three_pass_multi_slot_equals_single(three, G):
     import torch
     from eigenfns_amd.operator import MaxwellOperator, theta_multi
-    G = 128
     ws = MaxwellOperator(np.ones((G, G, G), np.float32), 1.0, max_chunk=2)
     ops = [MaxwellOperator(_eps(G, s), 7.5, kf, max_chunk=-1)
            for s, kf in ((1, (0, 0, 0)), (2, (0.3, 0.0, -0.2)))]
@@ -83,7 +86,7 @@ def test_three_pass_multi_slot_equals_single(three):
         assert torch.equal(Y[i * nrow:(i + 1) * nrow], single)
 
 
-@pytest.mark.parametrize("G", [64, 128])
+@pytest.mark.parametrize("G", [64, 128, 192])
 def test_three_pass_fused_steps_on_the_c4_network(three, G):
     import torch
     from eigenfns_amd.operator import MaxwellOperator
@@ -141,3 +144,15 @@ def test_five_pass_still_matches_oracle_where_three_is_default(G):
     finally:
         theta_engine(prev)
     assert theta_passes(G) == 3
+
+
+def test_split_three_pass_stays_opt_in_at_192():
+    """At 192³ the half-plane three passes are compiled but five stay the
+    default (measured faster there); forcing 3 selects them."""
+    from eigenfns_amd.operator import theta_engine, theta_passes
+    assert theta_passes(192) == 5
+    prev = theta_engine(3)
+    try:
+        assert theta_passes(192) == 3
+    finally:
+        theta_engine(prev)

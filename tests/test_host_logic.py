@@ -73,4 +73,5 @@ def test_tuning_candidates_respect_the_workspace_cap():
         assert all(ch * 3 * G ** 3 * 8 <= 1.1e9 for _, ch in c)
         assert (5, autotune.default_chunk(G)) in c
     assert {p for p, _ in autotune.candidates(128)} == {3, 5}
-    assert {p for p, _ in autotune.candidates(192)} == {5}
+    assert {p for p, _ in autotune.candidates(192)} == {3, 5}   # half-plane three passes
+    assert {p for p, _ in autotune.candidates(256)} == {5}
