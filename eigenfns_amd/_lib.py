@@ -41,6 +41,10 @@ _SIGNATURES = {
     "efn_precondition": (_i, [_vp, _vp, _vp, _i, _vp]),
     "efn_theta_multi": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _vp]),
     "efn_precondition_multi": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "efn_theta_ragged": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _vp]),
+    "efn_precondition_ragged": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "efn_gram_ragged": (_i, [_i, _vp, _vp, _vp, _vp, _i64, _i, _vp, _vp, _sz, _vp]),
+    "efn_combine_ragged": (_i, [_i, _vp, _vp, _vp, _vp, _i64, _vp, _f, _vp, _vp, _vp, _vp]),
     "efn_precondition_simple": (_i, [_vp, _vp, _vp, _i, _f, _vp]),
     "efn_cheb_step": (_i, [_vp, _vp, _vp, _vp, _f, _f, _i, _vp]),
     "efn_kpm_step": (_i, [_vp, _vp, _vp, _vp, _i, _i, _f, _i, _vp]),

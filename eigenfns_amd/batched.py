@@ -19,8 +19,15 @@ only, with reduction orders that depend only on the grid size and block width
 from the request's own counter-based stream), so a request's eigenvalues are
 bitwise the same alone and in any mix.
 
-Storage per slot (fixed, HBM-resident): [X; W; P] and [ΘX; ΘW; ΘP] (3m rows
-each), R, two m-row temporaries, the carry and the locked rows (zero-padded to
+Block widths may differ per request (``Request.m`` / ``Request.guard``, at
+most the engine's width): every launch is ragged — slot s runs its own m_s
+rows through the same Θ / preconditioner / Gram / combine launches
+(efn_theta_ragged, efn_gram_ragged, efn_combine_ragged) — so problems of
+different widths (the k-points of a band-structure path) share one engine.
+
+Storage per slot (fixed, HBM-resident, sized by the engine's width M):
+[X; W; P] and [ΘX; ΘW; ΘP] (3M rows each; a slot of width m uses the leading
+3m), R, two M-row temporaries, the carry and the locked rows (zero-padded to
 the engine capacity so deflation runs at one shape).
 """
 from __future__ import annotations
@@ -32,11 +39,10 @@ import numpy as np
 import torch
 
 from .linalg import (COMBINE_IN_PLACE_MAX, DEAD_SPREAD, ORTHO_METHODS, RR_DEAD_PENALTY, SVQB_DROP,
-                     cholqr_coeffs_batched, combine_multi,
-                     combine_multi_split, copy_multi, dead_spread, eigh_batched_device, eigh_stack_host,
-                     gram_multi, one_blas_thread, rowreduce_strided, row_axpby_norm_strided,
-                     rowstats_strided, MAX_PROBLEMS)
-from .operator import MaxwellOperator, precondition_multi, theta_multi
+                     cholqr_coeffs_batched, combine_ragged, dead_spread, eigh_batched_device,
+                     eigh_list_host, gram_ragged, one_blas_thread, rowreduce_strided,
+                     row_axpby_norm_strided, rowstats_strided, MAX_PROBLEMS)
+from .operator import MaxwellOperator, precondition_ragged, theta_ragged
 from .solver import STAGNATION_WINDOW, StagnationLock
 
 
@@ -49,6 +55,8 @@ class Request:
     seed: int = 0
     rid: int = 0
     meta: dict = field(default_factory=dict)
+    m: int | None = None       # block width (None: the engine's)
+    guard: int | None = None   # guard rows (None: the engine's)
     # filled by the engine
     vals: np.ndarray | None = None
     res: np.ndarray | None = None
@@ -64,6 +72,7 @@ class Request:
 
 class _Slot:
     def __init__(self, req: Request, m: int, guard: int):
+        # m: this request's block width (≤ the engine's), guard its guard rows
         self.req = req
         self.op = req.op
         self.nev = req.nev
@@ -81,9 +90,10 @@ class _Slot:
 
 
 class BatchedLOBPCG:
-    """Lockstep LOBPCG over up to `nslots` (≤ 32) problems of grid size G.
+    """Lockstep LOBPCG over up to `nslots` (≤ 64) problems of grid size G.
 
-    Parameters follow lobpcg_blocks (m, guard, tol, maxit); `max_nev` sizes the
+    Parameters follow lobpcg_blocks (m, guard, tol, maxit); `m` is also the
+    widest block a request may bring (Request.m), `max_nev` sizes the
     per-slot locked storage.  `keep_vectors` copies each finished request's
     locked vectors out of the engine (otherwise only eigenvalues and residuals
     are returned)."""
@@ -124,8 +134,8 @@ class BatchedLOBPCG:
         if ortho not in ORTHO_METHODS:
             raise ValueError(f"ortho must be one of {ORTHO_METHODS}, got {ortho!r}")
         self.ortho = ortho
-        per = self.m - self.guard
-        self.cap = int(np.ceil(max_nev / per)) * per + self.m
+        # locked rows: at most nev plus one block's overshoot
+        self.cap = int(max_nev) + self.m
         self.max_nev = int(max_nev)
         G = self.G
         shape = (2, G, G, G)
@@ -168,65 +178,85 @@ class BatchedLOBPCG:
             raise ValueError("request grid size does not match the engine")
         if req.nev > self.max_nev:
             raise ValueError(f"nev {req.nev} exceeds the engine capacity {self.max_nev}")
+        m = int(req.m) if req.m is not None else self.m
+        guard = int(req.guard) if req.guard is not None else self.guard
+        if not 0 < m <= self.m:
+            raise ValueError(f"request width {m} must be in [1, {self.m}] (the engine's width)")
+        if not 0 < guard < m:
+            raise ValueError("need 0 < guard < width")
         for s, sl in enumerate(self.slots):
             if sl is None:
-                self.slots[s] = _Slot(req, self.m, self.guard)
+                self.slots[s] = _Slot(req, m, guard)
                 self.L[s * self.cap:(s + 1) * self.cap].zero_()
                 req.t_admit = time.perf_counter()
                 return s
         raise RuntimeError("no free slot")
 
-    # ---- row views ----------------------------------------------------------
+    # ---- row views (slot s of width w: X, W, P are rows [0, w), [w, 2w),
+    # [2w, 3w) of its 3M-row region) ------------------------------------------
+    def _w(self, s):
+        return self.slots[s].m
+
     def _X(self, s):
-        return self.S[s * 3 * self.m:s * 3 * self.m + self.m]
+        b = s * 3 * self.m
+        return self.S[b:b + self._w(s)]
 
     def _W(self, s):
-        return self.S[s * 3 * self.m + self.m:s * 3 * self.m + 2 * self.m]
+        b, w = s * 3 * self.m, self._w(s)
+        return self.S[b + w:b + 2 * w]
 
     def _P(self, s):
-        return self.S[s * 3 * self.m + 2 * self.m:(s + 1) * 3 * self.m]
+        b, w = s * 3 * self.m, self._w(s)
+        return self.S[b + 2 * w:b + 3 * w]
 
     def _XW(self, s):
-        return self.S[s * 3 * self.m:s * 3 * self.m + 2 * self.m]
+        b = s * 3 * self.m
+        return self.S[b:b + 2 * self._w(s)]
 
     def _Sb(self, s):
-        return self.S[s * 3 * self.m:(s + 1) * 3 * self.m]
+        b = s * 3 * self.m
+        return self.S[b:b + 3 * self._w(s)]
 
     def _HSb(self, s):
-        return self.HS[s * 3 * self.m:(s + 1) * 3 * self.m]
-
-    def _rows(self, buf, s, width):
-        return buf[s * width:(s + 1) * width]
+        b = s * 3 * self.m
+        return self.HS[b:b + 3 * self._w(s)]
 
     def _t1(self, s):
-        return self.T12[2 * s * self.m:(2 * s + 1) * self.m]
+        return self.T12[2 * s * self.m:2 * s * self.m + self._w(s)]
 
     def _t2(self, s):
-        return self.T12[(2 * s + 1) * self.m:(2 * s + 2) * self.m]
+        return self.T12[(2 * s + 1) * self.m:(2 * s + 1) * self.m + self._w(s)]
 
     def _combine_back(self, idx, C, blocks):
         """block_j ← C_jᵀ·block_j for slots idx: in place up to
         COMBINE_IN_PLACE_MAX rows, else through T1 and a copy."""
+        ws = [self._w(s) for s in idx]
         if self.m <= COMBINE_IN_PLACE_MAX:
-            combine_multi(C, blocks, blocks)
+            combine_ragged(C, blocks, ws, ws, blocks)
         else:
             Ts = [self._t1(s) for s in idx]
-            combine_multi(C, blocks, Ts)
-            copy_multi(Ts, blocks)
+            combine_ragged(C, blocks, ws, ws, Ts)
+            for T, B in zip(Ts, blocks):
+                B.copy_(T)
 
     def _Lk(self, s, k):
         return self.L[s * self.cap:s * self.cap + k]
 
     # ---- batched building blocks ---------------------------------------------
+    def _project_out(self, idx, basis, target, rows_b, rows_t):
+        """target(s) ← target(s) − basis(s)(basis(s)ᴴ target(s)), one Gram and
+        one combine launch over slots idx (ragged row counts)."""
+        Bs = [basis(s) for s in idx]
+        Ts = [target(s) for s in idx]
+        C = gram_ragged(Bs, Ts)
+        combine_ragged(-C, Bs, [rows_b(s) for s in idx], [rows_t(s) for s in idx], Ts, Xs=Ts, alpha=1.0)
+
     def _deflate(self, idx, target):
         """target(s) ← target(s) − L_s(L_sᴴ target(s)) for slots idx (zero-padded locked rows)."""
         nl = max(self.slots[s].n_locked for s in idx)
         if nl == 0:
             return
-        Ls = [self._Lk(s, nl) for s in idx]
-        Ts = [target(s) for s in idx]
-        C = gram_multi(Ls, Ts)
-        combine_multi(-C, Ls, Ts, Xs=Ts, alpha=1.0)
+        self._project_out(idx, lambda s: self._Lk(s, nl), target, lambda s: nl, self._w)
 
     @staticmethod
     def _whiten(w, V):
@@ -239,42 +269,84 @@ class BatchedLOBPCG:
         # select rather than scale by zero: a dropped direction's vector may be non-finite
         return torch.where(keep[:, None, :], V * scale[:, None, :], torch.zeros_like(V)), keep
 
-    def _whiten_small(self, G):
-        """(V·w^{-1/2} with dropped directions zeroed, keep mask) of a Gram stack:
-        on the host in NumPy when the eigensolve is there (one copy of the
-        coefficients back; masks as NumPy), on the device otherwise.  With
-        Cholesky-QR the coefficients and mask (rows below the rank) come from
-        the device kernel in every mode."""
+    def _eigh_device_ragged(self, A, widths, lowest=None):
+        """(w, V) of each problem's leading widths[j] block of the padded device
+        stack A, grouped by width (one batched eigensolve per width), padded
+        back: V's leading widths[j] rows and min(lowest, width) columns, w
+        padded with +inf."""
+        nprob, K = int(A.shape[0]), int(A.shape[1])
+        cols = K if lowest is None else lowest
+        w = torch.full((nprob, cols), float("inf"), dtype=torch.float64, device=A.device)
+        V = torch.zeros((nprob, K, cols), dtype=torch.complex128, device=A.device)
+        for k in sorted(set(widths)):
+            js = [j for j, wj in enumerate(widths) if wj == k]
+            jt = torch.as_tensor(js, device=A.device)
+            lo = None if lowest is None else min(lowest, k)
+            wk, Vk = eigh_batched_device(A[jt][:, :k, :k].contiguous(), lowest=lo)
+            w[jt, :wk.shape[1]] = wk
+            V[jt, :k, :Vk.shape[2]] = Vk
+        return w, V
+
+    def _whiten_small(self, G, widths):
+        """(V·w^{-1/2} with dropped directions zeroed, keep mask) of a padded
+        Gram stack (problem j of width widths[j]): on the host in NumPy when
+        the eigensolve is there (one copy of the coefficients back; masks as
+        NumPy), on the device otherwise.  With Cholesky-QR the coefficients and
+        mask (rows below the rank) come from the device kernel in every mode:
+        the padding is zero rows, which the pivoted Cholesky never selects."""
         if self.ortho == "cholqr":
             C, rank = cholqr_coeffs_batched(G)
             return C, torch.arange(G.shape[1], device=G.device)[None, :] < rank[:, None]
+        K = int(G.shape[1])
         if self.small_dense == "host":
-            w, V = eigh_stack_host(G.cpu().numpy())
-            wmax = w.max(axis=1, keepdims=True)
-            keep = (w > SVQB_DROP * np.maximum(wmax, 1e-300)) & (wmax > 0)
-            scale = np.where(keep, 1.0 / np.sqrt(np.where(keep, w, 1.0)), 0.0)
-            return torch.as_tensor(V * scale[:, None, :], device=G.device), keep
-        return self._whiten(*eigh_batched_device(G))
+            Gh = G.cpu().numpy()
+            res = eigh_list_host([Gh[j, :k, :k] for j, k in enumerate(widths)])
+            Cw = np.zeros((len(widths), K, K), dtype=np.complex128)
+            keep = np.zeros((len(widths), K), dtype=bool)
+            for j, (k, (w, V)) in enumerate(zip(widths, res)):
+                wmax = w.max()
+                kp = (w > SVQB_DROP * max(wmax, 1e-300)) & (wmax > 0)
+                Cw[j, :k, :k] = V * np.where(kp, 1.0 / np.sqrt(np.where(kp, w, 1.0)), 0.0)[None, :]
+                keep[j, :k] = kp
+            return torch.as_tensor(Cw, device=G.device), keep
+        w, V = self._eigh_device_ragged(G, widths)
+        w = torch.where(torch.isinf(w), torch.zeros_like(w), w)
+        return self._whiten(w, V)
 
     def _svqb(self, idx, block, passes=2):
         """SVQB of block(s) for slots idx, two sweeps, written back in place.
-        Returns the last sweep's keep mask (len(idx), rows): a dropped direction
+        Returns the last sweep's keep mask (len(idx), M): a dropped direction
         is an exactly zero row of the result."""
         keep = None
+        widths = [self._w(s) for s in idx]
         for _ in range(passes):
             Bs = [block(s) for s in idx]
-            Cw, keep = self._whiten_small(gram_multi(Bs, Bs, hermitian=True))
+            Cw, keep = self._whiten_small(gram_ragged(Bs, Bs, hermitian=True), widths)
             self._combine_back(idx, Cw, Bs)
         return keep
 
     def _check_dead(self, idx, dead):
         """Compare the dead-row bookkeeping with measured row norms (tests)."""
-        m = self.m
-        rn2 = rowreduce_strided(0, self.S, None, 3 * m, 3 * m, 3 * m, self.nslots * 3 * m,
-                                self.n).real.cpu().numpy().reshape(self.nslots, 3 * m)
+        M = self.m
+        rn2 = rowreduce_strided(0, self.S, None, 3 * M, 3 * M, 3 * M, self.nslots * 3 * M,
+                                self.n).real.cpu().numpy().reshape(self.nslots, 3 * M)
         for j, s in enumerate(idx):
-            if not np.array_equal(dead[j], np.sqrt(rn2[s]) < 0.5):
+            w = self._w(s)
+            if not np.array_equal(dead[j], np.sqrt(rn2[s, :3 * w]) < 0.5):
                 raise AssertionError(f"slot {s}: dead-row bookkeeping disagrees with row norms")
+
+    def _pack_rr(self, idx, Cs):
+        """Padded (nslots, 3M, 2M) [c/‖c‖ | c'/‖c'‖] of the per-slot lowest
+        Ritz coefficient blocks Cs[j] (3w × w), c' = c with its X rows zeroed."""
+        M = self.m
+        out = np.zeros((len(idx), 3 * M, 2 * M), dtype=np.complex128)
+        for j, (s, C) in enumerate(zip(idx, Cs)):
+            w = self._w(s)
+            Cp = C.copy()
+            Cp[:w] = 0
+            out[j, :3 * w, :w] = C / np.maximum(np.linalg.norm(C, axis=0), 1e-20)[None, :]
+            out[j, :3 * w, w:2 * w] = Cp / np.maximum(np.linalg.norm(Cp, axis=0), 1e-20)[None, :]
+        return out
 
     def _rr_host(self, idx, A, keep_w, keepp):
         """Rayleigh–Ritz coefficients [c/‖c‖ | c'/‖c'‖] with the eigensolve on the
@@ -282,63 +354,80 @@ class BatchedLOBPCG:
         Dead rows (solver.py:296-297, norm < 0.5) without another pass over S:
         X norms from this step's residual statistics; W and P rows are unit or
         exactly zero after SVQB / whitening (their keep masks)."""
-        m = self.m
         A = A.cpu().numpy()
         keep_w = keep_w.cpu().numpy() if torch.is_tensor(keep_w) else keep_w
         keepp = keepp.cpu().numpy() if torch.is_tensor(keepp) else keepp
-        dead = np.stack([~np.concatenate([self._xn[s] >= 0.5, keep_w[j], keepp[j]])
-                         for j, s in enumerate(idx)])
+        mats, dead = [], []
+        for j, s in enumerate(idx):
+            w = self._w(s)
+            d = ~np.concatenate([self._xn[s, :w] >= 0.5, keep_w[j, :w], keepp[j, :w]])
+            dead.append(d)
+            Aj = A[j, :3 * w, :3 * w]
+            Aj = (Aj + Aj.conj().T) / 2           # symmetrised, dead rows penalised (solver.py:294-297)
+            penalty = max(RR_DEAD_PENALTY, 100.0 * float(np.abs(np.diag(Aj)).max()))
+            Aj[np.diag_indices(3 * w)] += np.where(d, penalty * (1.0 + DEAD_SPREAD * np.arange(3 * w)), 0.0)
+            mats.append(Aj)
         if self.verify_dead_rows:
             self._check_dead(idx, dead)
-        for j in range(len(idx)):   # symmetrised, dead rows penalised (solver.py:294-297)
-            A[j] = (A[j] + A[j].conj().T) / 2
-            penalty = max(RR_DEAD_PENALTY, 100.0 * float(np.abs(np.diag(A[j])).max()))
-            A[j][np.diag_indices(3 * m)] += np.where(dead[j], penalty * (1.0 + DEAD_SPREAD * np.arange(3 * m)),
-                                                     0.0)
-        _, C = eigh_stack_host(A, lowest=m)
-        Cp = C.copy()
-        Cp[:, :m] = 0
-        nx = np.maximum(np.linalg.norm(C, axis=1), 1e-20)
-        npn = np.maximum(np.linalg.norm(Cp, axis=1), 1e-20)
-        return torch.as_tensor(np.concatenate([C / nx[:, None, :], Cp / npn[:, None, :]], axis=2),
-                               device=self.S.device)
+        res = eigh_list_host(mats, lowest=[self._w(s) for s in idx])
+        return torch.as_tensor(self._pack_rr(idx, [V for _, V in res]), device=self.S.device)
 
     def _rr_device(self, idx, A, keep_w, keepp):
         """_rr_host with the penalty, eigensolve and normalisation on the device."""
-        m = self.m
+        M = self.m
+        widths = [self._w(s) for s in idx]
+        K = 3 * M
         xn = torch.stack([self._xn_dev[s] for s in idx])   # views: no host-to-device copy
-        dead = ~torch.cat([xn >= 0.5, keep_w, keepp], dim=1)
+        # dead-row mask in the padded layout: row r of slot j is X row r (< w),
+        # W row r − w or P row r − 2w; rows past 3w are padding
+        r = torch.arange(K, device=A.device)
+        wt = torch.as_tensor(widths, device=A.device)[:, None]
+        xr = torch.clamp(r[None, :], max=M - 1).expand(len(idx), K)
+        wr = torch.clamp(r[None, :] - wt, 0, M - 1)
+        pr = torch.clamp(r[None, :] - 2 * wt, 0, M - 1)
+        alive = torch.where(r[None, :] < wt, torch.gather(xn, 1, xr) >= 0.5,
+                            torch.where(r[None, :] < 2 * wt, torch.gather(keep_w, 1, wr),
+                                        torch.gather(keepp, 1, pr)))
+        dead = ~alive & (r[None, :] < 3 * wt)
         if self.verify_dead_rows:
-            self._check_dead(idx, dead.cpu().numpy())
+            dh = dead.cpu().numpy()
+            self._check_dead(idx, [dh[j, :3 * w] for j, w in enumerate(widths)])
         A = (A + A.conj().transpose(1, 2)) / 2
         penalty = torch.clamp(100.0 * torch.diagonal(A, dim1=1, dim2=2).abs().max(dim=1).values,
                               min=RR_DEAD_PENALTY)
-        A = A + torch.diag_embed(torch.where(dead, penalty[:, None] * dead_spread(3 * m, A.device)[None, :],
+        A = A + torch.diag_embed(torch.where(dead, penalty[:, None] * dead_spread(K, A.device)[None, :],
                                              0.0).to(A.dtype))
-        _, C = eigh_batched_device(A, lowest=m)
-        Cp = C.clone()
-        Cp[:, :m] = 0
-        nx = torch.clamp(torch.linalg.vector_norm(C, dim=1), min=1e-20)
-        npn = torch.clamp(torch.linalg.vector_norm(Cp, dim=1), min=1e-20)
-        return torch.cat([C / nx[:, None, :], Cp / npn[:, None, :]], dim=2)
+        _, C = self._eigh_device_ragged(A, [3 * w for w in widths], lowest=M)
+        C2 = torch.zeros((len(idx), K, 2 * M), dtype=torch.complex128, device=A.device)
+        for j, w in enumerate(widths):
+            c = C[j, :3 * w, :w]
+            cp = c.clone()
+            cp[:w] = 0
+            C2[j, :3 * w, :w] = c / torch.clamp(torch.linalg.vector_norm(c, dim=0), min=1e-20)[None, :]
+            C2[j, :3 * w, w:2 * w] = cp / torch.clamp(torch.linalg.vector_norm(cp, dim=0), min=1e-20)[None, :]
+        return C2
 
-    def _theta(self, idx, src, row0_src, dst, row0_dst, stride_src, stride_dst, nrow):
+    def _theta(self, idx, part):
+        """Θ of the X rows (part "X") or the W and P rows ("WP") of slots idx
+        into the same rows of HS: one ragged launch set per 64 slots."""
+        M = self.m
         for c in range(0, len(idx), MAX_PROBLEMS):
-            part = idx[c:c + MAX_PROBLEMS]
-            theta_multi(self.ws, [self.slots[s].op for s in part], src,
-                        [s * stride_src + row0_src for s in part], dst,
-                        [s * stride_dst + row0_dst for s in part], nrow)
-        for s in idx:
-            self.slots[s].req.theta_applications += nrow
+            sub = idx[c:c + MAX_PROBLEMS]
+            off = [s * 3 * M + (0 if part == "X" else self._w(s)) for s in sub]
+            rows = [self._w(s) * (1 if part == "X" else 2) for s in sub]
+            theta_ragged(self.ws, [self.slots[s].op for s in sub], self.S, off, self.HS, off, rows)
+            for s, n in zip(sub, rows):
+                self.slots[s].req.theta_applications += n
 
     # ---- phases -------------------------------------------------------------
     def _init_blocks(self, idx):
-        m = self.m
+        M = self.m
         for s in idx:
             sl = self.slots[s]
+            m = sl.m
             X = self._X(s)
             if sl.nc:
-                X[:sl.nc].copy_(self.CARRY[s * m:s * m + sl.nc])
+                X[:sl.nc].copy_(self.CARRY[s * M:s * M + sl.nc])
             if sl.nc < m:
                 Z = sl.op.random_block(m, sl.req.seed, kind="normal", row0=sl.block_idx * m)
                 Z = Z[:m - sl.nc]
@@ -348,44 +437,42 @@ class BatchedLOBPCG:
             self._P(s).zero_()
         self._deflate(idx, self._X)
         self._svqb(idx, self._X)
-        self._theta(idx, self.S, 0, self.HS, 0, 3 * m, 3 * m, m)
+        self._theta(idx, "X")
         for s in idx:
             sl = self.slots[s]
             sl.phase = "iter"
             sl.it = 0
             sl.stag = StagnationLock(self.tol, self.stagnation)
-            sl.n_lock = min(m - self.guard, sl.nev - sl.locked_vals.size)
+            sl.n_lock = min(sl.m - sl.guard, sl.nev - sl.locked_vals.size)
 
     def _iterate(self, idx, lam):
-        m = self.m
+        M = self.m
+        w = self._w
         t = time.perf_counter()
         # W = K⁻¹R, deflated, projected out of X twice, SVQB (solver.py:251-257)
         for c in range(0, len(idx), MAX_PROBLEMS):
             part = idx[c:c + MAX_PROBLEMS]
-            precondition_multi(self.ws, [self.slots[s].op for s in part], self.R,
-                               [s * m for s in part], self.S, [s * 3 * m + m for s in part], m)
+            precondition_ragged(self.ws, [self.slots[s].op for s in part], self.R,
+                                [s * M for s in part], self.S, [s * 3 * M + w(s) for s in part],
+                                [w(s) for s in part])
         self._deflate(idx, self._W)
         for _ in range(2):
-            Xs = [self._X(s) for s in idx]
-            Ws = [self._W(s) for s in idx]
-            combine_multi(-gram_multi(Xs, Ws), Xs, Ws, Xs=Ws, alpha=1.0)
+            self._project_out(idx, self._X, self._W, w, w)
         t = self._tick("precond+deflate+proj", t)
         keep_w = self._svqb(idx, self._W)
         t = self._tick("svqb_W", t)
         # P projected out of [X, W] twice, fixed-shape whitening (solver.py:258-274)
         for _ in range(2):
-            XWs = [self._XW(s) for s in idx]
-            Ps = [self._P(s) for s in idx]
-            combine_multi(-gram_multi(XWs, Ps), XWs, Ps, Xs=Ps, alpha=1.0)
+            self._project_out(idx, self._XW, self._P, lambda s: 2 * w(s), w)
         Ps = [self._P(s) for s in idx]
-        Cp, keepp = self._whiten_small(gram_multi(Ps, Ps, hermitian=True))
+        Cp, keepp = self._whiten_small(gram_ragged(Ps, Ps, hermitian=True), [w(s) for s in idx])
         self._combine_back(idx, Cp, Ps)
         t = self._tick("P", t)
         # Θ(W), Θ(P) fresh; Rayleigh–Ritz on [X; W; P] (solver.py:275-302)
-        self._theta(idx, self.S, m, self.HS, m, 3 * m, 3 * m, 2 * m)
+        self._theta(idx, "WP")
         t = self._tick("theta_WP", t)
         Sbs = [self._Sb(s) for s in idx]
-        A = gram_multi(Sbs, [self._HSb(s) for s in idx], hermitian=True)
+        A = gram_ragged(Sbs, [self._HSb(s) for s in idx], hermitian=True)
         # X ← S·c_i/‖S·c_i‖ and P ← S·c'_i/‖S·c'_i‖ (c' = c with its X rows
         # zeroed; solver.py:298-307).  [X; W; P] is orthonormal by construction
         # (SVQB, projections, whitening) and dead rows get exactly zero weight,
@@ -399,17 +486,21 @@ class BatchedLOBPCG:
         t = self._tick("RR_gram+eigh", t)
         Xs = [self._X(s) for s in idx]
         Ps = [self._P(s) for s in idx]
-        if 2 * m <= COMBINE_IN_PLACE_MAX:
-            combine_multi_split(C2, Sbs, Xs, Ps, m)
+        m_in = [3 * w(s) for s in idx]
+        m_out = [2 * w(s) for s in idx]
+        split = [w(s) for s in idx]
+        if 2 * M <= COMBINE_IN_PLACE_MAX:
+            combine_ragged(C2, Sbs, m_in, m_out, Xs, Ps, split)
         else:
             T1s = [self._t1(s) for s in idx]
             T2s = [self._t2(s) for s in idx]
-            combine_multi_split(C2, Sbs, T1s, T2s, m)
-            copy_multi(T1s, Xs)
-            copy_multi(T2s, Ps)
+            combine_ragged(C2, Sbs, m_in, m_out, T1s, T2s, split)
+            for T1, T2, X, P in zip(T1s, T2s, Xs, Ps):
+                X.copy_(T1)
+                P.copy_(T2)
         self._deflate(idx, self._X)
         t = self._tick("update_XP", t)
-        self._theta(idx, self.S, 0, self.HS, 0, 3 * m, 3 * m, m)
+        self._theta(idx, "X")
         t = self._tick("theta_X", t)
         for s in idx:
             self.slots[s].it += 1
@@ -418,7 +509,7 @@ class BatchedLOBPCG:
     def _finish_block(self, s, lam, rn, order):
         """Lock the lowest n_lock rows, keep the rest as carry (solver.py:316-345)."""
         sl = self.slots[s]
-        m = self.m
+        M = self.m
         if not np.isfinite(lam[order[:sl.n_lock]]).all():
             raise RuntimeError(f"request {sl.req.rid}: rank collapse in block {sl.block_idx}")
         X = self._X(s)
@@ -427,7 +518,7 @@ class BatchedLOBPCG:
         base = s * self.cap
         self.L[base + sl.n_locked:base + sl.n_locked + sl.n_lock] = X[lock_idx]
         sl.nc = int(carry_idx.numel())
-        self.CARRY[s * m:s * m + sl.nc] = X[carry_idx]
+        self.CARRY[s * M:s * M + sl.nc] = X[carry_idx]
         sl.n_locked += sl.n_lock
         sl.locked_vals = np.concatenate([sl.locked_vals, lam[order[:sl.n_lock]]])
         sl.locked_res = np.concatenate([sl.locked_res, rn[order[:sl.n_lock]]])
@@ -452,34 +543,36 @@ class BatchedLOBPCG:
         if not act:
             return finished
         self.steps += 1
-        m = self.m
+        M = self.m
         t = time.perf_counter()
         init = [s for s in act if self.slots[s].phase == "init"]
         if init:
             self._init_blocks(init)
             t = self._tick("init", t)
-        # Rayleigh quotients, residuals, dead rows for the X rows of every slot
-        M = self.nslots * m
+        # Rayleigh quotients, residuals, dead rows for the leading M rows of
+        # every slot (its X rows; a narrower slot's trailing rows are ignored)
+        NM = self.nslots * M
         # one read of X and ΘX for ⟨x,Θx⟩, |x|², |Θx|²; R = ΘX − λX with |R|² in the same pass
-        st = rowstats_strided(self.S, self.HS, m, 3 * m, 3 * m, M, self.n)
+        st = rowstats_strided(self.S, self.HS, M, 3 * M, 3 * M, NM, self.n)
         lam, xn2, hx2 = st[:, 0], st[:, 2], st[:, 3]
         if self.normalized_rq:
             lam = lam / torch.clamp(xn2, min=1e-30)
-        rr2 = row_axpby_norm_strided(self.HS, 3 * m, torch.ones_like(lam), self.S, 3 * m, -lam, m,
-                                     M, self.n, self.R, m)
-        self._xn_dev = torch.sqrt(xn2).view(self.nslots, m)   # device masks (small_dense="device")
+        rr2 = row_axpby_norm_strided(self.HS, 3 * M, torch.ones_like(lam), self.S, 3 * M, -lam, M,
+                                     NM, self.n, self.R, M)
+        self._xn_dev = torch.sqrt(xn2).view(self.nslots, M)   # device masks (small_dense="device")
         stats = torch.stack([lam, xn2, rr2, hx2]).cpu().numpy()
         t = self._tick("residuals+sync", t)
-        lam_h = stats[0].reshape(self.nslots, m)
-        xn = np.sqrt(stats[1]).reshape(self.nslots, m)
+        lam_h = stats[0].reshape(self.nslots, M)
+        xn = np.sqrt(stats[1]).reshape(self.nslots, M)
         self._xn = xn
-        rn_h = (np.sqrt(stats[2]) / np.maximum(np.sqrt(stats[3]), 1e-30)).reshape(self.nslots, m)
+        rn_h = (np.sqrt(stats[2]) / np.maximum(np.sqrt(stats[3]), 1e-30)).reshape(self.nslots, M)
         cont = []
         for s in act:
             sl = self.slots[s]
-            dead = xn[s] < 0.5
-            l = np.where(dead, np.inf, lam_h[s])
-            r = np.where(dead, np.inf, rn_h[s])
+            w = sl.m
+            dead = xn[s, :w] < 0.5
+            l = np.where(dead, np.inf, lam_h[s, :w])
+            r = np.where(dead, np.inf, rn_h[s, :w])
             order = np.argsort(l)
             worst = float(r[order[:sl.n_lock]].max())
             if worst < self.tol or sl.it >= self.maxit or sl.stag.stalled(worst):

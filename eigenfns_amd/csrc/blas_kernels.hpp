@@ -16,23 +16,38 @@ namespace efn {
 
 // ---------------------------------------------------------------------------
 // Gram partials: part[s][i][j] = Σ_{k in split s} conj(A[i][k])·B[j][k]
-constexpr int kMaxProb = 32;  // problems per multi-problem launch (pointer lists in kernel args)
+constexpr int kMaxProb = 64;  // problems per multi-problem launch (pointer lists in kernel args)
 struct PairPtrs {
   const float2* a[kMaxProb];
   const float2* b[kMaxProb];
 };
+// Per-problem sizes of a ragged launch (ragged = 0: every problem has the
+// launch's sizes).  A ragged Gram's output keeps the launch's (largest) shape
+// per problem, zero outside the problem's own ma × mb.
+struct ProbDims {
+  short ma[kMaxProb];
+  short mb[kMaxProb];
+  int ragged;
+};
 // Output row i < split goes to out + i·n, row i ≥ split to out2 + (i − split)·n
 // (split = m_out: one contiguous output block).
+// Ragged launches (ragged = 1) give each problem its own m_in, m_out and split
+// (C keeps the launch's largest m_in × m_out per problem).
 struct CombinePtrs {
   const float2* s[kMaxProb];
   const float2* x[kMaxProb];
   float2* out[kMaxProb];
   float2* out2[kMaxProb];
+  short m_in_p[kMaxProb];
+  short m_out_p[kMaxProb];
+  short split_p[kMaxProb];
   int split;
+  int ragged;
 };
 
 __device__ __forceinline__ float2* combine_row(const CombinePtrs& cp, int prob, int i, long long n) {
-  return i < cp.split ? cp.out[prob] + (long long)i * n : cp.out2[prob] + (long long)(i - cp.split) * n;
+  const int split = cp.ragged ? cp.split_p[prob] : cp.split;
+  return i < split ? cp.out[prob] + (long long)i * n : cp.out2[prob] + (long long)(i - split) * n;
 }
 
 // Output tile of blockIdx.y: tiles_j > 0 — row-major over a tiles_i × tiles_j
@@ -221,7 +236,7 @@ __device__ __forceinline__ void gram_block_of(int L, int ntile, int nsplit, int&
 // read).  Only the kept sub-tiles are loaded and multiplied.
 template <class P, int NA, int NB, bool SKIP10>
 __device__ __forceinline__ void gram_tile(const float2* __restrict__ A, const float2* __restrict__ B,
-                                          int ma, int mb, int i0, int j0, long long n, long long kb,
+                                          int ma, int mb, int ostride, int i0, int j0, long long n, long long kb,
                                           long long ke, int wave, double2 (*red)[kMWaveTile * kMWaveTile],
                                           double2* __restrict__ out) {
   typedef typename P::in_t T;
@@ -431,13 +446,13 @@ __device__ __forceinline__ void gram_tile(const float2* __restrict__ A, const fl
         if (!keep(a, b)) continue;
         const int i = i0 + 16 * a + P::row(g, q), j = j0 + 16 * b + r;
         const double2 o = red[0][lds_at(a, b, q)];
-        if (i < ma && j < mb) out[(long long)i * mb + j] = make_double2(v[a][b][q].x + o.x, v[a][b][q].y + o.y);
+        if (i < ma && j < mb) out[(long long)i * ostride + j] = make_double2(v[a][b][q].x + o.x, v[a][b][q].y + o.y);
       }
 }
 
 template <class P>
-__global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, int mb, int tiles_j,
-                                                        int ntile, int nsplit, long long n,
+__global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, ProbDims pd, int ma, int mb,
+                                                        int tiles_j, int ntile, int nsplit, long long n,
                                                         long long kper, double2* __restrict__ part) {
   __shared__ double2 red[2][kMWaveTile * kMWaveTile];
   const float2* __restrict__ A = pp.a[blockIdx.z];
@@ -449,6 +464,10 @@ __global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, 
   gram_block_of(blockIdx.x, ntile, nsplit, split, tile);
   tile_of(tile, tiles_j, ti_, tj_);
   const int i0 = ti_ * kMWaveTile, j0 = tj_ * kMWaveTile;
+  // this problem's sizes (the partials keep the launch's ma × mb layout)
+  const int ma_p = pd.ragged ? pd.ma[blockIdx.z] : ma;
+  const int mb_p = pd.ragged ? pd.mb[blockIdx.z] : mb;
+  if (i0 >= ma_p || j0 >= mb_p) return;  // block-uniform: a tile past this problem's edge
   const long long kb_blk = (long long)split * kper;
   const long long ke_blk = kb_blk + kper < n ? kb_blk + kper : n;
   const long long kw = kper / 4;
@@ -456,19 +475,19 @@ __global__ __launch_bounds__(256, 2) void gram_mfma_kernel(PairPtrs pp, int ma, 
   const long long ke = kb + kw < ke_blk ? kb + kw : ke_blk;
   double2* out = part + ((long long)blockIdx.z * nsplit + split) * ma * mb;
   // sub-tile shape of this tile (block-uniform)
-  const bool fa = i0 + 16 < ma, fb = j0 + 16 < mb;
+  const bool fa = i0 + 16 < ma_p, fb = j0 + 16 < mb_p;
   const bool diag = tiles_j < 0 && ti_ == tj_;
   if (fa && fb) {
     if (diag)
-      gram_tile<P, 2, 2, true>(A, B, ma, mb, i0, j0, n, kb, ke, wave, red, out);
+      gram_tile<P, 2, 2, true>(A, B, ma_p, mb_p, mb, i0, j0, n, kb, ke, wave, red, out);
     else
-      gram_tile<P, 2, 2, false>(A, B, ma, mb, i0, j0, n, kb, ke, wave, red, out);
+      gram_tile<P, 2, 2, false>(A, B, ma_p, mb_p, mb, i0, j0, n, kb, ke, wave, red, out);
   } else if (fa) {
-    gram_tile<P, 2, 1, false>(A, B, ma, mb, i0, j0, n, kb, ke, wave, red, out);
+    gram_tile<P, 2, 1, false>(A, B, ma_p, mb_p, mb, i0, j0, n, kb, ke, wave, red, out);
   } else if (fb) {
-    gram_tile<P, 1, 2, false>(A, B, ma, mb, i0, j0, n, kb, ke, wave, red, out);
+    gram_tile<P, 1, 2, false>(A, B, ma_p, mb_p, mb, i0, j0, n, kb, ke, wave, red, out);
   } else {
-    gram_tile<P, 1, 1, false>(A, B, ma, mb, i0, j0, n, kb, ke, wave, red, out);
+    gram_tile<P, 1, 1, false>(A, B, ma_p, mb_p, mb, i0, j0, n, kb, ke, wave, red, out);
   }
 }
 
@@ -485,12 +504,16 @@ __global__ void zero_double2_kernel(double2* __restrict__ out, long long count) 
 // Hermitian — entry (i, j) with i > j is the conjugate of (j, i) and the
 // diagonal is real.
 __global__ void sum_partials_kernel(const double2* __restrict__ part, int nsplit, long long count,
-                                    int mb, int herm, double2* __restrict__ out) {
+                                    int mb, int herm, ProbDims pd, double2* __restrict__ out) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= count) return;
   part += (long long)blockIdx.y * nsplit * count;
   out += (long long)blockIdx.y * count;
   const int i = (int)(e / mb), j = (int)(e - (long long)i * mb);
+  if (pd.ragged && (i >= pd.ma[blockIdx.y] || j >= pd.mb[blockIdx.y])) {  // past this problem's edge
+    out[e] = make_double2(0.0, 0.0);
+    return;
+  }
   if (herm && i > j) return;  // written by the thread of (j, i)
   // four independent chains (split s into chain s % 4 while four remain),
   // then (c0 + c1) + (c2 + c3): a fixed order that depends on nsplit only
@@ -589,7 +612,14 @@ template <int NA>
 __global__ __launch_bounds__(256, 2) void combine_mfma_kernel(const float2* __restrict__ Call, int m_in,
                                                               int m_out, CombinePtrs cp, long long n,
                                                               float alpha, int vec) {
+  // C keeps the launch's m_in × m_out per problem (row stride m_out); a
+  // ragged problem uses its own first m_in_p rows and m_out_p columns
   const float2* __restrict__ C = Call + (long long)blockIdx.z * m_in * m_out;
+  const int ldc = m_out;
+  if (cp.ragged) {
+    m_in = cp.m_in_p[blockIdx.z];
+    m_out = cp.m_out_p[blockIdx.z];
+  }
   const float2* S = cp.s[blockIdx.z];
   const float2* X = cp.x[blockIdx.z];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -599,6 +629,7 @@ __global__ __launch_bounds__(256, 2) void combine_mfma_kernel(const float2* __re
   const bool wfull = vec && c0 + 64 <= n;  // wave-uniform: all 64 columns, 16-byte access
   const bool full = vec && cl + 3 < n;
   const int i0 = blockIdx.y * 16 * NA;
+  if (i0 >= m_out) return;  // block-uniform: a row tile past this problem's outputs
   // C column of each A row; rows past m_out read column m_out − 1 and count as zero
   int ic[NA];
   bool iok[NA];
@@ -624,7 +655,7 @@ __global__ __launch_bounds__(256, 2) void combine_mfma_kernel(const float2* __re
     sv[0] = *reinterpret_cast<const float4*>(row + cl);
     sv[1] = *reinterpret_cast<const float4*>(row + cl + 2);
 #pragma unroll
-    for (int a = 0; a < NA; ++a) cv[a] = C[(long long)j * m_out + ic[a]];
+    for (int a = 0; a < NA; ++a) cv[a] = C[(long long)j * ldc + ic[a]];
   };
   auto fetch = [&](int j0, float4 (&sv)[2], float2 (&cv)[NA]) {
     const int j = j0 + g;
@@ -644,7 +675,7 @@ __global__ __launch_bounds__(256, 2) void combine_mfma_kernel(const float2* __re
       sv[1] = make_float4(e[2].x, e[2].y, e[3].x, e[3].y);
     }
 #pragma unroll
-    for (int a = 0; a < NA; ++a) cv[a] = C[(long long)j * m_out + ic[a]];
+    for (int a = 0; a < NA; ++a) cv[a] = C[(long long)j * ldc + ic[a]];
   };
   auto step = [&](const float4 (&sv)[2], const float2 (&cv_)[NA]) {
     const float sr[4] = {sv[0].x, sv[0].z, sv[1].x, sv[1].z};

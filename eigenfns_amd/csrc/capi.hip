@@ -716,6 +716,7 @@ static efn::VecMap single_map(const efn_ctx* c, int m) {
   vm.in_off[0] = 0;
   vm.out_off[0] = 0;
   vm.nrow = m > 0 ? m : 1;
+  vm.nslot = 1;
   return vm;
 }
 
@@ -884,10 +885,53 @@ static int build_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void*
     rq.vm.out_off[i] = out_off[i];
   }
   rq.vm.nrow = nrow;
+  rq.vm.nslot = nslot;
   rq.in = (const float2*)in_base;
   rq.m = nslot * nrow;
   rq.ea.out = (float2*)out_base;
   return 0;
+}
+
+// ragged slots: slot i holds rows[i] vectors (0 allowed); its launch vectors
+// follow those of slot i − 1.  Returns 1 when there is nothing to do.
+static int build_ragged(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
+                        const int64_t* in_off, void* out_base, const int64_t* out_off, const int* rows,
+                        OpRequest& rq) {
+  if (!ws) return fail(EFN_EINVAL, "workspace ctx is NULL");
+  if (ws->chunk <= 0) return fail(EFN_EINVAL, "workspace ctx was created without a workspace");
+  if (nslot < 0 || nslot > efn::kMaxSlots)
+    return fail(EFN_EINVAL, "nslot must be in [0, " + std::to_string(efn::kMaxSlots) + "]");
+  if (nslot == 0) return 1;
+  if (!ctxs || !in_off || !out_off || !rows || !in_base || !out_base) return fail(EFN_EINVAL, "NULL pointer");
+  memset(&rq.vm, 0, sizeof(rq.vm));
+  long long total = 0;
+  for (int i = 0; i < nslot; ++i) {
+    if (!ctxs[i]) return fail(EFN_EINVAL, "NULL ctx in slot list");
+    if (ctxs[i]->G != ws->G) return fail(EFN_EINVAL, "all slots must share the workspace grid size");
+    if (in_off[i] < 0 || out_off[i] < 0 || rows[i] < 0) return fail(EFN_EINVAL, "negative slot offset or rows");
+    rq.vm.desc[i] = ctxs[i]->d_desc;
+    rq.vm.in_off[i] = in_off[i];
+    rq.vm.out_off[i] = out_off[i];
+    rq.vm.row0[i] = (int)total;
+    total += rows[i];
+    if (total >= (1LL << 30)) return fail(EFN_EINVAL, "too many vectors in one launch");
+  }
+  rq.vm.row0[nslot] = (int)total;
+  rq.vm.nrow = 0;
+  rq.vm.nslot = nslot;
+  if (total == 0) return 1;
+  rq.in = (const float2*)in_base;
+  rq.m = (int)total;
+  rq.ea.out = (float2*)out_base;
+  return 0;
+}
+
+static bool ragged_overlap(int nslot, const int64_t* in_off, const int64_t* out_off, const int* rows) {
+  for (int i = 0; i < nslot; ++i)
+    for (int j = 0; j < nslot; ++j)
+      if (rows[i] > 0 && rows[j] > 0 && in_off[i] < out_off[j] + rows[j] && out_off[j] < in_off[i] + rows[i])
+        return true;
+  return false;
 }
 
 int efn_theta_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
@@ -923,6 +967,36 @@ int efn_precondition_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const v
         if (in_off[i] < out_off[j] + nrow && out_off[j] < in_off[i] + nrow)
           return fail(EFN_EINVAL, "input and output rows overlap");
   }
+  rq.ea.alpha = 1.f;
+  rq.ea.beta = 0.f;
+  return ws->run(ws, rq, (hipStream_t)stream);
+}
+
+int efn_theta_ragged(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
+                     const int64_t* in_off, void* out_base, const int64_t* out_off, const int* rows,
+                     float alpha, float beta, void* stream) {
+  OpRequest rq{};
+  rq.kind = efn::kTheta;
+  rq.epi = efn::kStore;
+  const int rc = build_ragged(ws, nslot, ctxs, in_base, in_off, out_base, out_off, rows, rq);
+  if (rc) return rc > 0 ? EFN_OK : rc;
+  if (in_base == out_base && ragged_overlap(nslot, in_off, out_off, rows))
+    return fail(EFN_EINVAL, "input and output rows overlap");
+  rq.ea.alpha = alpha;
+  rq.ea.beta = beta;
+  return ws->run(ws, rq, (hipStream_t)stream);
+}
+
+int efn_precondition_ragged(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
+                            const int64_t* in_off, void* out_base, const int64_t* out_off, const int* rows,
+                            void* stream) {
+  OpRequest rq{};
+  rq.kind = efn::kPrecond;
+  rq.epi = efn::kStore;
+  const int rc = build_ragged(ws, nslot, ctxs, in_base, in_off, out_base, out_off, rows, rq);
+  if (rc) return rc > 0 ? EFN_OK : rc;
+  if (in_base == out_base && ragged_overlap(nslot, in_off, out_off, rows))
+    return fail(EFN_EINVAL, "input and output rows overlap");
   rq.ea.alpha = 1.f;
   rq.ea.beta = 0.f;
   return ws->run(ws, rq, (hipStream_t)stream);
@@ -982,13 +1056,28 @@ size_t efn_gram_multi_work_bytes(int nprob, int ma, int mb, int64_t n) {
 }  // extern "C"
 
 namespace {
+// ma_p / mb_p (both or neither): per-problem sizes of a ragged launch, at most
+// ma / mb; the output keeps ma × mb per problem, zero outside each problem's own.
 int gram_multi_impl(int nprob, const void* const* A, const void* const* B, int ma, int mb, int64_t n,
-                    void* out, void* work, size_t work_bytes, bool upper, void* stream) {
+                    void* out, void* work, size_t work_bytes, bool upper, void* stream,
+                    const int* ma_p = nullptr, const int* mb_p = nullptr) {
   if (nprob < 0 || nprob > efn::kMaxProb)
     return fail(EFN_EINVAL, "nprob must be in [0, " + std::to_string(efn::kMaxProb) + "]");
   if (ma < 0 || mb < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
   if (nprob == 0 || ma == 0 || mb == 0) return EFN_OK;
   if (!A || !B || !out) return fail(EFN_EINVAL, "NULL pointer");
+  efn::ProbDims pd;
+  memset(&pd, 0, sizeof(pd));
+  if (ma_p) {
+    pd.ragged = 1;
+    for (int i = 0; i < nprob; ++i) {
+      if (ma_p[i] < 0 || mb_p[i] < 0 || ma_p[i] > ma || mb_p[i] > mb)
+        return fail(EFN_EINVAL, "ragged sizes must lie in [0, launch size]");
+      if (upper && ma_p[i] != mb_p[i]) return fail(EFN_EINVAL, "a Hermitian product needs square blocks");
+      pd.ma[i] = (short)ma_p[i];
+      pd.mb[i] = (short)mb_p[i];
+    }
+  }
   hipStream_t s = (hipStream_t)stream;
   if (n == 0) {
     const long long cnt = (long long)nprob * ma * mb;
@@ -1000,7 +1089,8 @@ int gram_multi_impl(int nprob, const void* const* A, const void* const* B, int m
   efn::PairPtrs pp;
   memset(&pp, 0, sizeof(pp));
   for (int i = 0; i < nprob; ++i) {
-    if (!A[i] || !B[i]) return fail(EFN_EINVAL, "NULL operand pointer");
+    if ((!A[i] || !B[i]) && !(pd.ragged && (pd.ma[i] == 0 || pd.mb[i] == 0)))
+      return fail(EFN_EINVAL, "NULL operand pointer");
     pp.a[i] = (const float2*)A[i];
     pp.b[i] = (const float2*)B[i];
   }
@@ -1012,7 +1102,11 @@ int gram_multi_impl(int nprob, const void* const* A, const void* const* B, int m
   bool aligned = (n % 2) == 0;
   for (int i = 0; i < nprob; ++i)
     aligned = aligned && ((uintptr_t)A[i] % 16 == 0) && ((uintptr_t)B[i] % 16 == 0);
-  const int engine = aligned ? g_gram_engine.load() : EFN_GRAM_VALU_F64;
+  int engine = aligned ? g_gram_engine.load() : EFN_GRAM_VALU_F64;
+  if (pd.ragged) {  // per-problem sizes are implemented by the matrix-core engines
+    if (!aligned) return fail(EFN_EUNSUPPORTED, "ragged Gram needs 16-byte aligned blocks of even length");
+    if (engine == EFN_GRAM_VALU_F64) engine = EFN_GRAM_MFMA_F64;
+  }
   const int edge = engine == EFN_GRAM_VALU_F64 ? efn::kGT : efn::kMWaveTile;
   const int ti = (ma + edge - 1) / edge, tj = (mb + edge - 1) / edge;
   const int ntile = upper ? ti * (ti + 1) / 2 : ti * tj;
@@ -1023,14 +1117,14 @@ int gram_multi_impl(int nprob, const void* const* A, const void* const* B, int m
                        kper, (double2*)work);
   else if (engine == EFN_GRAM_MFMA_F64)
     hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF64>, dim3((unsigned)(ns * ntile), 1, (unsigned)nprob),
-                       dim3(256), 0, s, pp, ma, mb, tiles_j, ntile, ns, (long long)n, kper, (double2*)work);
+                       dim3(256), 0, s, pp, pd, ma, mb, tiles_j, ntile, ns, (long long)n, kper, (double2*)work);
   else
     hipLaunchKernelGGL(efn::gram_mfma_kernel<efn::GramF32>, dim3((unsigned)(ns * ntile), 1, (unsigned)nprob),
-                       dim3(256), 0, s, pp, ma, mb, tiles_j, ntile, ns, (long long)n, kper, (double2*)work);
+                       dim3(256), 0, s, pp, pd, ma, mb, tiles_j, ntile, ns, (long long)n, kper, (double2*)work);
   EFN_LAUNCH_CHECK();
   const long long count = (long long)ma * mb;
   hipLaunchKernelGGL(efn::sum_partials_kernel, dim3((unsigned)((count + 255) / 256), (unsigned)nprob),
-                     dim3(256), 0, s, (const double2*)work, ns, count, mb, upper ? 1 : 0, (double2*)out);
+                     dim3(256), 0, s, (const double2*)work, ns, count, mb, upper ? 1 : 0, pd, (double2*)out);
   EFN_LAUNCH_CHECK();
   return EFN_OK;
 }
@@ -1051,6 +1145,20 @@ int efn_gram_multi_herm(int nprob, const void* const* A, const void* const* B, i
 int efn_gram(const void* A, int ma, const void* B, int mb, int64_t n, void* out, void* work,
              size_t work_bytes, void* stream) {
   return gram_multi_impl(1, &A, &B, ma, mb, n, out, work, work_bytes, false, stream);
+}
+
+int efn_gram_ragged(int nprob, const void* const* A, const void* const* B, const int* ma, const int* mb,
+                    int64_t n, int hermitian, void* out, void* work, size_t work_bytes, void* stream) {
+  if (nprob < 0 || nprob > efn::kMaxProb)
+    return fail(EFN_EINVAL, "nprob must be in [0, " + std::to_string(efn::kMaxProb) + "]");
+  if (nprob == 0) return EFN_OK;
+  if (!ma || !mb) return fail(EFN_EINVAL, "NULL size list");
+  int mam = 0, mbm = 0;
+  for (int i = 0; i < nprob; ++i) {
+    mam = std::max(mam, ma[i]);
+    mbm = std::max(mbm, mb[i]);
+  }
+  return gram_multi_impl(nprob, A, B, mam, mbm, n, out, work, work_bytes, hermitian != 0, stream, ma, mb);
 }
 
 }  // extern "C"
@@ -1104,19 +1212,38 @@ int efn_gram_noalloc(const void* A, int ma, const void* B, int mb, int64_t n, vo
 }
 
 namespace {
+// min_p / mout_p / split_p (all or none): per-problem sizes of a ragged
+// launch, at most m_in / m_out; C keeps m_in × m_out per problem.
 int combine_impl(int nprob, const void* C, const void* const* S, int m_in, int m_out, int64_t n,
                  const void* const* X, float alpha, void* const* out, void* const* out2, int split,
-                 void* stream) {
+                 void* stream, const int* min_p = nullptr, const int* mout_p = nullptr,
+                 const int* split_p = nullptr) {
   if (nprob < 0 || nprob > efn::kMaxProb)
     return fail(EFN_EINVAL, "nprob must be in [0, " + std::to_string(efn::kMaxProb) + "]");
   if (m_in < 0 || m_out < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
   if (nprob == 0 || m_out == 0 || n == 0) return EFN_OK;
   if (!out || (m_in > 0 && (!C || !S))) return fail(EFN_EINVAL, "NULL pointer");
   if (split < 0 || split > m_out) return fail(EFN_EINVAL, "split must be in [0, m_out]");
-  if (split < m_out && !out2) return fail(EFN_EINVAL, "NULL second output");
+  if (split < m_out && !out2 && !min_p) return fail(EFN_EINVAL, "NULL second output");
   efn::CombinePtrs cp;
   memset(&cp, 0, sizeof(cp));
   cp.split = split;
+  if (min_p) {
+    cp.ragged = 1;
+    for (int i = 0; i < nprob; ++i) {
+      if (min_p[i] < 0 || mout_p[i] < 0 || min_p[i] > m_in || mout_p[i] > m_out || split_p[i] < 0 ||
+          split_p[i] > mout_p[i])
+        return fail(EFN_EINVAL, "ragged sizes must lie in [0, launch size] and split in [0, m_out]");
+      if (split_p[i] < mout_p[i] && !out2) return fail(EFN_EINVAL, "NULL second output");
+      cp.m_in_p[i] = (short)min_p[i];
+      cp.m_out_p[i] = (short)mout_p[i];
+      cp.split_p[i] = (short)split_p[i];
+    }
+  }
+  // this problem's sizes
+  auto pin = [&](int i) { return min_p ? min_p[i] : m_in; };
+  auto pout = [&](int i) { return mout_p ? mout_p[i] : m_out; };
+  auto psplit = [&](int i) { return split_p ? split_p[i] : split; };
   auto al16 = [](const void* q) { return ((uintptr_t)q & 15) == 0; };
   // [p, p + rows·n) of complex64 against S's [S, S + m_in·n)
   auto overlaps_s = [&](const void* p, long long rows, const void* s) {
@@ -1127,16 +1254,18 @@ int combine_impl(int nprob, const void* C, const void* const* S, int m_in, int m
   };
   int vec = (n & 1) == 0;
   for (int i = 0; i < nprob; ++i) {
-    if ((split > 0 && !out[i]) || (split < m_out && !out2[i]) || (m_in > 0 && !S[i]))
+    const int mi = pin(i), mo = pout(i), sp = psplit(i);
+    if (mo == 0) continue;  // nothing written for this problem
+    if ((sp > 0 && !out[i]) || (sp < mo && !out2[i]) || (mi > 0 && !S[i]))
       return fail(EFN_EINVAL, "NULL block pointer");
     // in place needs one row tile: every reader of a column is its writer, so
     // an output block that overlaps S anywhere (not only at S itself) is
     // limited to one row tile of outputs
-    if (m_in > 0 && m_out > kCombineInPlaceMax &&
-        (overlaps_s(split > 0 ? out[i] : nullptr, split, S[i]) ||
-         overlaps_s(out2 && split < m_out ? out2[i] : nullptr, m_out - split, S[i])))
+    if (mi > 0 && mo > kCombineInPlaceMax &&
+        (overlaps_s(sp > 0 ? out[i] : nullptr, sp, S[i]) ||
+         overlaps_s(out2 && sp < mo ? out2[i] : nullptr, mo - sp, S[i])))
       return fail(EFN_EINVAL, "outputs may overlap S only for m_out <= " + std::to_string(kCombineInPlaceMax));
-    cp.s[i] = m_in > 0 ? (const float2*)S[i] : nullptr;
+    cp.s[i] = mi > 0 ? (const float2*)S[i] : nullptr;
     cp.x[i] = X ? (const float2*)X[i] : nullptr;
     cp.out[i] = (float2*)out[i];
     cp.out2[i] = out2 ? (float2*)out2[i] : nullptr;
@@ -1148,8 +1277,8 @@ int combine_impl(int nprob, const void* C, const void* const* S, int m_in, int m
   const unsigned gx = (unsigned)((n + 255) / 256);
   const float2* Cc = (const float2*)C;
   // matrix cores: 64 columns per wave, 16·NA output rows per block row (one
-  // row tile up to 64 outputs, so S is read once)
-  if (g_combine_mfma.load() && m_out > 8) {
+  // row tile up to 64 outputs, so S is read once); ragged launches always
+  if ((g_combine_mfma.load() && m_out > 8) || cp.ragged) {
     if (m_out <= 16)
       hipLaunchKernelGGL((efn::combine_mfma_kernel<1>), dim3(gx, 1, nprob), dim3(256), 0, s, Cc, m_in,
                          m_out, cp, (long long)n, alpha, vec);
@@ -1191,6 +1320,21 @@ int efn_combine_multi(int nprob, const void* C, const void* const* S, int m_in, 
 int efn_combine_multi_split(int nprob, const void* C, const void* const* S, int m_in, int m_out,
                             int64_t n, void* const* out, void* const* out2, int split, void* stream) {
   return combine_impl(nprob, C, S, m_in, m_out, n, nullptr, 0.f, out, out2, split, stream);
+}
+
+int efn_combine_ragged(int nprob, const void* C, const void* const* S, const int* m_in, const int* m_out,
+                       int64_t n, const void* const* X, float alpha, void* const* out, void* const* out2,
+                       const int* split, void* stream) {
+  if (nprob < 0 || nprob > efn::kMaxProb)
+    return fail(EFN_EINVAL, "nprob must be in [0, " + std::to_string(efn::kMaxProb) + "]");
+  if (nprob == 0) return EFN_OK;
+  if (!m_in || !m_out || !split) return fail(EFN_EINVAL, "NULL size list");
+  int mim = 0, mom = 0;
+  for (int i = 0; i < nprob; ++i) {
+    mim = std::max(mim, m_in[i]);
+    mom = std::max(mom, m_out[i]);
+  }
+  return combine_impl(nprob, C, S, mim, mom, n, X, alpha, out, out2, mom, stream, m_in, m_out, split);
 }
 
 int efn_heev_batched(int nprob, int n, const void* A, double* w, void* V, int max_sweeps, void* stream) {

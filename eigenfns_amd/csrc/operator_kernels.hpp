@@ -69,16 +69,20 @@ struct OpDesc {
 };
 
 // Vector map of one launch: launch-local vector v belongs to slot v / nrow and
-// is row v % nrow of that slot; the slot's operator, input and output blocks
-// are given per slot (offsets in vectors from the launch's base pointers).
-// One slot is the plain single-operator call; several slots batch operators of
-// one grid size (k-points, structures, requests) into the same passes.
-constexpr int kMaxSlots = 32;
+// is row v % nrow of that slot (nrow > 0), or — ragged launches, nrow = 0 — to
+// the slot s with row0[s] ≤ v < row0[s + 1], row v − row0[s]; the slot's
+// operator, input and output blocks are given per slot (offsets in vectors
+// from the launch's base pointers).  One slot is the plain single-operator
+// call; several slots batch operators of one grid size (k-points, structures,
+// requests, of equal or different block widths) into the same passes.
+constexpr int kMaxSlots = 64;
 struct VecMap {
   const OpDesc* desc[kMaxSlots];
   long long in_off[kMaxSlots];
   long long out_off[kMaxSlots];
+  int row0[kMaxSlots + 1];
   int nrow;
+  int nslot;
 };
 
 struct VecRef {
@@ -87,9 +91,22 @@ struct VecRef {
 };
 
 __device__ __forceinline__ VecRef map_vector(const VecMap& vm, long long gv) {
-  // a launch covers fewer than 2³¹ vectors: 32-bit division
-  const int slot = int(gv) / vm.nrow;
-  const long long row = int(gv) - slot * vm.nrow;
+  // a launch covers fewer than 2³¹ vectors: 32-bit arithmetic
+  int slot;
+  long long row;
+  if (vm.nrow > 0) {
+    slot = int(gv) / vm.nrow;
+    row = int(gv) - slot * vm.nrow;
+  } else {  // binary search of the (block-uniform) vector in the row prefix table
+    int lo = 0, hi = vm.nslot - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (vm.row0[mid] <= int(gv)) lo = mid;
+      else hi = mid - 1;
+    }
+    slot = lo;
+    row = int(gv) - vm.row0[lo];
+  }
   VecRef r;
   r.desc = vm.desc[slot];
   r.in_vec = vm.in_off[slot] + row;

@@ -308,8 +308,8 @@ def apply_chunked(fn, X, rows: int):
     return torch.cat([fn(X[s:s + rows]) for s in range(0, X.shape[0], rows)], dim=0)
 
 
-# ---- multi-problem forms (one launch for up to 32 independent problems) ------
-MAX_PROBLEMS = 32
+# ---- multi-problem forms (one launch for up to 64 independent problems) ------
+MAX_PROBLEMS = 64
 
 
 def _ptrs(ts):
@@ -375,6 +375,62 @@ def combine_multi_split(C: torch.Tensor, Ss, outs, outs2, split: int) -> None:
     _lib.check(_lib.load().efn_combine_multi_split(nprob, C.data_ptr(), _ptrs(Ss), m_in, m_out, n,
                                                    _ptrs(outs), _ptrs(outs2), int(split),
                                                    stream_ptr()), "combine_multi_split")
+
+
+def gram_ragged(As, Bs, hermitian: bool = False) -> torch.Tensor:
+    """(nprob, max m_a, max m_b) complex128: A_pᴴ·B_p per pair, each problem
+    with its own row counts, in one launch; zero outside each problem's own
+    m_a × m_b (problems may be empty).  hermitian: square Hermitian products."""
+    nprob = len(As)
+    if nprob == 0 or nprob != len(Bs):
+        raise ValueError("gram_ragged: one A and one B per problem")
+    if nprob > MAX_PROBLEMS:
+        raise ValueError(f"gram_ragged: at most {MAX_PROBLEMS} problems per launch")
+    import ctypes as C
+    ma = [int(A.shape[0]) for A in As]
+    mb = [int(B.shape[0]) for B in Bs]
+    ref = next((A for A in As if A.shape[0]), As[0])
+    n = _flat_n(ref)
+    for A, B in zip(As, Bs):
+        if (A.shape[0] and (not A.is_contiguous() or _flat_n(A) != n)) or \
+                (B.shape[0] and (not B.is_contiguous() or _flat_n(B) != n)):
+            raise ValueError("gram_ragged: blocks must be contiguous with one row length")
+    mam, mbm = max(ma), max(mb)
+    out = torch.empty((nprob, mam, mbm), dtype=torch.complex128, device=ref.device)
+    if mam == 0 or mbm == 0:
+        return out
+    lib = _lib.load()
+    work = SCRATCH.get(lib.efn_gram_multi_work_bytes(nprob, mam, mbm, n), "gram")
+    _lib.check(lib.efn_gram_ragged(nprob, _ptrs([A if A.shape[0] else None for A in As]),
+                                   _ptrs([B if B.shape[0] else None for B in Bs]),
+                                   (C.c_int * nprob)(*ma), (C.c_int * nprob)(*mb), n, int(bool(hermitian)),
+                                   out.data_ptr(), work.data_ptr(), work.numel(), stream_ptr()),
+               "gram_ragged")
+    return out
+
+
+def combine_ragged(C: torch.Tensor, Ss, m_in, m_out, outs, outs2=None, split=None, Xs=None,
+                   alpha: float = 0.0) -> None:
+    """out_p (rows < split[p]) and out2_p (the rest) of alpha·X_p + C_pᵀ·S_p with
+    problem p's own m_in[p] rows of S and m_out[p] outputs; C is (nprob,
+    max m_in, max m_out), each problem reading its leading block."""
+    import ctypes as Ct
+    nprob = len(Ss)
+    if nprob > MAX_PROBLEMS:
+        raise ValueError(f"combine_ragged: at most {MAX_PROBLEMS} problems per launch")
+    C = C.to(device=outs[0].device, dtype=torch.complex64).contiguous()
+    if C.shape[1] < max(m_in) or C.shape[2] < max(m_out):
+        raise ValueError("combine_ragged: C smaller than the largest problem")
+    if C.shape[1] != max(m_in) or C.shape[2] != max(m_out):
+        C = C[:, :max(m_in), :max(m_out)].contiguous()
+    split = list(m_out) if split is None else list(split)
+    ref = next((S for S in Ss if S is not None and S.shape[0]), outs[0])
+    n = _flat_n(ref)
+    iv = lambda xs: (Ct.c_int * nprob)(*[int(x) for x in xs])
+    _lib.check(_lib.load().efn_combine_ragged(nprob, C.data_ptr(), _ptrs(Ss), iv(m_in), iv(m_out), n,
+                                              None if Xs is None else _ptrs(Xs), float(alpha), _ptrs(outs),
+                                              None if outs2 is None else _ptrs(outs2), iv(split),
+                                              stream_ptr()), "combine_ragged")
 
 
 def copy_multi(srcs, dsts) -> None:
@@ -510,6 +566,22 @@ def eigh_stack_host(a: np.ndarray, lowest: int | None = None):
     if lowest is not None and lowest < a.shape[-1]:
         w, V = w[:, :lowest], V[:, :, :lowest]
     return w, V
+
+
+def eigh_list_host(mats, lowest=None):
+    """[(w, V)] of FP64 Hermitian eigendecompositions of a list of numpy
+    matrices of any sizes (the thread pool of eigh_stack_host); `lowest`: one
+    count for all, or one per matrix, of the lowest eigenpairs to keep."""
+    mats = [(a + a.conj().T) / 2 for a in mats]
+    if len(mats) == 1:
+        res = [np.linalg.eigh(mats[0])]
+    else:
+        with one_blas_thread():
+            res = list(_pool().map(np.linalg.eigh, mats))
+    if lowest is None:
+        return res
+    lows = lowest if isinstance(lowest, (list, tuple)) else [lowest] * len(mats)
+    return [(w[:k], V[:, :k]) for (w, V), k in zip(res, lows)]
 
 
 def eigh_batched_device(A: torch.Tensor, lowest: int | None = None, max_sweeps: int = 0,

@@ -324,11 +324,14 @@ def theta_passes(grid_size: int) -> int:
     return rc
 
 
+MAX_SLOTS = 64   # operators per multi / ragged launch
+
+
 def _multi(kind, ws_op, ops, in_base, in_off, out_base, out_off, nrow, alpha=1.0, beta=0.0):
     import ctypes as C
     n = len(ops)
-    if n > 32:
-        raise ValueError("at most 32 operators per multi launch")
+    if n > MAX_SLOTS:
+        raise ValueError(f"at most {MAX_SLOTS} operators per multi launch")
     for op in ops:
         if op.grid_size != ws_op.grid_size:
             raise ValueError("multi launch: all operators must share the grid size")
@@ -357,3 +360,39 @@ def theta_multi(ws_op, ops, in_base, in_off, out_base, out_off, nrow, alpha=1.0,
 def precondition_multi(ws_op, ops, in_base, in_off, out_base, out_off, nrow):
     """Multi-operator transverse-projection preconditioner (see theta_multi)."""
     _multi("precond", ws_op, ops, in_base, in_off, out_base, out_off, nrow)
+
+
+def _ragged(kind, ws_op, ops, in_base, in_off, out_base, out_off, rows, alpha=1.0, beta=0.0):
+    import ctypes as C
+    n = len(ops)
+    if n > MAX_SLOTS:
+        raise ValueError(f"at most {MAX_SLOTS} operators per ragged launch")
+    if not (len(in_off) == len(out_off) == len(rows) == n):
+        raise ValueError("ragged launch: one offset pair and row count per operator")
+    for op in ops:
+        if op.grid_size != ws_op.grid_size:
+            raise ValueError("ragged launch: all operators must share the grid size")
+    ctxs = (C.c_void_p * n)(*[op._ctx.value for op in ops])
+    io = (C.c_int64 * n)(*[int(x) for x in in_off])
+    oo = (C.c_int64 * n)(*[int(x) for x in out_off])
+    rw = (C.c_int * n)(*[int(x) for x in rows])
+    lib = _lib.load()
+    if kind == "theta":
+        rc = lib.efn_theta_ragged(ws_op._ctx, n, ctxs, in_base.data_ptr(), io, out_base.data_ptr(),
+                                  oo, rw, float(alpha), float(beta), stream_ptr())
+    else:
+        rc = lib.efn_precondition_ragged(ws_op._ctx, n, ctxs, in_base.data_ptr(), io,
+                                         out_base.data_ptr(), oo, rw, stream_ptr())
+    _lib.check(rc, f"{kind}_ragged")
+
+
+def theta_ragged(ws_op, ops, in_base, in_off, out_base, out_off, rows, alpha=1.0, beta=0.0):
+    """theta_multi with a row count per operator: slot i maps rows in_off[i] ..
+    in_off[i]+rows[i]−1 of `in_base` to rows out_off[i] .. of `out_base`, so
+    one launch set covers blocks of different widths."""
+    _ragged("theta", ws_op, ops, in_base, in_off, out_base, out_off, rows, alpha, beta)
+
+
+def precondition_ragged(ws_op, ops, in_base, in_off, out_base, out_off, rows):
+    """Ragged multi-operator preconditioner (see theta_ragged)."""
+    _ragged("precond", ws_op, ops, in_base, in_off, out_base, out_off, rows)

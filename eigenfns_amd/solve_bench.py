@@ -2,7 +2,7 @@
 
     python -m eigenfns_amd.solve_bench c1             plain LOBPCG, 32³
     python -m eigenfns_amd.solve_bench c2             plain LOBPCG, 128³, 128 bands (+ 16³ vs dense FP64)
-    python -m eigenfns_amd.solve_bench c3 [--n-k N]   64 k-points: batched engines vs one at a time
+    python -m eigenfns_amd.solve_bench c3 [--n-k N]   64 k-points: one ragged engine, width buckets, one at a time
     python -m eigenfns_amd.solve_bench c4kpm          KPM moments at 256³, degree 12000, 16 probes
     python -m eigenfns_amd.solve_bench c4cheb         ChebSI build stage at 192³ (m = 104, 3000 × 2)
     python -m eigenfns_amd.solve_bench c5 [--n N --rate R]   Poisson request stream, batched vs plain
@@ -98,12 +98,15 @@ def run_c2(ortho: str = "cholqr"):
     return res
 
 
-def run_c3(n_k: int | None = None, plain: bool = True, bucket_step: int = 4):
-    """Every k-point is a request; widths are rounded up to a multiple of
-    `bucket_step` and each bucket is one engine (own host thread and stream)
-    whose guard is the bucket minus its largest nev (≥ 4), so each request
-    locks its nev = width − 4 bands in one block.  The plain baseline solves
-    each k-point alone with block = width, guard 4."""
+def run_c3(n_k: int | None = None, plain: bool = True, bucket_step: int = 4, mode: str = "both",
+           ragged_slots: int = 64):
+    """Every k-point is a request.  "ragged": one engine holds every k-point
+    at its own width (guard 4, as the plain solver) and each kernel of an
+    iteration is one ragged launch over all of them.  "buckets": widths are
+    rounded up to a multiple of `bucket_step` and each bucket is one engine
+    (own host thread and stream) whose guard is the bucket minus its largest
+    nev (≥ 4), so each request locks its nev = width − 4 bands in one block.
+    The plain baseline solves each k-point alone with block = width, guard 4."""
     from .batched import BatchedLOBPCG, Request
     from .operator import MaxwellOperator
     from .solver import lobpcg_blocks
@@ -152,15 +155,53 @@ def run_c3(n_k: int | None = None, plain: bool = True, bucket_step: int = 4):
             raise errors[0]
         return done
 
-    done, tb = _sync_time(batched)
-    res = {"case": "C3", "grid": c["grid"], "k_points": len(probs),
-           "buckets": {b: len(v) for b, v in buckets.items()},
-           "batched_seconds": round(tb, 2), "batched_k_per_s": round(len(probs) / tb, 3)}
+    def ragged():
+        # one engine: every k-point its own width and the plain solver's guard,
+        # all of them in the same ragged launches (up to 64 slots)
+        done = {}
+        wmax = max(w for _, w, _ in probs)
+        eng = BatchedLOBPCG(c["grid"], wmax, c["guard"], min(len(probs), ragged_slots),
+                            max(nev for *_, nev in probs), c["tol"], 400)
+        pending = list(enumerate(probs))
+        with one_blas_thread():
+            while pending or eng.active():
+                while pending and eng.free_slots():
+                    j, (k, w, nev) = pending.pop(0)
+                    eng.admit(Request(MaxwellOperator(eps, L, k, max_chunk=-1), nev, seed=j, rid=j,
+                                      m=w, guard=c["guard"]))
+                for req in eng.step():
+                    done[req.rid] = req.vals
+                    req.op = None
+                    _progress(f"C3 ragged: k-point {req.rid} done ({len(done)}/{len(probs)})")
+        steps = eng.steps
+        del eng
+        return done, steps
+
+    from .linalg import one_blas_thread
+    res = {"case": "C3", "grid": c["grid"], "k_points": len(probs)}
     errs = []
-    for j in range(len(probs)):
-        g = _golden(f"c3_k{j}")
-        if g is not None and j in done:
-            errs.append(_rel_err(done[j], g))
+    if mode in ("ragged", "both"):
+        (done_r, steps), tr = _sync_time(ragged)
+        res.update({"ragged_seconds": round(tr, 2), "ragged_k_per_s": round(len(probs) / tr, 3),
+                    "ragged_slots": min(len(probs), ragged_slots), "ragged_engine_steps": steps})
+        for j in range(len(probs)):
+            g = _golden(f"c3_k{j}")
+            if g is not None and j in done_r:
+                errs.append(_rel_err(done_r[j], g))
+        done = done_r
+        torch.cuda.empty_cache()
+    if mode in ("buckets", "both"):
+        done, tb = _sync_time(batched)
+        res.update({"buckets": {b: len(v) for b, v in buckets.items()},
+                    "batched_seconds": round(tb, 2), "batched_k_per_s": round(len(probs) / tb, 3)})
+        for j in range(len(probs)):
+            g = _golden(f"c3_k{j}")
+            if g is not None and j in done:
+                errs.append(_rel_err(done[j], g))
+        if mode == "both":
+            res["max_rel_diff_ragged_vs_buckets"] = max(_rel_err(done_r[j], done[j]) for j in done)
+        torch.cuda.empty_cache()
+    tb = res.get("batched_seconds") or res.get("ragged_seconds")
     res["max_rel_err_vs_fp64_oracle"] = max(errs) if errs else None
     res["oracle_checked"] = len(errs)
     if plain:
@@ -177,6 +218,7 @@ def run_c3(n_k: int | None = None, plain: bool = True, bucket_step: int = 4):
         pl, tp = _sync_time(one_at_a_time)
         res.update({"plain_seconds": round(tp, 2), "plain_k_per_s": round(len(probs) / tp, 3),
                     "speedup": round(tp / tb, 2),
+                    "speedup_ragged": round(tp / res["ragged_seconds"], 2) if "ragged_seconds" in res else None,
                     "max_rel_diff_batched_vs_plain": max(_rel_err(done[j], pl[j]) for j in pl)})
     return res
 
@@ -265,6 +307,9 @@ def main(argv=None):
     ap.add_argument("--n-k", type=int, default=None)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--bucket-step", type=int, default=4)
+    ap.add_argument("--c3-mode", default="both", choices=["ragged", "buckets", "both"],
+                    help="C3: one ragged engine, width buckets, or both")
+    ap.add_argument("--slots", type=int, default=64, help="C3: slots of the ragged engine")
     ap.add_argument("--degree", type=int, default=None)
     ap.add_argument("--outers", type=int, default=None)
     ap.add_argument("--n", type=int, default=200)
@@ -282,7 +327,8 @@ def main(argv=None):
     elif args.case == "c2":
         out = run_c2(args.ortho)
     elif args.case == "c3":
-        out = run_c3(args.n_k, plain=not args.no_plain, bucket_step=args.bucket_step)
+        out = run_c3(args.n_k, plain=not args.no_plain, bucket_step=args.bucket_step, mode=args.c3_mode,
+                     ragged_slots=args.slots)
     elif args.case == "c4kpm":
         out = run_c4kpm(args.degree)
     elif args.case == "c4cheb":

@@ -69,7 +69,7 @@ int efn_ctx_basis(const efn_ctx* ctx, float* kn_dev, float* t_dev, void* stream)
 int efn_theta(efn_ctx* ctx, const void* in_c64, void* out_c64, int m, float alpha, float beta,
               void* stream);
 
-/* Multi-operator Θ: one launch set over `nslot` (≤ 32) operators of the same
+/* Multi-operator Θ: one launch set over `nslot` (≤ 64) operators of the same
  * grid size — Bloch k-points, structures or independent requests.  Slot i
  * applies ctxs[i] to rows in_off[i] .. in_off[i]+nrow−1 of in_base and writes
  * rows out_off[i] .. of out_base (offsets in vectors of 2·G³ complex64).  The
@@ -82,6 +82,19 @@ int efn_theta_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in
 int efn_precondition_multi(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
                            const int64_t* in_off, void* out_base, const int64_t* out_off, int nrow,
                            void* stream);
+
+/* Ragged multi-operator Θ / preconditioner: as efn_theta_multi and
+ * efn_precondition_multi, but slot i holds rows[i] vectors (block widths may
+ * differ, 0 allowed), so one launch set covers problems of different block
+ * widths — e.g. every Bloch k-point of a band-structure path at its own
+ * width.  Replaces a loop of operator.py:153-155 / :157-161 calls over
+ * operators and widths; no reference counterpart as a single call. */
+int efn_theta_ragged(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
+                     const int64_t* in_off, void* out_base, const int64_t* out_off, const int* rows,
+                     float alpha, float beta, void* stream);
+int efn_precondition_ragged(efn_ctx* ws, int nslot, efn_ctx* const* ctxs, const void* in_base,
+                            const int64_t* in_off, void* out_base, const int64_t* out_off,
+                            const int* rows, void* stream);
 
 /* Real-space fields of m vectors (post-solve; operator.py:169-184 and the
  * energy density of run_modes.py:116-124 / run_interior.py:252-262):
@@ -181,12 +194,22 @@ int efn_gram(const void* A_c64, int ma, const void* B_c64, int mb, int64_t n, vo
 int efn_gram_noalloc(const void* A_c64, int ma, const void* B_c64, int mb, int64_t n,
                      void* out_c128, void* stream);
 
-/* Gram blocks of `nprob` (≤ 32) independent problems in one launch:
+/* Gram blocks of `nprob` (≤ 64) independent problems in one launch:
  * out[p] = A_pᴴ·B_p (ma×mb complex128 each, contiguous), A_p / B_p device
  * pointers given in host arrays.  solver.py:35-43 per problem. */
 size_t efn_gram_multi_work_bytes(int nprob, int ma, int mb, int64_t n);
 int efn_gram_multi(int nprob, const void* const* A_c64, const void* const* B_c64, int ma, int mb,
                    int64_t n, void* out_c128, void* work, size_t work_bytes, void* stream);
+
+/* Ragged Gram: problem p is ma[p] × mb[p] (the blocks may differ in row
+ * count).  out holds nprob blocks of max(ma) × max(mb) complex128, zero
+ * outside each problem's own ma[p] × mb[p]; work as efn_gram_multi_work_bytes
+ * at the largest sizes.  hermitian != 0: square problems known to be
+ * Hermitian (upper tiles computed, mirrored).  solver.py:35-43 per problem;
+ * runs on the matrix-core engines and needs 16-byte aligned blocks. */
+int efn_gram_ragged(int nprob, const void* const* A_c64, const void* const* B_c64, const int* ma,
+                    const int* mb, int64_t n, int hermitian, void* out_c128, void* work,
+                    size_t work_bytes, void* stream);
 
 /* Hermitian form of efn_gram_multi for products known to be Hermitian
  * (A_pᴴA_p, or A_pᴴ·Θ·A_p that the caller symmetrises anyway): only the upper
@@ -263,6 +286,16 @@ int efn_combine_multi(int nprob, const void* C_c64, const void* const* S_c64, in
 int efn_combine_multi_split(int nprob, const void* C_c64, const void* const* S_c64, int m_in,
                             int m_out, int64_t n, void* const* out_c64, void* const* out2_c64,
                             int split, void* stream);
+
+/* Ragged combine: problem p has m_in[p] rows of S, m_out[p] outputs and its
+ * own split (rows < split[p] to out[p], the rest to out2[p], which may be
+ * NULL when every split equals m_out); C holds nprob blocks of
+ * max(m_in) × max(m_out) complex64 of which each problem reads its leading
+ * m_in[p] × m_out[p].  X / alpha as efn_combine_multi.  solver.py:298-307
+ * per problem; matrix-core engine. */
+int efn_combine_ragged(int nprob, const void* C_c64, const void* const* S_c64, const int* m_in,
+                       const int* m_out, int64_t n, const void* const* X_c64, float alpha,
+                       void* const* out_c64, void* const* out2_c64, const int* split, void* stream);
 
 /* Row reductions in FP64 (solver.py:228-231, interior.py:75-82):
  *   op 0: out[i] = (Σ_k |A[i][k]|², 0)      op 1: out[i] = Σ_k conj(A[i][k])·B[i][k]

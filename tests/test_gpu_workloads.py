@@ -91,6 +91,31 @@ def test_c3_kpoints_batched_match_fp64_oracle(golden):
         assert err < TOL
 
 
+def test_c3_kpoints_in_one_ragged_engine_match_fp64_oracle(golden):
+    """The golden k-points at their own widths (guard 4, the plain solver's)
+    beside four other k-points of the path, all in one ragged engine."""
+    from eigenfns_amd.batched import Request, solve_batched
+    from eigenfns_amd.operator import MaxwellOperator
+    z, meta = golden
+    keys = sorted(k for k in meta if k.startswith("c3_k"))
+    if not keys:
+        pytest.skip("no C3 k-points in solve_golden.npz")
+    eps, L = wl.network(wl.C3["n_points"], wl.C3["grid"])
+    probs = wl.c3_problems()
+    gold = {meta[key]["index"] for key in keys}
+    idx = sorted(gold | {10, 25, 40, 55})
+    reqs = [Request(MaxwellOperator(eps, L, probs[j][0], max_chunk=-1), probs[j][2], seed=j, rid=j,
+                    m=probs[j][1], guard=wl.C3["guard"]) for j in idx]
+    done = solve_batched(reqs, wl.C3["grid"], m=max(r.m for r in reqs), guard=wl.C3["guard"],
+                         nslots=len(reqs), tol=wl.C3["tol"])
+    assert len(done) == len(reqs)
+    for req in done:
+        if req.rid in gold:
+            err = _rel(req.vals, z[f"c3_k{req.rid}"])
+            print(f"C3 k-point {req.rid} (width {req.m}) in the ragged engine: max rel err {err:.2e}")
+            assert err < TOL
+
+
 def test_serving_sample_matches_fp64_oracle(golden):
     from eigenfns_amd.serving import StructureCache, serve_batched
     z, meta = golden
