@@ -1539,6 +1539,58 @@ int rocblas_handle_for(rocblas_handle* out) {
   *out = it->second;
   return EFN_OK;
 }
+// rocSOLVER's own scratch for the batched eigensolve, owned by this library:
+// sized once per (n, nprob) with rocBLAS's size query and handed to the handle
+// as a user workspace, so a call never allocates — which is what lets a
+// stream capture (hipGraph) record it.  A size not seen before is queried and
+// allocated outside any capture (the first, eager call).
+struct RocWorkspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  std::map<std::pair<int, int>, size_t> need;  // (n, nprob) → bytes
+};
+
+int rocsolver_workspace_for(rocblas_handle h, int n, int nprob, hipStream_t s, double2* V, double* w,
+                            double* E, rocblas_int* info) {
+  thread_local std::map<int, RocWorkspace> spaces;
+  int dev = 0;
+  EFN_HIP_CHECK(hipGetDevice(&dev));
+  RocWorkspace& ws = spaces[dev];
+  auto it = ws.need.find({n, nprob});
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  EFN_HIP_CHECK(hipStreamIsCapturing(s, &cap));
+  if (it == ws.need.end()) {
+    if (cap != hipStreamCaptureStatusNone)
+      return fail(EFN_EINVAL, "heevd_batched: size " + std::to_string(n) + " x " + std::to_string(nprob) +
+                                  " first seen under stream capture; run it once uncaptured");
+    size_t bytes = 0;
+    if (rocblas_start_device_memory_size_query(h) != rocblas_status_success)
+      return fail(EFN_EHIP, "rocblas_start_device_memory_size_query failed");
+    (void)rocsolver_zheevd_strided_batched(h, rocblas_evect_original, rocblas_fill_lower, n,
+                                           (rocblas_double_complex*)V, n, (rocblas_stride)n * n, w,
+                                           (rocblas_stride)n, E, (rocblas_stride)n, info, nprob);
+    if (rocblas_stop_device_memory_size_query(h, &bytes) != rocblas_status_success)
+      return fail(EFN_EHIP, "rocblas_stop_device_memory_size_query failed");
+    it = ws.need.emplace(std::make_pair(n, nprob), bytes).first;
+  }
+  if (it->second > ws.bytes) {
+    if (cap != hipStreamCaptureStatusNone)
+      return fail(EFN_EINVAL, "heevd_batched: workspace must grow under stream capture; run it once uncaptured");
+    EFN_HIP_CHECK(hipStreamSynchronize(s));
+    if (ws.ptr) (void)hipFree(ws.ptr);
+    ws.ptr = nullptr;
+    ws.bytes = 0;
+    const size_t bytes = ((it->second + 4095) / 4096) * 4096;
+    if (hipMalloc(&ws.ptr, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(EFN_ENOMEM, "rocSOLVER workspace of " + std::to_string(bytes) + " bytes");
+    }
+    ws.bytes = bytes;
+  }
+  if (ws.bytes && rocblas_set_workspace(h, ws.ptr, ws.bytes) != rocblas_status_success)
+    return fail(EFN_EHIP, "rocblas_set_workspace failed");
+  return EFN_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -1566,6 +1618,7 @@ int efn_heevd_batched(int nprob, int n, const void* A, double* w, void* V, void*
   const size_t i_bytes = ((3 * (size_t)nprob * sizeof(int) + 255) / 256) * 256;
   double* E = (double*)work;
   rocblas_int* info = (rocblas_int*)((char*)work + e_bytes);
+  if ((rc = rocsolver_workspace_for(h, n, nprob, s, (double2*)V, w, E, info))) return rc;
   int* zero = (int*)(info + nprob);
   int* redo = zero + nprob;
   double2* T = (double2*)((char*)work + e_bytes + i_bytes);

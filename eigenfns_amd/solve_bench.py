@@ -99,7 +99,7 @@ def run_c2(ortho: str = "cholqr"):
 
 
 def run_c3(n_k: int | None = None, plain: bool = True, bucket_step: int = 4, mode: str = "both",
-           ragged_slots: int = 64):
+           ragged_slots: int = 64, ragged_engines: int = 2):
     """Every k-point is a request.  "ragged": one engine holds every k-point
     at its own width (guard 4, as the plain solver) and each kernel of an
     iteration is one ragged launch over all of them.  "buckets": widths are
@@ -156,25 +156,45 @@ def run_c3(n_k: int | None = None, plain: bool = True, bucket_step: int = 4, mod
         return done
 
     def ragged():
-        # one engine: every k-point its own width and the plain solver's guard,
-        # all of them in the same ragged launches (up to 64 slots)
-        done = {}
-        wmax = max(w for _, w, _ in probs)
-        eng = BatchedLOBPCG(c["grid"], wmax, c["guard"], min(len(probs), ragged_slots),
-                            max(nev for *_, nev in probs), c["tol"], 400)
-        pending = list(enumerate(probs))
+        # every k-point at its own width and the plain solver's guard, all of
+        # an engine's k-points in the same ragged launches; `ragged_engines`
+        # engines (own host thread and stream, k-points dealt round-robin in
+        # width order) so one engine's host phases overlap another's kernels
+        import threading
+        done, steps, errors = {}, [], []
+        order = sorted(range(len(probs)), key=lambda j: probs[j][1])
+        parts = [order[e::ragged_engines] for e in range(ragged_engines)]
+
+        def worker(items):
+            try:
+                with torch.cuda.stream(torch.cuda.Stream()):
+                    wmax = max(probs[j][1] for j in items)
+                    eng = BatchedLOBPCG(c["grid"], wmax, c["guard"], min(len(items), ragged_slots),
+                                        max(probs[j][2] for j in items), c["tol"], 400)
+                    pending = list(items)
+                    while pending or eng.active():
+                        while pending and eng.free_slots():
+                            j = pending.pop(0)
+                            k, w, nev = probs[j]
+                            eng.admit(Request(MaxwellOperator(eps, L, k, max_chunk=-1), nev, seed=j, rid=j,
+                                              m=w, guard=c["guard"]))
+                        for req in eng.step():
+                            done[req.rid] = req.vals
+                            req.op = None
+                            _progress(f"C3 ragged: k-point {req.rid} done ({len(done)}/{len(probs)})")
+                    torch.cuda.current_stream().synchronize()
+                    steps.append(eng.steps)
+            except BaseException as exc:
+                errors.append(exc)
+
+        ths = [threading.Thread(target=worker, args=(it,)) for it in parts if it]
         with one_blas_thread():
-            while pending or eng.active():
-                while pending and eng.free_slots():
-                    j, (k, w, nev) = pending.pop(0)
-                    eng.admit(Request(MaxwellOperator(eps, L, k, max_chunk=-1), nev, seed=j, rid=j,
-                                      m=w, guard=c["guard"]))
-                for req in eng.step():
-                    done[req.rid] = req.vals
-                    req.op = None
-                    _progress(f"C3 ragged: k-point {req.rid} done ({len(done)}/{len(probs)})")
-        steps = eng.steps
-        del eng
+            for th in ths:
+                th.start()
+            for th in ths:
+                th.join()
+        if errors:
+            raise errors[0]
         return done, steps
 
     from .linalg import one_blas_thread
@@ -183,7 +203,7 @@ def run_c3(n_k: int | None = None, plain: bool = True, bucket_step: int = 4, mod
     if mode in ("ragged", "both"):
         (done_r, steps), tr = _sync_time(ragged)
         res.update({"ragged_seconds": round(tr, 2), "ragged_k_per_s": round(len(probs) / tr, 3),
-                    "ragged_slots": min(len(probs), ragged_slots), "ragged_engine_steps": steps})
+                    "ragged_engines": ragged_engines, "ragged_engine_steps": steps})
         for j in range(len(probs)):
             g = _golden(f"c3_k{j}")
             if g is not None and j in done_r:
@@ -309,7 +329,8 @@ def main(argv=None):
     ap.add_argument("--bucket-step", type=int, default=4)
     ap.add_argument("--c3-mode", default="both", choices=["ragged", "buckets", "both"],
                     help="C3: one ragged engine, width buckets, or both")
-    ap.add_argument("--slots", type=int, default=64, help="C3: slots of the ragged engine")
+    ap.add_argument("--slots", type=int, default=64, help="C3: slots per ragged engine")
+    ap.add_argument("--engines", type=int, default=2, help="C3: ragged engines (threads and streams)")
     ap.add_argument("--degree", type=int, default=None)
     ap.add_argument("--outers", type=int, default=None)
     ap.add_argument("--n", type=int, default=200)
@@ -328,7 +349,7 @@ def main(argv=None):
         out = run_c2(args.ortho)
     elif args.case == "c3":
         out = run_c3(args.n_k, plain=not args.no_plain, bucket_step=args.bucket_step, mode=args.c3_mode,
-                     ragged_slots=args.slots)
+                     ragged_slots=args.slots, ragged_engines=args.engines)
     elif args.case == "c4kpm":
         out = run_c4kpm(args.degree)
     elif args.case == "c4cheb":

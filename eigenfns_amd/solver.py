@@ -34,7 +34,7 @@ import torch
 
 from .linalg import (COMBINE_IN_PLACE_MAX, DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_DROP, apply_chunked,
                      combine, combine_into, combine_multi_split, dead_spread, deflate, deflate_chunk_rows,
-                     eigh,
+                     eigh, eigh_batched_device,
                      gram, gram_herm, normalize_rows, ortho_block, ortho_coeffs, row_axpby,
                      row_axpby_norm_strided, rowdot, rownorm, rownorm2, rowstats_strided,
                      svqb_coeffs, ORTHO_METHODS)
@@ -129,7 +129,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                   verbose: bool = True, log_every: int = 0, initial_locked=None,
                   initial_vals=None, on_block=None, locked_storage: str = "auto",
                   theta_chunk: int = 0, normalized_rq: bool = True,
-                  stagnation: int = STAGNATION_WINDOW, ortho: str = "cholqr"):
+                  stagnation: int = STAGNATION_WINDOW, ortho: str = "cholqr", graph: bool = False):
     """Bottom-up deflated block LOBPCG.  Returns (eigenvalues, locked_vectors, stats).
 
     Locks the lowest (m − guard) Ritz pairs of each block once their relative
@@ -143,9 +143,19 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
     `ortho`: "cholqr" (default: pivoted FP64 Cholesky-QR on the device for the
     SVQB steps and the P whitening, no host round trip; the Rayleigh–Ritz
     eigensolve is unchanged) or "svqb" (the reference's eigendecomposition-based
-    orthonormalisation, solver.py:113-126)."""
+    orthonormalisation, solver.py:113-126).
+    `graph`: the steady-state iteration (everything between two residual
+    read-backs: preconditioner, projections, Cholesky-QR, Θ, the Rayleigh–Ritz
+    matrix; then the update, Θ(X) and the next residuals) is captured once per
+    block as two HIP graphs around the Rayleigh–Ritz eigensolve (rocSOLVER on
+    the device, uncaptured) and replayed: three submissions per iteration
+    instead of some forty launches, no host sync but the residual read-back;
+    needs ortho="cholqr"; results are bitwise those of the uncaptured loop
+    with device eigensolves."""
     if m <= guard:
         raise ValueError("m must exceed guard")
+    if graph and ortho != "cholqr":
+        raise ValueError("graph=True needs ortho='cholqr' (no host eigensolve inside the iteration)")
     if ortho not in ORTHO_METHODS:
         raise ValueError(f"ortho must be one of {ORTHO_METHODS}, got {ortho!r}")
     G = op.grid_size
@@ -210,36 +220,33 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
         stag = StagnationLock(tol, stagnation)
         stalled = False
         it = 0
-        for it in range(maxit):
-            # one read of X and ΘX for ⟨x,Θx⟩, |x|², |Θx|²; R = ΘX − λX with |R|² in the
-            # same pass; one host sync for all four (solver.py:228-238)
+
+        def residual_stats():
+            # one read of X and ΘX for ⟨x,Θx⟩, |x|², |Θx|²; R = ΘX − λX (into T2)
+            # with |R|² in the same pass (solver.py:228-238)
             st = rowstats_strided(X, HX, m, m, m, m, n)
             lam = st[:, 0] / torch.clamp(st[:, 2], min=1e-30) if normalized_rq else st[:, 0]
             rr2 = row_axpby_norm_strided(HX, m, torch.ones_like(lam), X, m, -lam, m, m, n, T2, m)
+            return torch.stack([lam, st[:, 2], rr2, st[:, 3]])
+
+        def small_eigh(A):
+            if graph:
+                w, V = eigh_batched_device(((A + A.conj().T) / 2)[None])
+                return w[0], V[0]
+            return eigh(A)
+
+        def iteration_tail(have_p):
+            """The device work of one iteration after the host's convergence
+            check, ending with the next iteration's residual statistics."""
+            A, nb = tail_rr_matrix(have_p)
+            return tail_update(small_eigh(A)[1], nb)
+
+        def tail_rr_matrix(have_p):
+            """Preconditioner, projections, orthonormalisation, Θ of W and P and
+            the penalised Rayleigh–Ritz matrix."""
             R = T2
-            host = torch.stack([lam, st[:, 2], rr2, st[:, 3]]).cpu().numpy()
-            lam_h = host[0]
-            rn_h = np.sqrt(host[2]) / np.maximum(np.sqrt(host[3]), 1e-30)
-            dead_rows = np.sqrt(host[1]) < 0.5
-            lam_h = np.where(dead_rows, np.inf, lam_h)
-            rn_h = np.where(dead_rows, np.inf, rn_h)
-            order = np.argsort(lam_h)
-            worst = float(rn_h[order[:n_lock]].max())
-            if worst < tol:
-                break
-            if stag.stalled(worst):
-                stalled = True
-                break
-            if log_every and it % log_every == 0:
-                leak = float(gram(L[:nl], X).abs().max().item()) if nl else 0.0
-                lo = np.sort(lam_h)
-                print(f"    it {it:3d}  lam[0,{n_lock - 1},{m - 1}] = {lo[0]:.4f} "
-                      f"{lo[n_lock - 1]:.4f} {lo[-1]:.4f}  res q50/q90 "
-                      f"{np.quantile(rn_h, 0.5):.1e}/{np.quantile(rn_h, 0.9):.1e}  "
-                      f"leak {leak:.1e}", flush=True)
             # W = K⁻¹R, deflated, projected out of X twice, SVQB (solver.py:251-257)
             op.precondition_into(R, W)
-            stats.precond_applications += m
             _deflate_inplace(W, L, nl)
             for _ in range(2):
                 combine_into(-gram(X, W), X, W, X=W, alpha=1.0)
@@ -265,14 +272,18 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             nb = (3 if have_p else 2) * m
             Sb, HSb = S[:nb], HS[:nb]
             op.theta_into(Sb[m:], HSb[m:])
-            stats.theta_applications += nb - m
             A = gram_herm(Sb, HSb)
             A = (A + A.conj().T) / 2
             dead = rownorm(Sb) < 0.5
             # max(1e4, 100·max|diag A|) (solver.py:299), kept on the device
             penalty = torch.clamp(100.0 * torch.diagonal(A).abs().max(), min=RR_DEAD_PENALTY)
             A = A + torch.diag(torch.where(dead, penalty * dead_spread(nb, A.device), 0.0).to(A.dtype))
-            wA, VA = eigh(A)
+            return A, nb
+
+        def tail_update(VA, nb):
+            """The Rayleigh–Ritz update of X and P, deflation, Θ(X) and the next
+            residual statistics."""
+            Sb = S[:nb]
             # Xn = S·c/‖S·c‖, Pn = S·c'/‖S·c'‖ (c' = c with its X rows zeroed;
             # solver.py:298-307).  S is orthonormal by construction and dead rows
             # get zero weight, so ‖S·c‖ = ‖c‖ to FP32 rounding: the norms are
@@ -290,10 +301,58 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                 X.copy_(T1)
                 P.copy_(T2)
             _deflate_inplace(X, L, nl)
-            have_p = True
             op.theta_into(X, HX)
-            stats.theta_applications += m
+            return residual_stats()
+
+        # steady-state iterations (P present) of this block: one eager run
+        # (sizes every scratch buffer and rocSOLVER's workspace), then one
+        # capture of each side of the Rayleigh–Ritz eigensolve, then replays;
+        # the eigensolve itself (rocSOLVER, which copies host data inside a
+        # call and so cannot be captured) runs uncaptured between the two
+        # graphs on the same stream, without a host sync
+        g, g_stats, warm = None, None, False
+        sdev = residual_stats()
+        for it in range(maxit):
+            host = sdev.cpu().numpy()
+            lam_h = host[0]
+            rn_h = np.sqrt(host[2]) / np.maximum(np.sqrt(host[3]), 1e-30)
+            dead_rows = np.sqrt(host[1]) < 0.5
+            lam_h = np.where(dead_rows, np.inf, lam_h)
+            rn_h = np.where(dead_rows, np.inf, rn_h)
+            order = np.argsort(lam_h)
+            worst = float(rn_h[order[:n_lock]].max())
+            if worst < tol:
+                break
+            if stag.stalled(worst):
+                stalled = True
+                break
+            if log_every and it % log_every == 0:
+                leak = float(gram(L[:nl], X).abs().max().item()) if nl else 0.0
+                lo = np.sort(lam_h)
+                print(f"    it {it:3d}  lam[0,{n_lock - 1},{m - 1}] = {lo[0]:.4f} "
+                      f"{lo[n_lock - 1]:.4f} {lo[-1]:.4f}  res q50/q90 "
+                      f"{np.quantile(rn_h, 0.5):.1e}/{np.quantile(rn_h, 0.9):.1e}  "
+                      f"leak {leak:.1e}", flush=True)
+            if graph and have_p and warm:
+                if g is None:
+                    g = (torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph())
+                    with torch.cuda.graph(g[0]):
+                        g_A, g_nb = tail_rr_matrix(True)
+                    g_V = torch.empty_like(g_A)
+                    with torch.cuda.graph(g[1]):
+                        g_stats = tail_update(g_V, g_nb)
+                g[0].replay()
+                g_V.copy_(small_eigh(g_A)[1])
+                g[1].replay()
+                sdev = g_stats
+            else:
+                sdev = iteration_tail(have_p)
+                warm = warm or have_p
+            stats.precond_applications += m
+            stats.theta_applications += (2 if have_p else 1) * m + m
+            have_p = True
             stats.iterations += 1
+        g = g_stats = g_A = g_V = None
         # honest final values with a fresh ΘX (solver.py:316-330)
         op.theta_into(X, HX)
         stats.theta_applications += m
