@@ -316,6 +316,8 @@ def main():
     ap.add_argument("--grid", type=int, default=G_BENCH)
     ap.add_argument("--m", type=int, default=M_BENCH)
     ap.add_argument("--chunk", type=int, default=0, help="vectors per workspace pass (0 = auto)")
+    ap.add_argument("--passes", type=int, default=0, choices=[0, 3, 5],
+                    help="Θ pass structure (0 = the tuning table's choice for the grid)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-extra", action="store_true",
                     help="also time Θ at 128³ (m = 32) and a KPM step at 256³ on CPU and GPU")
@@ -348,7 +350,7 @@ def main():
     t_raster = time.perf_counter() - t0
     print(f"[bench] network N={NET['n_points']} seed {NET['seed']} rasterised at {G}^3 in "
           f"{t_raster:.1f}s, ff={filling_fraction(eps):.4f}, L={L:.4f}", file=sys.stderr, flush=True)
-    op = MaxwellOperator(eps, L, max_chunk=args.chunk)
+    op = MaxwellOperator(eps, L, max_chunk=args.chunk, passes=args.passes)
     X = normalize_rows(op.random_block(m, seed=0))
     Y = torch.empty_like(X)
     stream = torch.cuda.current_stream()
@@ -395,6 +397,7 @@ def main():
         "dtype": "c64",
         "data": "synthetic",
         "config": {"workload": f"theta_apply_{G}^3_m{m}",
+                   "passes": op.passes,
                    "structure": f"synthetic periodic 4-NN rod network N={NET['n_points']} "
                                 f"seed {NET['seed']} r={NET['radius_factor']}·mean bond, "
                                 f"eps 8.41/1, periodic raster", "grid": G, "block": m,
@@ -402,7 +405,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": committed_traffic(G, m, op.chunk),
-                     "kernel": ("fused Θ-apply (xin, yzmid, xout)" if op.passes == 3 else "fused Θ-apply (zin, ypass, xmid, ypass, zout)"),
+                     "kernel": {3: "fused Θ-apply (xin_split, yzlds, xout_split)" if G == 192 else
+                                "fused Θ-apply (xin, yzmid, xout)",
+                                5: "fused Θ-apply (zin, ypass, xmid, ypass, zout)"}[op.passes],
                      "alg_bytes_per_launch": int(alg_bytes),
                      "launch_ms": round(launch_s * 1e3, 4)},
         "step_ms_p10_p50_p90": [round(float(np.percentile(step_ms, q)), 4) for q in (10, 50, 90)],

@@ -54,18 +54,19 @@ def _sync_time(fn):
     return out, time.perf_counter() - t0
 
 
-def run_c1(ortho: str = "cholqr"):
+def run_c1(ortho: str = "cholqr", graph: bool = False):
     from .operator import MaxwellOperator
     from .solver import lobpcg_blocks
     c = wl.C1
     eps, L = wl.network(c["n_points"], c["grid"])
     op = MaxwellOperator(eps, L)
-    lobpcg_blocks(op, 4, m=8, guard=4, tol=1e-3, verbose=False, ortho=ortho)  # warm
+    lobpcg_blocks(op, 4, m=8, guard=4, tol=1e-3, verbose=False, ortho=ortho, graph=graph)  # warm
     (vals, _, st), t = _sync_time(lambda: lobpcg_blocks(op, c["nev"], m=c["m"], guard=c["guard"],
                                                         tol=c["tol"], seed=0, verbose=False,
-                                                        ortho=ortho))
+                                                        ortho=ortho, graph=graph))
     return {"case": "C1", "grid": c["grid"], "nev": c["nev"], "m": c["m"], "guard": c["guard"],
-            "tol": c["tol"], "ortho": ortho, "small_dense": os.environ.get("EFN_SMALL_DENSE", "host"),
+            "tol": c["tol"], "ortho": ortho, "graph": graph,
+            "small_dense": "device" if graph else os.environ.get("EFN_SMALL_DENSE", "host"),
             "gpu_seconds": round(t, 3), "iterations": st.iterations,
             "theta_applications": st.theta_applications,
             "blocks": [{k: r[k] for k in ("iters", "max_res_locked", "converged", "stagnated")}
@@ -73,17 +74,18 @@ def run_c1(ortho: str = "cholqr"):
             "max_rel_err_vs_fp64_oracle": _rel_err(vals, _golden("c1")), "vals": vals.tolist()}
 
 
-def run_c2(ortho: str = "cholqr"):
+def run_c2(ortho: str = "cholqr", graph: bool = False):
     from .operator import MaxwellOperator
     from .solver import lobpcg_blocks
     c = wl.C2
     res = {"case": "C2", "grid": c["grid"], "nev": c["nev"], "m": c["m"], "guard": c["guard"],
-           "tol": c["tol"], "ortho": ortho, "small_dense": os.environ.get("EFN_SMALL_DENSE", "host")}
+           "tol": c["tol"], "ortho": ortho, "graph": graph,
+           "small_dense": "device" if graph else os.environ.get("EFN_SMALL_DENSE", "host")}
     # parity: the same network at 16³ against the dense FP64 solve
     eps16, L = wl.network(c["n_points"], c["dense_grid"])
     op16 = MaxwellOperator(eps16, L)
     vals16, _, _ = lobpcg_blocks(op16, c["nev"], m=c["m"], guard=c["guard"], tol=c["tol"],
-                                 verbose=False, ortho=ortho)
+                                 verbose=False, ortho=ortho, graph=graph)
     res["dense16_max_rel_err"] = _rel_err(vals16, _golden("c2dense"))
     del op16
     eps, L = wl.network(c["n_points"], c["grid"])
@@ -91,7 +93,7 @@ def run_c2(ortho: str = "cholqr"):
     with contextlib.redirect_stdout(sys.stderr):    # per-block progress lines
         (vals, _, st), t = _sync_time(lambda: lobpcg_blocks(
             op, c["nev"], m=c["m"], guard=c["guard"], tol=c["tol"], seed=0, verbose=True,
-            ortho=ortho))
+            ortho=ortho, graph=graph))
     res.update({"gpu_seconds": round(t, 2), "iterations": st.iterations,
                 "theta_applications": st.theta_applications,
                 "lambda_1_128": [float(vals[0]), float(vals[-1])]})
@@ -340,13 +342,15 @@ def main(argv=None):
     ap.add_argument("--ortho", default="cholqr", choices=["svqb", "cholqr"],
                     help="C1/C2: orthonormalisation of the LOBPCG blocks")
     ap.add_argument("--total", type=int, default=None, help="C5: length of the seeded stream")
+    ap.add_argument("--graph", action="store_true",
+                    help="C1/C2: steady-state iterations as replayed HIP graphs (device eigensolves)")
     ap.add_argument("--engine-voxels", type=float, default=None,
                     help="C5: cap slots x G^3 per engine (latency setting)")
     args = ap.parse_args(argv)
     if args.case == "c1":
-        out = run_c1(args.ortho)
+        out = run_c1(args.ortho, args.graph)
     elif args.case == "c2":
-        out = run_c2(args.ortho)
+        out = run_c2(args.ortho, args.graph)
     elif args.case == "c3":
         out = run_c3(args.n_k, plain=not args.no_plain, bucket_step=args.bucket_step, mode=args.c3_mode,
                      ragged_slots=args.slots, ragged_engines=args.engines)
