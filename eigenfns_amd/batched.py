@@ -186,7 +186,14 @@ class BatchedLOBPCG:
             raise ValueError("need 0 < guard < width")
         for s, sl in enumerate(self.slots):
             if sl is None:
-                self.slots[s] = _Slot(req, m, guard)
+                sl = self.slots[s] = _Slot(req, m, guard)
+                # the slot's row views, fixed while it is occupied
+                b, M = s * 3 * self.m, self.m
+                sl.X, sl.W, sl.P = self.S[b:b + m], self.S[b + m:b + 2 * m], self.S[b + 2 * m:b + 3 * m]
+                sl.XW, sl.Sb, sl.HSb = self.S[b:b + 2 * m], self.S[b:b + 3 * m], self.HS[b:b + 3 * m]
+                if self.T12 is not None:
+                    sl.t1 = self.T12[2 * s * M:2 * s * M + m]
+                    sl.t2 = self.T12[(2 * s + 1) * M:(2 * s + 1) * M + m]
                 self.L[s * self.cap:(s + 1) * self.cap].zero_()
                 req.t_admit = time.perf_counter()
                 return s
@@ -198,34 +205,28 @@ class BatchedLOBPCG:
         return self.slots[s].m
 
     def _X(self, s):
-        b = s * 3 * self.m
-        return self.S[b:b + self._w(s)]
+        return self.slots[s].X
 
     def _W(self, s):
-        b, w = s * 3 * self.m, self._w(s)
-        return self.S[b + w:b + 2 * w]
+        return self.slots[s].W
 
     def _P(self, s):
-        b, w = s * 3 * self.m, self._w(s)
-        return self.S[b + 2 * w:b + 3 * w]
+        return self.slots[s].P
 
     def _XW(self, s):
-        b = s * 3 * self.m
-        return self.S[b:b + 2 * self._w(s)]
+        return self.slots[s].XW
 
     def _Sb(self, s):
-        b = s * 3 * self.m
-        return self.S[b:b + 3 * self._w(s)]
+        return self.slots[s].Sb
 
     def _HSb(self, s):
-        b = s * 3 * self.m
-        return self.HS[b:b + 3 * self._w(s)]
+        return self.slots[s].HSb
 
     def _t1(self, s):
-        return self.T12[2 * s * self.m:2 * s * self.m + self._w(s)]
+        return self.slots[s].t1
 
     def _t2(self, s):
-        return self.T12[(2 * s + 1) * self.m:(2 * s + 1) * self.m + self._w(s)]
+        return self.slots[s].t2
 
     def _combine_back(self, idx, C, blocks):
         """block_j ← C_jᵀ·block_j for slots idx: in place up to

@@ -38,7 +38,7 @@ DEFLATE_CHUNK = 128     # solver.py:60
 
 
 def _flat_n(X: torch.Tensor) -> int:
-    return int(X[0].numel()) if X.shape[0] else int(np.prod(X.shape[1:]))
+    return int(X.numel()) // int(X.shape[0]) if X.shape[0] else int(np.prod(X.shape[1:]))
 
 
 def _small_dense_on_host() -> bool:
@@ -391,9 +391,10 @@ def gram_ragged(As, Bs, hermitian: bool = False) -> torch.Tensor:
     mb = [int(B.shape[0]) for B in Bs]
     ref = next((A for A in As if A.shape[0]), As[0])
     n = _flat_n(ref)
+    row = ref.shape[1:]
     for A, B in zip(As, Bs):
-        if (A.shape[0] and (not A.is_contiguous() or _flat_n(A) != n)) or \
-                (B.shape[0] and (not B.is_contiguous() or _flat_n(B) != n)):
+        if (A.shape[0] and (A.shape[1:] != row or not A.is_contiguous())) or \
+                (B.shape[0] and (B.shape[1:] != row or not B.is_contiguous())):
             raise ValueError("gram_ragged: blocks must be contiguous with one row length")
     mam, mbm = max(ma), max(mb)
     out = torch.empty((nprob, mam, mbm), dtype=torch.complex128, device=ref.device)

@@ -101,7 +101,7 @@ def _summary(done, wall, turns, engines):
 
 def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slots=32,
                   hbm_budget=200e9, verbose=False, threads=True, progress=False,
-                  engines_per_grid=2, engine_voxels=None, small_dense="host", max_nev=None,
+                  engines_per_grid=1, engine_voxels=None, small_dense="host", max_nev=None,
                   ortho="cholqr"):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
 
@@ -111,9 +111,12 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     engines can share the GPU.  Requests are admitted FIFO per grid.
     engines_per_grid > 1 splits each grid's slots over that many engines
     (threads, streams) pulling from the same queue, so one half's host work
-    (five small eigensolves and syncs per iteration) overlaps the other
-    half's kernels: 2 measured +16 % solves/s over 1 (48 requests, 32³–64³),
-    3 no further gain.
+    overlaps the other half's kernels: 2 measured +16 % solves/s over 1 (48
+    requests, 32³–64³) while each step's host work was several milliseconds
+    of Python per engine; with the slot views cached and the engine step's
+    host work down to the eigensolves, one engine per grid is as fast or
+    faster (serving sample: 11.5–11.7 against 9.5–11.0 solves/s), the
+    kernels of the 64³ engine being what the stream waits for.
 
     Slots are sized for the service's band limit `max_nev` (default: the
     request stream configuration's upper bound, 64), not for the requests
@@ -278,7 +281,7 @@ if __name__ == "__main__":
     ap.add_argument("--plain", action="store_true")
     ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--sequential", action="store_true", help="one host thread, one stream")
-    ap.add_argument("--engines-per-grid", type=int, default=2)
+    ap.add_argument("--engines-per-grid", type=int, default=1)
     ap.add_argument("--small-dense", choices=["device", "host", "hybrid"], default="host")
     ap.add_argument("--ortho", choices=["svqb", "cholqr"], default="cholqr")
     args = ap.parse_args()
