@@ -70,6 +70,34 @@ class Request:
     t_done: float = 0.0
 
 
+def rr_dead_rows(widths, K: int, xn: torch.Tensor, keep_w: torch.Tensor,
+                 keepp: torch.Tensor) -> torch.Tensor:
+    """Dead rows (len(widths), K) of padded Rayleigh–Ritz stacks
+    (solver.py:296-297): row r of problem j is X row r (r < w_j), W row r − w_j
+    (r < 2w_j) or P row r − 2w_j (r < 3w_j), dead when the X row's norm is
+    below 0.5 or the W / P direction was dropped by the whitening; rows from
+    3w_j on are padding, not dead.  xn (X row norms), keep_w and keepp are
+    (nprob, ≥ max width) — a ragged launch pads them to its widest problem
+    only — and every gather index is clamped to the operand's own width."""
+    wmax = max(widths)
+    if K < 3 * wmax:
+        raise ValueError(f"rr_dead_rows: K = {K} below three times the widest problem ({wmax})")
+    for name, t in (("xn", xn), ("keep_w", keep_w), ("keepp", keepp)):
+        if t.dim() != 2 or t.shape[0] != len(widths) or t.shape[1] < wmax:
+            raise ValueError(f"rr_dead_rows: {name} {tuple(t.shape)} narrower than the widest "
+                             f"problem ({len(widths)} × {wmax})")
+    dev = xn.device
+    r = torch.arange(K, device=dev)[None, :]
+    wt = torch.as_tensor(widths, device=dev)[:, None]
+
+    def take(mask, i):
+        return torch.gather(mask, 1, torch.clamp(i, 0, mask.shape[1] - 1).expand(len(widths), K))
+
+    alive = torch.where(r < wt, take(xn, r) >= 0.5,
+                        torch.where(r < 2 * wt, take(keep_w, r - wt), take(keepp, r - 2 * wt)))
+    return ~alive & (r < 3 * wt)
+
+
 class _Slot:
     def __init__(self, req: Request, m: int, guard: int):
         # m: this request's block width (≤ the engine's), guard its guard rows
@@ -377,19 +405,11 @@ class BatchedLOBPCG:
         """_rr_host with the penalty, eigensolve and normalisation on the device."""
         M = self.m
         widths = [self._w(s) for s in idx]
-        K = 3 * M
+        # the ragged Gram pads to three times the widest slot in idx, which is
+        # below 3M once the engine's widest requests have left
+        K = int(A.shape[1])
         xn = torch.stack([self._xn_dev[s] for s in idx])   # views: no host-to-device copy
-        # dead-row mask in the padded layout: row r of slot j is X row r (< w),
-        # W row r − w or P row r − 2w; rows past 3w are padding
-        r = torch.arange(K, device=A.device)
-        wt = torch.as_tensor(widths, device=A.device)[:, None]
-        xr = torch.clamp(r[None, :], max=M - 1).expand(len(idx), K)
-        wr = torch.clamp(r[None, :] - wt, 0, M - 1)
-        pr = torch.clamp(r[None, :] - 2 * wt, 0, M - 1)
-        alive = torch.where(r[None, :] < wt, torch.gather(xn, 1, xr) >= 0.5,
-                            torch.where(r[None, :] < 2 * wt, torch.gather(keep_w, 1, wr),
-                                        torch.gather(keepp, 1, pr)))
-        dead = ~alive & (r[None, :] < 3 * wt)
+        dead = rr_dead_rows(widths, K, xn, keep_w, keepp)
         if self.verify_dead_rows:
             dh = dead.cpu().numpy()
             self._check_dead(idx, [dh[j, :3 * w] for j, w in enumerate(widths)])

@@ -101,10 +101,12 @@ def _requests(idx=None):
             for i, (seed, L, k, nev, w, g) in enumerate(PROBLEMS) if idx is None or i in idx]
 
 
-@pytest.mark.parametrize("small_dense", ["host", "device"])
-def test_mixed_width_engine_matches_dense(small_dense):
+@pytest.mark.parametrize("small_dense,engine_m", [("host", 24), ("device", 24), ("device", 32)])
+def test_mixed_width_engine_matches_dense(small_dense, engine_m):
+    """engine_m 32: every request narrower than the engine, so the Rayleigh–Ritz
+    stacks are padded to three times the widest active request, not 3·m."""
     from eigenfns_amd.batched import solve_batched
-    done = solve_batched(_requests(), 8, m=24, guard=4, nslots=4, tol=1e-6, maxit=300,
+    done = solve_batched(_requests(), 8, m=engine_m, guard=4, nslots=4, tol=1e-6, maxit=300,
                          verify_dead_rows=True, small_dense=small_dense)
     assert len(done) == len(PROBLEMS)
     for req in done:

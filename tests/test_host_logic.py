@@ -75,3 +75,30 @@ def test_tuning_candidates_respect_the_workspace_cap():
     assert {p for p, _ in autotune.candidates(128)} == {3, 5}
     assert {p for p, _ in autotune.candidates(192)} == {3, 5}   # half-plane three passes
     assert {p for p, _ in autotune.candidates(256)} == {5}
+
+
+def test_rr_dead_rows_with_masks_narrower_than_the_engine():
+    """Dead rows of padded Rayleigh–Ritz stacks when the keep masks are only as
+    wide as the widest active problem (a ragged launch's padding), narrower
+    than the engine's block: every row matches a per-problem loop, padding
+    rows are not dead, and a mask narrower than the widest problem is refused
+    (torch's CPU gather checks its indices, so an index past a mask raises)."""
+    import numpy as np
+    import torch
+    from eigenfns_amd.batched import rr_dead_rows
+    rng = np.random.default_rng(4)
+    widths, M = [3, 5, 2], 8
+    wa = max(widths)
+    K = 3 * wa
+    xn = torch.from_numpy(rng.uniform(0.0, 1.0, (3, M)))
+    keep_w = torch.from_numpy(rng.random((3, wa)) < 0.7)
+    keepp = torch.from_numpy(rng.random((3, wa)) < 0.7)
+    dead = rr_dead_rows(widths, K, xn, keep_w, keepp).numpy()
+    for j, w in enumerate(widths):
+        ref = np.concatenate([xn[j, :w].numpy() < 0.5, ~keep_w[j, :w].numpy(), ~keepp[j, :w].numpy(),
+                              np.zeros(K - 3 * w, dtype=bool)])
+        assert np.array_equal(dead[j], ref)
+    with pytest.raises(ValueError):
+        rr_dead_rows(widths, K, xn, keep_w[:, :wa - 1], keepp)
+    with pytest.raises(ValueError):
+        rr_dead_rows(widths, K - 1, xn, keep_w, keepp)
