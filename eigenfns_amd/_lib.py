@@ -77,6 +77,9 @@ _SIGNATURES = {
     "efn_row_axpby_norm_strided": (_i, [_vp, _i64, _vp, _vp, _i64, _vp, _i, _i, _i64, _vp, _i64, _vp,
                                         _vp, _sz, _vp]),
     "efn_row_axpby_strided": (_i, [_vp, _i64, _vp, _vp, _i64, _vp, _i, _i, _i64, _vp, _i64, _vp]),
+    "efn_rowstats_sel": (_i, [_vp, _i64, _vp, _i64, _vp, _i, _i, _i, _i64, _vp, _vp, _sz, _vp]),
+    "efn_row_axpby_norm_sel": (_i, [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _i, _i, _i, _i64, _vp, _i64,
+                                    _vp, _vp, _sz, _vp]),
     "efn_axpby": (_i, [_vp, _f, _vp, _f, _i64, _vp, _vp]),
     "efn_copy_multi": (_i, [_i, _vp, _vp, _i64, _vp]),
     "efn_random_block": (_i, [_vp, _i, C.c_uint64, _i64, _i, _vp, _vp]),

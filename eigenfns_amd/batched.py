@@ -4,8 +4,8 @@ The serving path: many (structure, Bloch k-point, band count)
 solves of one grid size advance in lockstep, one LOBPCG iteration per engine
 step, through shared launches — multi-operator Θ / preconditioner
 (efn_theta_multi: each slot's vectors carry their own frame tables and ε),
-multi-problem Gram and combine (efn_gram_multi / efn_combine_multi), strided
-row reductions over every slot's rows at once — and new requests take a slot
+multi-problem Gram and combine (efn_gram_multi / efn_combine_multi), row
+reductions over the occupied slots' rows at once — and new requests take a slot
 as soon as one frees (continuous batching).  Each slot runs the reference's
 algorithm (eigenfns/solver.py:136-382): guard warm start, re-deflation every
 iteration, SVQB (1e-5 drop), fixed-shape P whitening, Rayleigh–Ritz with the
@@ -41,7 +41,7 @@ import torch
 from .linalg import (COMBINE_IN_PLACE_MAX, DEAD_SPREAD, ORTHO_METHODS, RR_DEAD_PENALTY, SVQB_DROP,
                      cholqr_coeffs_batched, combine_ragged, dead_spread, eigh_batched_device,
                      eigh_list_host, gram_ragged, one_blas_thread, rowreduce_strided,
-                     row_axpby_norm_strided, rowstats_strided, MAX_PROBLEMS)
+                     row_axpby_norm_sel, rowstats_sel, MAX_PROBLEMS)
 from .operator import MaxwellOperator, precondition_ragged, theta_ragged
 from .solver import STAGNATION_WINDOW, StagnationLock
 
@@ -147,12 +147,12 @@ class BatchedLOBPCG:
         self.verify_dead_rows = verify_dead_rows  # also measure the row norms and compare (tests)
         # the SVQB / whitening / Rayleigh–Ritz eigensolves: "host" (LAPACK over a
         # thread pool, masks and normalisation in NumPy; the default), "device"
-        # (efn_heev_batched and the masks on the device: no host round trip) or
-        # "hybrid" (SVQB and whitening on the device, Rayleigh–Ritz on the
-        # host).  The device Jacobi takes 7.7 ms per 72 × 72 Rayleigh–Ritz
-        # matrix (its eigenvectors live in global memory: A and V do not both
-        # fit in LDS at n = 72) against 0.4 ms for one host LAPACK call, and the
-        # engines' syncs are already overlapped by the second engine per grid.
+        # (rocSOLVER's batched divide and conquer, efn_heevd_batched, one call
+        # per distinct size, and the masks on the device: no host round trip)
+        # or "hybrid" (SVQB and whitening on the device, Rayleigh–Ritz on the
+        # host).  A batched device call takes about 1.5 ms whatever the batch,
+        # against 0.4 ms for one host LAPACK call at 72 × 72 (many calls run
+        # at once in the pool), so the host stays the default.
         if small_dense not in ("device", "host", "hybrid"):
             raise ValueError(f"small_dense must be 'device', 'host' or 'hybrid', got {small_dense!r}")
         self.small_dense = small_dense
@@ -185,6 +185,7 @@ class BatchedLOBPCG:
         self.slots: list[_Slot | None] = [None] * nslots
         self.steps = 0
         self.timing = {}  # host seconds per phase (includes waits at the phase's syncs)
+        self._sel_key = None  # occupied (slot, width) pairs of the cached row selection
 
     def _tick(self, name, t0):
         t1 = time.perf_counter()
@@ -557,6 +558,20 @@ class BatchedLOBPCG:
         sl.phase = "init"
         return None
 
+    def _row_selection(self, act):
+        """(device selection slot << 16 | r, host slot and row indices, device
+        linear indices slot·M + r) of the X rows of the occupied slots act,
+        rebuilt only when the occupied slots or their widths change."""
+        key = tuple((s, self.slots[s].m) for s in act)
+        if key != self._sel_key:
+            slots = np.repeat([s for s, _ in key], [w for _, w in key])
+            rows = np.concatenate([np.arange(w) for _, w in key])
+            dev = self.S.device
+            self._sel = (torch.as_tensor(((slots << 16) | rows).astype(np.int32), device=dev),
+                         slots, rows, torch.as_tensor(slots * self.m + rows, device=dev))
+            self._sel_key = key
+        return self._sel
+
     # ---- one engine step = one LOBPCG iteration for every active slot -------
     def step(self) -> list:
         finished = []
@@ -570,23 +585,29 @@ class BatchedLOBPCG:
         if init:
             self._init_blocks(init)
             t = self._tick("init", t)
-        # Rayleigh quotients, residuals, dead rows for the leading M rows of
-        # every slot (its X rows; a narrower slot's trailing rows are ignored)
-        NM = self.nslots * M
+        # Rayleigh quotients, residuals, dead rows of the X rows of the occupied
+        # slots only (each slot's own width: no padding rows, no free slots)
+        sel, sel_slot, sel_row, sel_lin = self._row_selection(act)
         # one read of X and ΘX for ⟨x,Θx⟩, |x|², |Θx|²; R = ΘX − λX with |R|² in the same pass
-        st = rowstats_strided(self.S, self.HS, M, 3 * M, 3 * M, NM, self.n)
+        st = rowstats_sel(self.S, self.HS, sel, M, self.nslots, 3 * M, 3 * M, self.n)
         lam, xn2, hx2 = st[:, 0], st[:, 2], st[:, 3]
         if self.normalized_rq:
             lam = lam / torch.clamp(xn2, min=1e-30)
-        rr2 = row_axpby_norm_strided(self.HS, 3 * M, torch.ones_like(lam), self.S, 3 * M, -lam, M,
-                                     NM, self.n, self.R, M)
-        self._xn_dev = torch.sqrt(xn2).view(self.nslots, M)   # device masks (small_dense="device")
+        rr2 = row_axpby_norm_sel(self.HS, 3 * M, torch.ones_like(lam), self.S, 3 * M, -lam, sel, M,
+                                 self.nslots, self.n, self.R, M)
+        # device masks (small_dense="device"): X row norms in the (slot, row) layout
+        xnd = torch.zeros(self.nslots * M, dtype=torch.float64, device=self.S.device)
+        xnd[sel_lin] = torch.sqrt(xn2)
+        self._xn_dev = xnd.view(self.nslots, M)
         stats = torch.stack([lam, xn2, rr2, hx2]).cpu().numpy()
         t = self._tick("residuals+sync", t)
-        lam_h = stats[0].reshape(self.nslots, M)
-        xn = np.sqrt(stats[1]).reshape(self.nslots, M)
+        lam_h = np.zeros((self.nslots, M))
+        xn = np.zeros((self.nslots, M))
+        rn_h = np.zeros((self.nslots, M))
+        lam_h[sel_slot, sel_row] = stats[0]
+        xn[sel_slot, sel_row] = np.sqrt(stats[1])
+        rn_h[sel_slot, sel_row] = np.sqrt(stats[2]) / np.maximum(np.sqrt(stats[3]), 1e-30)
         self._xn = xn
-        rn_h = (np.sqrt(stats[2]) / np.maximum(np.sqrt(stats[3]), 1e-30)).reshape(self.nslots, M)
         cont = []
         for s in act:
             sl = self.slots[s]

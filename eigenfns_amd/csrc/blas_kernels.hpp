@@ -789,6 +789,28 @@ __device__ __forceinline__ long long phys_row(int i, int nrow, long long stride)
   return (long long)slot * stride + (i - slot * nrow);
 }
 
+// Row maps of the row-statistics kernels: without a selection, logical row i
+// is phys_row(i, nrow, stride); with one, sel[i] = slot << 16 | r names row r
+// of slot `slot` (physical slot·stride + r), and a pair outside r < nrow,
+// slot < nslot is no row at all (-1: nothing read or written, zero sums).
+struct RowSel {
+  const int* sel;
+  int nrow, nslot;
+};
+__device__ __forceinline__ bool sel_pair(const RowSel& rs, int i, int& slot, int& r) {
+  if (!rs.sel) {
+    slot = i / rs.nrow;
+    r = i - slot * rs.nrow;
+    return true;
+  }
+  const int v = rs.sel[i];
+  slot = v >> 16;
+  r = v & 0xFFFF;
+  if (v >= 0 && r < rs.nrow && slot < rs.nslot) return true;
+  slot = r = 0;
+  return false;
+}
+
 template <int OP>
 __global__ __launch_bounds__(256) void rowreduce_partial_kernel(const float2* __restrict__ A, long long sa,
                                                                 const float2* __restrict__ B, long long sb,
@@ -838,14 +860,16 @@ __global__ void rowreduce_final_kernel(const double2* __restrict__ part, int nsp
 // FP32 products (same split layout as rowreduce_partial_kernel).
 __global__ __launch_bounds__(256) void rowstats_partial_kernel(const float2* __restrict__ A, long long sa,
                                                                const float2* __restrict__ B, long long sb,
-                                                               int nrow, long long n, long long kper,
+                                                               RowSel rs, long long n, long long kper,
                                                                double2* __restrict__ part) {
   __shared__ double2 red[8];
   const int i = blockIdx.y, s = blockIdx.x, nsplit = gridDim.x;
   const long long kb = (long long)s * kper;
-  const long long ke = kb + kper < n ? kb + kper : n;
-  const float2* a = A + phys_row(i, nrow, sa) * n;
-  const float2* b = B + phys_row(i, nrow, sb) * n;
+  int slot, r;
+  const bool valid = sel_pair(rs, i, slot, r);
+  const long long ke = !valid ? kb : (kb + kper < n ? kb + kper : n);
+  const float2* a = A + ((long long)slot * sa + r) * n;
+  const float2* b = B + ((long long)slot * sb + r) * n;
   double dre = 0.0, dim_ = 0.0, aa = 0.0, bb = 0.0;
   for (long long k = kb + threadIdx.x; k < ke; k += 256) {
     const float2 x = a[k], y = b[k];
@@ -892,19 +916,21 @@ __global__ void rowstats_final_kernel(const double2* __restrict__ part, int nspl
 __global__ __launch_bounds__(256) void row_axpby_norm_kernel(const float2* __restrict__ X, long long sx,
                                                              const double* __restrict__ a,
                                                              const float2* __restrict__ Y, long long sy,
-                                                             const double* __restrict__ b, int nrow,
+                                                             const double* __restrict__ b, RowSel rs,
                                                              long long n, long long kper,
                                                              float2* __restrict__ out, long long so,
                                                              double2* __restrict__ part) {
   __shared__ double2 red[4];
   const int i = blockIdx.y, s = blockIdx.x, nsplit = gridDim.x;
   const long long kb = (long long)s * kper;
-  const long long ke = kb + kper < n ? kb + kper : n;
+  int slot, r;
+  const bool valid = sel_pair(rs, i, slot, r);
+  const long long ke = !valid ? kb : (kb + kper < n ? kb + kper : n);
   const float ai = (float)a[i];
   const float bi = Y ? (float)b[i] : 0.f;
-  const float2* xr = X + phys_row(i, nrow, sx) * n;
-  const float2* yr = Y ? Y + phys_row(i, nrow, sy) * n : nullptr;
-  float2* orow = out + phys_row(i, nrow, so) * n;
+  const float2* xr = X + ((long long)slot * sx + r) * n;
+  const float2* yr = Y ? Y + ((long long)slot * sy + r) * n : nullptr;
+  float2* orow = out + ((long long)slot * so + r) * n;
   double acc = 0.0;
   for (long long k = kb + threadIdx.x; k < ke; k += 256) {
     const float2 x = xr[k];

@@ -1719,8 +1719,9 @@ size_t efn_rowstats_work_bytes(int m, int64_t n) {
   return 2 * efn_rowreduce_work_bytes(m, n);
 }
 
-int efn_rowstats_strided(const void* A, int64_t stride_a, const void* B, int64_t stride_b, int nrow,
-                         int m, int64_t n, double* out4, void* work, size_t work_bytes, void* stream) {
+static int rowstats_impl(const void* A, int64_t stride_a, const void* B, int64_t stride_b, const int* sel,
+                         int nrow, int nslot, int m, int64_t n, double* out4, void* work, size_t work_bytes,
+                         void* stream) {
   if (m < 0 || n <= 0) return fail(EFN_EINVAL, "need m >= 0 and n > 0");
   if (m == 0) return EFN_OK;
   if (nrow <= 0 || stride_a < nrow || stride_b < nrow) return fail(EFN_EINVAL, "row map needs 0 < nrow <= stride");
@@ -1730,8 +1731,8 @@ int efn_rowstats_strided(const void* A, int64_t stride_a, const void* B, int64_t
   const int ns = rowreduce_nsplit(n);
   const long long kper = (n + ns - 1) / ns;
   hipLaunchKernelGGL(efn::rowstats_partial_kernel, dim3(ns, m), dim3(256), 0, s, (const float2*)A,
-                     (long long)stride_a, (const float2*)B, (long long)stride_b, nrow, (long long)n, kper,
-                     (double2*)work);
+                     (long long)stride_a, (const float2*)B, (long long)stride_b, efn::RowSel{sel, nrow, nslot},
+                     (long long)n, kper, (double2*)work);
   EFN_LAUNCH_CHECK();
   hipLaunchKernelGGL(efn::rowstats_final_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
                      (const double2*)work, ns, m, out4);
@@ -1739,10 +1740,30 @@ int efn_rowstats_strided(const void* A, int64_t stride_a, const void* B, int64_t
   return EFN_OK;
 }
 
-int efn_row_axpby_norm_strided(const void* X, int64_t stride_x, const double* a, const void* Y,
-                               int64_t stride_y, const double* b, int nrow, int m, int64_t n, void* out,
-                               int64_t stride_out, double* norm2_c128, void* work, size_t work_bytes,
-                               void* stream) {
+int efn_rowstats_strided(const void* A, int64_t stride_a, const void* B, int64_t stride_b, int nrow,
+                         int m, int64_t n, double* out4, void* work, size_t work_bytes, void* stream) {
+  return rowstats_impl(A, stride_a, B, stride_b, nullptr, nrow, 0, m, n, out4, work, work_bytes, stream);
+}
+
+// selected rows: sel (device, m entries) = slot << 16 | r with r < nrow < 65536 and slot < nslot
+static const char* sel_check(const int* sel, int nrow, int nslot) {
+  if (!sel) return "NULL row selection";
+  if (nrow <= 0 || nrow > 0xFFFF) return "selected rows need 0 < nrow < 65536";
+  if (nslot <= 0 || nslot > 0x7FFF) return "selected rows need 0 < nslot < 32768";
+  return nullptr;
+}
+
+int efn_rowstats_sel(const void* A, int64_t stride_a, const void* B, int64_t stride_b, const int* sel,
+                     int nrow, int nslot, int m, int64_t n, double* out4, void* work, size_t work_bytes,
+                     void* stream) {
+  if (const char* e = sel_check(sel, nrow, nslot)) return fail(EFN_EINVAL, e);
+  return rowstats_impl(A, stride_a, B, stride_b, sel, nrow, nslot, m, n, out4, work, work_bytes, stream);
+}
+
+static int row_axpby_norm_impl(const void* X, int64_t stride_x, const double* a, const void* Y,
+                               int64_t stride_y, const double* b, const int* sel, int nrow, int nslot, int m,
+                               int64_t n, void* out, int64_t stride_out, double* norm2_c128, void* work,
+                               size_t work_bytes, void* stream) {
   if (m < 0 || n <= 0) return fail(EFN_EINVAL, "need m >= 0 and n > 0");
   if (m == 0) return EFN_OK;
   if (nrow <= 0 || stride_x < nrow || stride_out < nrow || (Y && stride_y < nrow))
@@ -1754,13 +1775,30 @@ int efn_row_axpby_norm_strided(const void* X, int64_t stride_x, const double* a,
   const int ns = rowreduce_nsplit(n);
   const long long kper = (n + ns - 1) / ns;
   hipLaunchKernelGGL(efn::row_axpby_norm_kernel, dim3(ns, m), dim3(256), 0, s, (const float2*)X,
-                     (long long)stride_x, a, (const float2*)Y, (long long)stride_y, b, nrow, (long long)n,
-                     kper, (float2*)out, (long long)stride_out, (double2*)work);
+                     (long long)stride_x, a, (const float2*)Y, (long long)stride_y, b,
+                     efn::RowSel{sel, nrow, nslot}, (long long)n, kper, (float2*)out, (long long)stride_out,
+                     (double2*)work);
   EFN_LAUNCH_CHECK();
   hipLaunchKernelGGL(efn::rowreduce_final_kernel, dim3((m + 255) / 256), dim3(256), 0, s,
                      (const double2*)work, ns, m, (double2*)norm2_c128);
   EFN_LAUNCH_CHECK();
   return EFN_OK;
+}
+
+int efn_row_axpby_norm_strided(const void* X, int64_t stride_x, const double* a, const void* Y,
+                               int64_t stride_y, const double* b, int nrow, int m, int64_t n, void* out,
+                               int64_t stride_out, double* norm2_c128, void* work, size_t work_bytes,
+                               void* stream) {
+  return row_axpby_norm_impl(X, stride_x, a, Y, stride_y, b, nullptr, nrow, 0, m, n, out, stride_out,
+                             norm2_c128, work, work_bytes, stream);
+}
+
+int efn_row_axpby_norm_sel(const void* X, int64_t stride_x, const double* a, const void* Y, int64_t stride_y,
+                           const double* b, const int* sel, int nrow, int nslot, int m, int64_t n, void* out,
+                           int64_t stride_out, double* norm2_c128, void* work, size_t work_bytes, void* stream) {
+  if (const char* e = sel_check(sel, nrow, nslot)) return fail(EFN_EINVAL, e);
+  return row_axpby_norm_impl(X, stride_x, a, Y, stride_y, b, sel, nrow, nslot, m, n, out, stride_out,
+                             norm2_c128, work, work_bytes, stream);
 }
 
 int efn_rowreduce(int op, const void* A, const void* B, int m, int64_t n, void* out, void* work,

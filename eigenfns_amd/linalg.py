@@ -480,6 +480,64 @@ def rowstats_strided(A, B, nrow: int, stride_a: int, stride_b: int, m: int, n: i
     return out
 
 
+def _check_sel(sel, nrow: int, nslot: int, n: int, blocks) -> int:
+    """Host checks of a row selection (slot << 16 | r entries, device int32)
+    against the blocks it addresses: each (block, stride) must hold nslot
+    slots of stride rows of length n with nrow ≤ stride.  Returns len(sel)."""
+    if not (torch.is_tensor(sel) and sel.dtype == torch.int32 and sel.dim() == 1 and sel.is_contiguous()):
+        raise ValueError("row selection must be a contiguous 1-D int32 tensor")
+    if not 0 < nrow < 65536 or not 0 < nslot < 32768:
+        raise ValueError("row selection needs 0 < nrow < 65536 and 0 < nslot < 32768")
+    for T, stride in blocks:
+        if T is None:
+            continue
+        if stride < nrow or _flat_n(T) != n or T.shape[0] < (nslot - 1) * stride + nrow:
+            raise ValueError(f"block {tuple(T.shape)} does not hold {nslot} slots of stride {stride} "
+                             f"(nrow {nrow}, row length {n})")
+        if sel.device != T.device:
+            raise ValueError("row selection and blocks on different devices")
+    return int(sel.numel())
+
+
+def rowstats_sel(A, B, sel, nrow: int, nslot: int, stride_a: int, stride_b: int, n: int):
+    """rowstats_strided over selected rows: sel[i] = slot << 16 | r names row r
+    of slot `slot` (physical row slot·stride + r of A and of B); (len(sel), 4)."""
+    m = _check_sel(sel, nrow, nslot, n, [(A, stride_a), (B, stride_b)])
+    out = torch.empty((m, 4), dtype=torch.float64, device=A.device)
+    if m == 0:
+        return out
+    lib = _lib.load()
+    work = SCRATCH.get(lib.efn_rowstats_work_bytes(m, n), "rowstats")
+    _lib.check(lib.efn_rowstats_sel(A.data_ptr(), stride_a, B.data_ptr(), stride_b, sel.data_ptr(), nrow,
+                                    nslot, m, n, out.data_ptr(), work.data_ptr(), work.numel(),
+                                    stream_ptr()), "rowstats_sel")
+    return out
+
+
+def row_axpby_norm_sel(X, stride_x, a, Y, stride_y, b, sel, nrow: int, nslot: int, n: int, out,
+                       stride_out):
+    """row_axpby_norm_strided over selected rows (see rowstats_sel); a, b per
+    selected row; returns |out_i|² (FP64, len(sel))."""
+    m = _check_sel(sel, nrow, nslot, n, [(X, stride_x), (Y, stride_y), (out, stride_out)])
+    a = torch.as_tensor(a, dtype=torch.float64, device=X.device).reshape(-1).contiguous()
+    if Y is not None:
+        b = torch.as_tensor(b, dtype=torch.float64, device=X.device).reshape(-1).contiguous()
+    if a.numel() != m or (Y is not None and b.numel() != m):
+        raise ValueError("row_axpby_norm_sel: one coefficient per selected row")
+    nrm = torch.empty(m, dtype=torch.complex128, device=X.device)
+    if m == 0:
+        return nrm.real
+    lib = _lib.load()
+    work = SCRATCH.get(lib.efn_rowreduce_work_bytes(m, n), "rowred")
+    _lib.check(lib.efn_row_axpby_norm_sel(X.data_ptr(), stride_x, a.data_ptr(),
+                                          None if Y is None else Y.data_ptr(), stride_y,
+                                          None if Y is None else b.data_ptr(), sel.data_ptr(), nrow, nslot,
+                                          m, n, out.data_ptr(), stride_out, nrm.data_ptr(),
+                                          work.data_ptr(), work.numel(), stream_ptr()),
+               "row_axpby_norm_sel")
+    return nrm.real
+
+
 def row_axpby_norm_strided(X, stride_x, a, Y, stride_y, b, nrow, m, n, out, stride_out):
     """out_i = a_i·X_i + b_i·Y_i over strided rows; returns |out_i|² (FP64, m) from
     the same pass."""

@@ -330,6 +330,21 @@ int efn_row_axpby_norm_strided(const void* X_c64, int64_t stride_x, const double
                                const void* Y_c64, int64_t stride_y, const double* b_dev, int nrow,
                                int m, int64_t n, void* out_c64, int64_t stride_out,
                                double* norm2_c128, void* work, size_t work_bytes, void* stream);
+/* The two above over selected rows (the same residual statistics,
+ * solver.py:228-238, for the occupied rows of slots of different widths):
+ * logical row i is row r of slot `slot` with sel_dev[i] = slot << 16 | r
+ * (device int32, m entries), physical row slot·stride + r of each operand;
+ * an entry with r ≥ nrow or slot ≥ nslot reads and writes nothing and gives
+ * zero sums.  0 < nrow < 65536, 0 < nslot < 32768, every stride ≥ nrow.
+ * Each selected row's sums are the strided entry point's for that row, bit
+ * for bit.  Same work buffers. */
+int efn_rowstats_sel(const void* A_c64, int64_t stride_a, const void* B_c64, int64_t stride_b,
+                     const int* sel_dev, int nrow, int nslot, int m, int64_t n, double* out4, void* work,
+                     size_t work_bytes, void* stream);
+int efn_row_axpby_norm_sel(const void* X_c64, int64_t stride_x, const double* a_dev, const void* Y_c64,
+                           int64_t stride_y, const double* b_dev, const int* sel_dev, int nrow, int nslot,
+                           int m, int64_t n, void* out_c64, int64_t stride_out, double* norm2_c128,
+                           void* work, size_t work_bytes, void* stream);
 
 /* out_i = a[i]·X_i + b[i]·Y_i with per-row FP64 coefficients on the device
  * (Y, b may be NULL).  R = HX − λX (solver.py:229), row scaling (:306-307). */

@@ -133,3 +133,42 @@ def test_mixed_width_request_matches_plain_solver():
     single, _, _ = lobpcg_blocks(req.op, req.nev, m=16, guard=4, tol=1e-6, seed=req.seed, verbose=False)
     batched = solve_batched([req], 8, m=24, guard=4, nslots=2, tol=1e-6)[0].vals
     assert np.abs(single - batched).max() < 1e-6 * single[-1]
+
+
+def test_selected_row_statistics_equal_strided():
+    """efn_rowstats_sel / efn_row_axpby_norm_sel give each selected row the
+    strided entry points' sums bit for bit; an entry outside nrow × nslot
+    reads and writes nothing and gives zero sums."""
+    import torch
+    from eigenfns_amd.linalg import (row_axpby_norm_sel, row_axpby_norm_strided, rowstats_sel,
+                                     rowstats_strided)
+    G, nslot, M = 8, 4, 6
+    S3 = 3 * M
+    n = 2 * G ** 3
+    S = _block(nslot * S3, G, 60)
+    HS = _block(nslot * S3, G, 61)
+    pairs = [(0, 0), (0, 1), (0, 2), (2, 0), (2, 3), (2, 5), (3, 0), (nslot, 0), (1, M)]
+    sel = torch.tensor([(s << 16) | r for s, r in pairs], dtype=torch.int32, device=S.device)
+    full = rowstats_strided(S, HS, M, S3, S3, nslot * M, n)
+    st = rowstats_sel(S, HS, sel, M, nslot, S3, S3, n)
+    for i, (s, r) in enumerate(pairs):
+        want = full[s * M + r] if s < nslot and r < M else torch.zeros(4, dtype=torch.float64, device=S.device)
+        assert torch.equal(st[i], want)
+    lam_full = full[:, 0] / full[:, 2]
+    R_full = torch.full((nslot * M, 2, G, G, G), 7.0 + 0j, dtype=torch.complex64, device=S.device)
+    R_sel = R_full.clone()
+    nf = row_axpby_norm_strided(HS, S3, torch.ones_like(lam_full), S, S3, -lam_full, M, nslot * M, n,
+                                R_full, M)
+    lam_sel = torch.stack([lam_full[s * M + r] if s < nslot and r < M else lam_full[0] for s, r in pairs])
+    ns = row_axpby_norm_sel(HS, S3, torch.ones_like(lam_sel), S, S3, -lam_sel, sel, M, nslot, n, R_sel, M)
+    touched = set()
+    for i, (s, r) in enumerate(pairs):
+        if s < nslot and r < M:
+            touched.add(s * M + r)
+            assert torch.equal(ns[i], nf[s * M + r])
+            assert torch.equal(R_sel[s * M + r], R_full[s * M + r])
+        else:
+            assert ns[i].item() == 0.0
+    for row in range(nslot * M):
+        if row not in touched:
+            assert bool((R_sel[row] == 7.0).all())
