@@ -8,9 +8,13 @@ lowest m−guard pairs, monotone final sort.  What changes is where things
 live: [X; W; P] and [ΘX; ΘW; ΘP] are two contiguous device blocks, so the
 Rayleigh–Ritz matrix is one Gram kernel and each update one combine kernel;
 the locked set stays in HBM (no host streaming).  The small FP64 algebra
-(the SVQB and whitening eigensolves and the Rayleigh–Ritz eigenproblem, at
-most 3m × 3m) runs in host LAPACK as in the reference (solver.py:119-121,
-263-265, 301), on the Gram matrices the device sums in FP64.  Rayleigh
+works on Gram matrices the device sums in FP64: the orthonormalisation of W
+and P is a pivoted Cholesky-QR on the device by default (ortho="cholqr"; the
+reference's SVQB eigendecompositions with ortho="svqb"), and the
+Rayleigh–Ritz eigenproblem (at most 3m × 3m) runs in host LAPACK as in the
+reference (solver.py:119-121, 263-265, 301), or on the device (rocSOLVER)
+with EFN_SMALL_DENSE=device and always when the iteration is graph-captured
+(graph=True).  Rayleigh
 quotients are normalised by ‖x‖² (the reference's interior.py:69-74 fix)
 unless ``normalized_rq=False``.
 
