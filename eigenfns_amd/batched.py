@@ -38,8 +38,9 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .linalg import (COMBINE_IN_PLACE_MAX, DEAD_SPREAD, ORTHO_METHODS, RR_DEAD_PENALTY, SVQB_DROP,
-                     cholqr_coeffs_batched, combine_ragged, dead_spread, eigh_batched_device,
+from .linalg import (COMBINE_IN_PLACE_MAX, DEAD_SPREAD, ORTHO_METHODS, QL_MAX_N, RR_DEAD_PENALTY,
+                     SVQB_DROP, cholqr_coeffs_batched, combine_ragged, dead_spread, eigh_batched_device,
+                     eigh_ragged_device,
                      eigh_list_host, gram_ragged, one_blas_thread, rowreduce_strided,
                      row_axpby_norm_sel, rowstats_sel, MAX_PROBLEMS)
 from .operator import MaxwellOperator, precondition_ragged, theta_ragged
@@ -301,18 +302,29 @@ class BatchedLOBPCG:
 
     def _eigh_device_ragged(self, A, widths, lowest=None):
         """(w, V) of each problem's leading widths[j] block of the padded device
-        stack A, grouped by width (one batched eigensolve per width), padded
-        back: V's leading widths[j] rows and min(lowest, width) columns, w
-        padded with +inf."""
+        stack A: one ragged launch of the library's QL kernel for the problems
+        up to 96 × 96, one rocSOLVER call per width above that; padded back:
+        V's leading widths[j] rows and min(lowest, width) columns, w padded
+        with +inf."""
         nprob, K = int(A.shape[0]), int(A.shape[1])
         cols = K if lowest is None else lowest
+        small = [j for j, wj in enumerate(widths) if 0 < wj <= QL_MAX_N]
+        if len(small) == nprob and K <= QL_MAX_N and nprob <= MAX_PROBLEMS:
+            return eigh_ragged_device(A, widths, cols)
         w = torch.full((nprob, cols), float("inf"), dtype=torch.float64, device=A.device)
         V = torch.zeros((nprob, K, cols), dtype=torch.complex128, device=A.device)
-        for k in sorted(set(widths)):
+        if small:
+            kq = max(widths[j] for j in small)
+            jt = torch.as_tensor(small, device=A.device)
+            lo = min(cols, kq)
+            wq, Vq = eigh_ragged_device(A[jt][:, :kq, :kq].contiguous(), [widths[j] for j in small], lo)
+            w[jt, :lo] = wq
+            V[jt, :kq, :lo] = Vq
+        for k in sorted(set(wj for wj in widths if wj > QL_MAX_N)):
             js = [j for j, wj in enumerate(widths) if wj == k]
             jt = torch.as_tensor(js, device=A.device)
             lo = None if lowest is None else min(lowest, k)
-            wk, Vk = eigh_batched_device(A[jt][:, :k, :k].contiguous(), lowest=lo)
+            wk, Vk = eigh_batched_device(A[jt][:, :k, :k].contiguous(), lowest=lo, method="dc")
             w[jt, :wk.shape[1]] = wk
             V[jt, :k, :Vk.shape[2]] = Vk
         return w, V

@@ -17,6 +17,7 @@
 #include "../../include/eigenfns_hip.h"
 #include "blas_kernels.hpp"
 #include "eig_kernels.hpp"
+#include "heevql_kernels.hpp"
 #include "cholqr_kernels.hpp"
 #include "operator_kernels.hpp"
 #include "raster_kernels.hpp"
@@ -1354,6 +1355,40 @@ int efn_heev_batched(int nprob, int n, const void* A, double* w, void* V, int ma
   }
   hipLaunchKernelGGL(efn::heev_jacobi_kernel, dim3((unsigned)nprob), dim3(efn::kEigThreads), lds,
                      (hipStream_t)stream, (const double2*)A, n, max_sweeps, w, (double2*)V, (const int*)nullptr);
+  EFN_LAUNCH_CHECK();
+  return EFN_OK;
+}
+
+int efn_heev_ql_batched(int nprob, int n, const int* n_each, const void* A, int lowest, double* w, void* V,
+                        int* info, void* stream) {
+  namespace ql = efn::ql;
+  if (nprob < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
+  if (n > ql::kMaxN) return fail(EFN_EINVAL, "n must be <= " + std::to_string(ql::kMaxN));
+  if (nprob == 0 || n == 0) return EFN_OK;
+  if (lowest < 1 || lowest > n) return fail(EFN_EINVAL, "lowest must be in [1, n]");
+  if (!A || !w || !V || !info) return fail(EFN_EINVAL, "NULL pointer");
+  const uintptr_t a0 = (uintptr_t)A, a1 = a0 + (uintptr_t)nprob * n * n * sizeof(double2);
+  const uintptr_t v0 = (uintptr_t)V, v1 = v0 + (uintptr_t)nprob * n * lowest * sizeof(double2);
+  if (a0 < v1 && v0 < a1) return fail(EFN_EINVAL, "A and V must not overlap");
+  ql::Dims dims;
+  memset(&dims, 0, sizeof(dims));
+  if (n_each) {
+    if (nprob > ql::kMaxRagged)
+      return fail(EFN_EINVAL, "per-problem sizes for at most " + std::to_string(ql::kMaxRagged) + " problems");
+    dims.ragged = 1;
+    for (int p = 0; p < nprob; ++p) {
+      if (n_each[p] < 0 || n_each[p] > n) return fail(EFN_EINVAL, "per-problem sizes must lie in [0, n]");
+      dims.n[p] = (short)n_each[p];
+    }
+  }
+  const size_t lds = (size_t)ql::Layout(n).bytes();
+  if (lds > 65536) {
+    const hipError_t e = hipFuncSetAttribute((const void*)ql::heev_ql_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return fail(EFN_EHIP, std::string("heev_ql LDS size: ") + hipGetErrorString(e));
+  }
+  hipLaunchKernelGGL(ql::heev_ql_kernel, dim3((unsigned)nprob), dim3(ql::kThreads), lds, (hipStream_t)stream,
+                     (const double2*)A, n, dims, lowest, w, (double2*)V, info);
   EFN_LAUNCH_CHECK();
   return EFN_OK;
 }

@@ -12,8 +12,9 @@ works on Gram matrices the device sums in FP64: the orthonormalisation of W
 and P is a pivoted Cholesky-QR on the device by default (ortho="cholqr"; the
 reference's SVQB eigendecompositions with ortho="svqb"), and the
 Rayleigh–Ritz eigenproblem (at most 3m × 3m) runs in host LAPACK as in the
-reference (solver.py:119-121, 263-265, 301), or on the device (rocSOLVER)
-with EFN_SMALL_DENSE=device and always when the iteration is graph-captured
+reference (solver.py:119-121, 263-265, 301), or on the device (the
+library's QL kernel up to 96 × 96, rocSOLVER above) with
+EFN_SMALL_DENSE=device and always when the iteration is graph-captured
 (graph=True).  Rayleigh
 quotients are normalised by ‖x‖² (the reference's interior.py:69-74 fix)
 unless ``normalized_rq=False``.
@@ -38,7 +39,7 @@ import torch
 
 from .linalg import (COMBINE_IN_PLACE_MAX, DEFLATE_CHUNK, RR_DEAD_PENALTY, SVQB_DROP, apply_chunked,
                      combine, combine_into, combine_multi_split, dead_spread, deflate, deflate_chunk_rows,
-                     eigh, eigh_batched_device,
+                     eigh, eigh_batched_device, QL_MAX_N,
                      gram, gram_herm, normalize_rows, ortho_block, ortho_coeffs, row_axpby,
                      row_axpby_norm_strided, rowdot, rownorm, rownorm2, rowstats_strided,
                      svqb_coeffs, ORTHO_METHODS)
@@ -150,10 +151,11 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
     orthonormalisation, solver.py:113-126).
     `graph`: the steady-state iteration (everything between two residual
     read-backs: preconditioner, projections, Cholesky-QR, Θ, the Rayleigh–Ritz
-    matrix; then the update, Θ(X) and the next residuals) is captured once per
-    block as two HIP graphs around the Rayleigh–Ritz eigensolve (rocSOLVER on
-    the device, uncaptured) and replayed: three submissions per iteration
-    instead of some forty launches, no host sync but the residual read-back;
+    matrix and its eigensolve on the device, the update, Θ(X) and the next
+    residuals) is captured once per block as one HIP graph and replayed — one
+    submission per iteration instead of some forty launches, no host sync but
+    the residual read-back — when 3m ≤ 96 (the library's QL eigensolver);
+    above that as two graphs around rocSOLVER's eigensolve (uncaptured);
     needs ortho="cholqr"; results are bitwise those of the uncaptured loop
     with device eigensolves."""
     if m <= guard:
@@ -309,12 +311,13 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             return residual_stats()
 
         # steady-state iterations (P present) of this block: one eager run
-        # (sizes every scratch buffer and rocSOLVER's workspace), then one
-        # capture of each side of the Rayleigh–Ritz eigensolve, then replays;
-        # the eigensolve itself (rocSOLVER, which copies host data inside a
-        # call and so cannot be captured) runs uncaptured between the two
-        # graphs on the same stream, without a host sync
+        # (sizes every scratch buffer), then one capture, then replays.  Up to
+        # 3m = 96 the whole iteration is one graph (the library's QL
+        # eigensolver allocates nothing); above, rocSOLVER (which copies host
+        # data inside a call and so cannot be captured) runs uncaptured
+        # between two graphs on the same stream, without a host sync
         g, g_stats, warm = None, None, False
+        one_graph = 3 * m <= QL_MAX_N
         sdev = residual_stats()
         for it in range(maxit):
             host = sdev.cpu().numpy()
@@ -338,7 +341,11 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                       f"{np.quantile(rn_h, 0.5):.1e}/{np.quantile(rn_h, 0.9):.1e}  "
                       f"leak {leak:.1e}", flush=True)
             if graph and have_p and warm:
-                if g is None:
+                if g is None and one_graph:
+                    g = (torch.cuda.CUDAGraph(),)
+                    with torch.cuda.graph(g[0]):
+                        g_stats = iteration_tail(True)
+                elif g is None:
                     g = (torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph())
                     with torch.cuda.graph(g[0]):
                         g_A, g_nb = tail_rr_matrix(True)
@@ -346,8 +353,9 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                     with torch.cuda.graph(g[1]):
                         g_stats = tail_update(g_V, g_nb)
                 g[0].replay()
-                g_V.copy_(small_eigh(g_A)[1])
-                g[1].replay()
+                if not one_graph:
+                    g_V.copy_(small_eigh(g_A)[1])
+                    g[1].replay()
                 sdev = g_stats
             else:
                 sdev = iteration_tail(have_p)

@@ -264,6 +264,22 @@ size_t efn_heevd_work_bytes(int nprob, int n);
 int efn_heevd_batched(int nprob, int n, const void* A_c128, double* w, void* V_c128, void* work,
                       size_t work_bytes, void* stream);
 
+/* The lowest `lowest` eigenpairs of nprob Hermitian matrices (n ≤ 96, FP64),
+ * one workgroup per matrix with everything in LDS: Householder reduction to
+ * a real tridiagonal matrix, implicit QL with Wilkinson shifts, and the
+ * back-transformation of the selected eigenvectors only.  Allocates nothing
+ * (can be captured in a HIP graph).  A: nprob × n × n complex128 row-major,
+ * only the lower triangle read; with n_each (host array, nprob ≤ 64) problem
+ * p is the leading n_each[p] × n_each[p] block of its n × n slot.  w: nprob ×
+ * lowest, ascending, +inf past min(lowest, n_p); V: nprob × n × lowest
+ * (V[p][i][j] = component i of eigenvector j, zero past n_p rows and
+ * min(lowest, n_p) columns); info (device, nprob ints): 0, or 1 + l when the
+ * QL iteration for eigenvalue l did not converge within 30·n_p iterations.
+ * A and V must not overlap.  The host and rocSOLVER eigensolves of
+ * solver.py:119-121, :263-265, :301 without a library workspace. */
+int efn_heev_ql_batched(int nprob, int n, const int* n_each, const void* A_c128, int lowest, double* w,
+                        void* V_c128, int* info_dev, void* stream);
+
 /* Process-wide combine engine: 1 = matrix cores (v_mfma_f32_16x16x4f32,
  * default, for m_out > 8), 0 = vector ALU; FP32 accumulation either way.
  * Returns the previous engine (engine < 0: query only).  No reference

@@ -1,7 +1,7 @@
 """GPU tests of the batched Hermitian eigensolvers — efn_heev_batched (the
-library's FP64 Jacobi kernel) and efn_heevd_batched (rocSOLVER's batched
-divide and conquer) — against LAPACK (numpy.linalg.eigh) on the same FP64
-matrices.
+library's FP64 Jacobi kernel), efn_heev_ql_batched (the library's tridiagonal
+QL kernel) and efn_heevd_batched (rocSOLVER's batched divide and conquer) —
+against LAPACK (numpy.linalg.eigh) on the same FP64 matrices.
 
 Bars: eigenvalues within 1e-13·max|λ| of LAPACK's; residual ‖AV − VΛ‖ and
 ‖VᴴV − I‖ below 1e-12 (relative to max|λ|); the smallest eigenvalues of a
@@ -41,7 +41,7 @@ def _check(A, w, V):
     return wr
 
 
-METHODS = ["dc", "jacobi"]
+METHODS = ["dc", "jacobi", "ql"]
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -91,7 +91,7 @@ def test_heev_batch_invariant_and_limits(method):
     for j in range(6):
         w1, V1 = eigh_batched_device(mats[j:j + 1], method=method)
         assert torch.equal(w1[0], wb[j]) and torch.equal(V1[0], Vb[j])
-    if method == "jacobi":
+    if method in ("jacobi", "ql"):
         with pytest.raises(ValueError):
             eigh_batched_device(torch.zeros((1, 97, 97), dtype=torch.complex128, device="cuda"),
                                 method=method)
@@ -132,3 +132,33 @@ def test_heev_decoupled_dead_block(method):
     for j in range(3):
         wr = _check(mats[j], w[j], V[j])
         assert np.abs(w[j][:24] - wr[:24]).max() < 1e-10 * np.abs(wr[:24]).min()
+
+
+def test_heev_ql_ragged_sizes_in_one_launch():
+    """Problems of sizes 1-96 padded into one (nprob, 96, 96) stack, the lowest
+    24 of each in one launch: each problem to the LAPACK bars, +inf / zero
+    padding past its own size, and bitwise the same as the problem alone."""
+    import torch
+    from eigenfns_amd.linalg import eigh_ragged_device
+    rng = np.random.default_rng(21)
+    sizes = [1, 5, 24, 30, 48, 72, 95, 96, 0]
+    K, low = 96, 24
+    A = np.zeros((len(sizes), K, K), np.complex128)
+    for j, n in enumerate(sizes):
+        if n:
+            A[j, :n, :n] = _rr_like(rng, n, dead=(n - 1,) if n > 3 else ())
+    At = torch.from_numpy(A).cuda()
+    w, V = eigh_ragged_device(At, sizes, low)
+    wn, Vn = w.cpu().numpy(), V.cpu().numpy()
+    for j, n in enumerate(sizes):
+        k = min(low, n)
+        if n:
+            wr = np.linalg.eigvalsh(A[j, :n, :n])
+            scale = np.abs(wr).max()
+            assert np.abs(wn[j, :k] - wr[:k]).max() < 1e-13 * scale
+            Vj = Vn[j, :n, :k]
+            assert np.abs(A[j, :n, :n] @ Vj - Vj * wn[j, :k]).max() < 1e-12 * scale
+            assert np.abs(Vj.conj().T @ Vj - np.eye(k)).max() < 1e-12
+        assert np.isinf(wn[j, k:]).all() and not Vn[j, n:].any() and not Vn[j, :, k:].any()
+        w1, V1 = eigh_ragged_device(At[j:j + 1], [n], low)
+        assert torch.equal(w1[0], w[j]) and torch.equal(V1[0], V[j])
