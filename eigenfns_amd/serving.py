@@ -101,7 +101,7 @@ def _summary(done, wall, turns, engines):
 
 def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slots=32,
                   hbm_budget=200e9, verbose=False, threads=True, progress=False,
-                  engines_per_grid=1, engine_voxels=None, small_dense="host", max_nev=None,
+                  engines_per_grid="auto", engine_voxels=None, small_dense="host", max_nev=None,
                   ortho="cholqr"):
     """Run the stream through per-grid batched engines.  Returns (results, summary).
 
@@ -109,14 +109,17 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     stream (default priority), so one engine's host work (small eigensolves,
     admission) overlaps the others' kernels, and small kernels of different
     engines can share the GPU.  Requests are admitted FIFO per grid.
-    engines_per_grid > 1 splits each grid's slots over that many engines
+    engines_per_grid (an int, a dict grid → count, or "auto": two engines
+    for the largest grid of the stream, one for each other grid) > 1 splits a
+    grid's slots over that many engines
     (threads, streams) pulling from the same queue, so one half's host work
     overlaps the other half's kernels: 2 measured +16 % solves/s over 1 (48
     requests, 32³–64³) while each step's host work was several milliseconds
     of Python per engine; with the slot views cached and the engine step's
-    host work down to the eigensolves, one engine per grid is as fast or
-    faster (serving sample: 11.5–11.7 against 9.5–11.0 solves/s), the
-    kernels of the 64³ engine being what the stream waits for.
+    host work down to the eigensolves, two engines for every grid lose to
+    the interpreter contention of six engine threads (serving sample 9.5–11.0
+    against 11.5–11.8 solves/s with one each), while two for the largest grid
+    only — the engine the stream waits for — measured 12.5 ("auto").
 
     Slots are sized for the service's band limit `max_nev` (default: the
     request stream configuration's upper bound, 64), not for the requests
@@ -143,11 +146,18 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     lock = threading.Lock()
     t0 = time.perf_counter()
 
-    k = max(1, int(engines_per_grid))
+    if engines_per_grid == "auto":
+        engines_per_grid = {max(grids): 2}
+    if isinstance(engines_per_grid, dict):   # per grid (keys may be strings)
+        per = {int(g): max(1, int(v)) for g, v in engines_per_grid.items()}
+        kmap = {g: per.get(g, 1) for g in grids}
+    else:
+        kmap = {g: max(1, int(engines_per_grid)) for g in grids}
 
     def worker(g, e):
         try:
             stream = torch.cuda.Stream()
+            k = kmap[g]
             with torch.cuda.stream(stream):
                 pending = queues[g]
                 nslots = max(1, min(-(-plan[g] // k), -(-len(pending) // k)))
@@ -189,7 +199,7 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
             errors.append(exc)
 
     workers = [threading.Thread(target=worker, args=(g, e), name=f"engine-{g}.{e}")
-               for g in grids for e in range(k)]
+               for g in grids for e in range(kmap[g])]
     with one_blas_thread():   # host BLAS at one thread while the engines' pools run
         for w in workers:
             w.start()
@@ -281,7 +291,8 @@ if __name__ == "__main__":
     ap.add_argument("--plain", action="store_true")
     ap.add_argument("--slots", type=int, default=32)
     ap.add_argument("--sequential", action="store_true", help="one host thread, one stream")
-    ap.add_argument("--engines-per-grid", type=int, default=1)
+    ap.add_argument("--engines-per-grid", default="auto",
+                    help='"auto" (two engines for the largest grid) or an engine count per grid')
     ap.add_argument("--small-dense", choices=["device", "host", "hybrid"], default="host")
     ap.add_argument("--ortho", choices=["svqb", "cholqr"], default="cholqr")
     args = ap.parse_args()
@@ -289,7 +300,9 @@ if __name__ == "__main__":
     specs = request_stream(args.n, args.seed, grids=grids, nev_range=(8, args.nev_max))
     cache = StructureCache()
     res, summ = serve_batched(specs, cache, max_slots=args.slots, verbose=True,
-                              threads=not args.sequential, engines_per_grid=args.engines_per_grid,
+                              threads=not args.sequential,
+                              engines_per_grid=(args.engines_per_grid if args.engines_per_grid == "auto"
+                                                else int(args.engines_per_grid)),
                               small_dense=args.small_dense, ortho=args.ortho)
     print(json.dumps({"batched": summ}), flush=True)
     if args.plain:
