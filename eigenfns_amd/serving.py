@@ -99,6 +99,19 @@ def _summary(done, wall, turns, engines):
             "host_phase_s": timing, **_stats(lat)}
 
 
+def engines_for(grids, engines_per_grid="auto") -> dict:
+    """Engines per grid size: "auto" gives the largest grid two and every other
+    grid one; an int applies to every grid; a dict (grid → count, keys may be
+    strings) to the grids it names, one elsewhere.  At least one each."""
+    grids = sorted(int(g) for g in grids)
+    if engines_per_grid == "auto":
+        engines_per_grid = {grids[-1]: 2} if grids else 1
+    if isinstance(engines_per_grid, dict):
+        per = {int(g): max(1, int(v)) for g, v in engines_per_grid.items()}
+        return {g: per.get(g, 1) for g in grids}
+    return {g: max(1, int(engines_per_grid)) for g in grids}
+
+
 def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slots=32,
                   hbm_budget=200e9, verbose=False, threads=True, progress=False,
                   engines_per_grid="auto", engine_voxels=None, small_dense="host", max_nev=None,
@@ -146,13 +159,7 @@ def serve_batched(specs, cache=None, m=24, guard=8, tol=1e-4, maxit=400, max_slo
     lock = threading.Lock()
     t0 = time.perf_counter()
 
-    if engines_per_grid == "auto":
-        engines_per_grid = {max(grids): 2} if grids else 1
-    if isinstance(engines_per_grid, dict):   # per grid (keys may be strings)
-        per = {int(g): max(1, int(v)) for g, v in engines_per_grid.items()}
-        kmap = {g: per.get(g, 1) for g in grids}
-    else:
-        kmap = {g: max(1, int(engines_per_grid)) for g in grids}
+    kmap = engines_for(grids, engines_per_grid)
 
     def worker(g, e):
         try:

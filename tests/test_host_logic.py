@@ -102,3 +102,15 @@ def test_rr_dead_rows_with_masks_narrower_than_the_engine():
         rr_dead_rows(widths, K, xn, keep_w[:, :wa - 1], keepp)
     with pytest.raises(ValueError):
         rr_dead_rows(widths, K - 1, xn, keep_w, keepp)
+
+
+def test_engines_per_grid_policy():
+    """Two engines for the largest grid by default, one elsewhere; an int for
+    every grid; a dict (string keys accepted) for the grids it names."""
+    from eigenfns_amd.serving import engines_for
+    assert engines_for([48, 32, 64]) == {32: 1, 48: 1, 64: 2}
+    assert engines_for([32]) == {32: 2}
+    assert engines_for([]) == {}
+    assert engines_for([32, 64], 3) == {32: 3, 64: 3}
+    assert engines_for([32, 64], 0) == {32: 1, 64: 1}
+    assert engines_for([32, 48, 64], {"48": 2}) == {32: 1, 48: 2, 64: 1}
