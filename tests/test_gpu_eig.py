@@ -162,3 +162,19 @@ def test_heev_ql_ragged_sizes_in_one_launch():
         assert np.isinf(wn[j, k:]).all() and not Vn[j, n:].any() and not Vn[j, :, k:].any()
         w1, V1 = eigh_ragged_device(At[j:j + 1], [n], low)
         assert torch.equal(w1[0], w[j]) and torch.equal(V1[0], V[j])
+
+
+def test_heev_ql_more_columns_than_the_stack():
+    """Asking for more eigenpairs than the stack's size (an engine whose
+    requests are all narrower than a third of its width) pads w with +inf and
+    V with zero columns; the leading columns are the full solve's."""
+    import torch
+    from eigenfns_amd.linalg import eigh_ragged_device
+    rng = np.random.default_rng(31)
+    A = torch.from_numpy(np.stack([_herm(rng, 24), _herm(rng, 24)])).cuda()
+    w, V = eigh_ragged_device(A, [24, 10], 32)
+    wf, Vf = eigh_ragged_device(A, [24, 10], 24)
+    assert w.shape == (2, 32) and V.shape == (2, 24, 32)
+    assert torch.equal(w[:, :24], wf) and torch.equal(V[:, :, :24], Vf)
+    assert torch.isinf(w[:, 24:]).all() and not V[:, :, 24:].any()
+    assert torch.isinf(w[1, 10:]).all()
