@@ -8,7 +8,8 @@ Cases (all from eigenfns_amd.workloads):
   c2dense   C2's network (1000 points) rasterised at 16³, Γ: lowest 128
             eigenvalues of the dense FP64 matrix (n = 8192).
   c3_k<j>   C3's network at 64³ for the two k-points of the path with the
-            narrowest blocks: oracle LOBPCG with the configuration's width,
+            narrowest blocks (case c3) and for the widest and a median-width
+            one (case c3wide): oracle LOBPCG with the configuration's width,
             nev = width − 4, guard 4, tol 1e-8.
   srv_<rid> the serving sample's requests at 32³ with nev ≤ 32: oracle
             LOBPCG, block 32, guard 8, tol 1e-8.
@@ -82,6 +83,27 @@ def case_c3():
     return out
 
 
+def case_c3wide():
+    """C3's widest k-point and the one whose width is the median of the path
+    (first by index): oracle LOBPCG at their own widths, guard 4."""
+    probs = wl.c3_problems()
+    widths = sorted(w for _, w, _ in probs)
+    med = widths[len(widths) // 2]
+    wide = max(range(len(probs)), key=lambda j: (probs[j][1], -j))
+    mid = min(range(len(probs)), key=lambda j: (abs(probs[j][1] - med), j))
+    eps, L = wl.network(wl.C3["n_points"], wl.C3["grid"])
+    out = {}
+    for j in (mid, wide):
+        k, width, nev = probs[j]
+        t0 = time.perf_counter()
+        vals, _ = _lobpcg(eps, L, k, nev, width, wl.C3["guard"])
+        out[f"c3_k{j}"] = (vals, dict(n_points=wl.C3["n_points"], grid=wl.C3["grid"], k=k,
+                                      width=width, nev=nev, index=j, solver="oracle LOBPCG",
+                                      tol=TOL))
+        print(f"c3_k{j}: width {width}, nev {nev} in {time.perf_counter() - t0:.0f} s", flush=True)
+    return out
+
+
 def case_srv():
     out = {}
     for sp in wl.serving_oracle_subset(wl.serving_sample()):
@@ -118,7 +140,8 @@ def case_c5():
     return out
 
 
-CASES = {"c1": case_c1, "c2dense": case_c2dense, "c3": case_c3, "srv": case_srv, "c5": case_c5}
+CASES = {"c1": case_c1, "c2dense": case_c2dense, "c3": case_c3, "c3wide": case_c3wide, "srv": case_srv,
+         "c5": case_c5}
 
 
 def main(names):

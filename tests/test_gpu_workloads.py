@@ -72,7 +72,9 @@ def test_c3_kpoints_batched_match_fp64_oracle(golden):
     from eigenfns_amd.batched import Request, solve_batched
     from eigenfns_amd.operator import MaxwellOperator
     z, meta = golden
-    keys = sorted(k for k in meta if k.startswith("c3_k"))
+    # the golden k-points whose bands fit a 16-wide block (the wider ones run
+    # at their own widths in the ragged-engine test below)
+    keys = sorted(k for k in meta if k.startswith("c3_k") and meta[k]["width"] <= 16)
     if not keys:
         pytest.skip("no C3 k-points in solve_golden.npz")
     eps, L = wl.network(wl.C3["n_points"], wl.C3["grid"])
