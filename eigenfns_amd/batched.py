@@ -148,12 +148,12 @@ class BatchedLOBPCG:
         self.verify_dead_rows = verify_dead_rows  # also measure the row norms and compare (tests)
         # the SVQB / whitening / Rayleigh–Ritz eigensolves: "host" (LAPACK over a
         # thread pool, masks and normalisation in NumPy; the default), "device"
-        # (rocSOLVER's batched divide and conquer, efn_heevd_batched, one call
-        # per distinct size, and the masks on the device: no host round trip)
-        # or "hybrid" (SVQB and whitening on the device, Rayleigh–Ritz on the
-        # host).  A batched device call takes about 1.5 ms whatever the batch,
-        # against 0.4 ms for one host LAPACK call at 72 × 72 (many calls run
-        # at once in the pool), so the host stays the default.
+        # (the library's QL kernel in one ragged launch up to 96 × 96,
+        # rocSOLVER per distinct size above, and the masks on the device: no
+        # host round trip) or "hybrid" (SVQB and whitening on the device,
+        # Rayleigh–Ritz on the host).  A device call takes about 2.3 ms at
+        # 72 × 72 whatever the batch, against 0.4 ms for one host LAPACK call
+        # (many run at once in the pool), so the host stays the default.
         if small_dense not in ("device", "host", "hybrid"):
             raise ValueError(f"small_dense must be 'device', 'host' or 'hybrid', got {small_dense!r}")
         self.small_dense = small_dense

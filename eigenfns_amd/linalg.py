@@ -651,8 +651,12 @@ def eigh_ragged_device(A: torch.Tensor, sizes, lowest: int | None = None):
     lowest), zero past sizes[p] rows and min(lowest, sizes[p]) columns.  A
     problem whose iteration failed gets NaN eigenvalues."""
     import ctypes as Ct
+    if A.dim() != 3 or A.shape[1] != A.shape[2]:
+        raise ValueError(f"eigh_ragged_device: expected a (nprob, K, K) stack, got {tuple(A.shape)}")
     A = A.to(torch.complex128).contiguous()
     nprob, K = int(A.shape[0]), int(A.shape[1])
+    if sizes is not None and len(sizes) != nprob:
+        raise ValueError(f"eigh_ragged_device: {len(sizes)} sizes for {nprob} problems")
     want = K if lowest is None else int(lowest)
     low = min(want, K)  # more columns than the stack has: padding only
     w = torch.empty((nprob, low), dtype=torch.float64, device=A.device)
