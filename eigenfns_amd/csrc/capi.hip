@@ -480,7 +480,7 @@ int rowreduce_nsplit(long long n) {
 
 extern "C" {
 
-int efn_version(void) { return 300; }  // 0.3.0: three-pass Θ, efn_theta_engine / efn_theta_passes, efn_gram_noalloc
+int efn_version(void) { return 400; }  // 0.4.0: ragged launches, selected-row statistics, efn_heev_ql_batched
 
 const char* efn_last_error(void) { return g_last_error.c_str(); }
 
