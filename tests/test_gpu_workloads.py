@@ -3,9 +3,10 @@
 The fixture tests/golden/solve_golden.npz (written on the CPU by
 tests/golden/make_solve_golden.py) holds: C1's 24 bands from the oracle's FP64
 LOBPCG at 32³; the dense FP64 spectrum (lowest 128) of C2's network at 16³;
-C3's two narrowest k-points at 64³ from the oracle LOBPCG; the serving
-sample's 32³ requests with nev ≤ 32 from the oracle LOBPCG.  Bar: 1e-6
-relative eigenvalue error, the accuracy the serving metric is quoted at.
+C3's two narrowest k-points (width 8), a median-width one (30) and the widest
+(48) at 64³ from the oracle LOBPCG; the serving sample's 32³ requests with
+nev ≤ 32 and one C5 request on each of 48³–128³ from the oracle LOBPCG.  Bar:
+1e-6 relative eigenvalue error, the accuracy the serving metric is quoted at.
 """
 import numpy as np
 import pytest
