@@ -115,14 +115,17 @@ std::atomic<int> g_theta_engine{EFN_THETA_AUTO};
 // registers and LDS up to 128³.  At 192³ the plane (295 KB) does not, and the
 // three passes run on half-planes (theta3s_kernels.hpp); 256³ keeps five.
 template <int G>
-constexpr bool has_three_pass() { return G == 64 || G == 96 || G == 128 || efn::split_pass_grid(G); }
+constexpr bool has_three_pass() {
+  return G == 32 || G == 48 || G == 64 || G == 96 || G == 128 || efn::split_pass_grid(G);
+}
 
 // The pass structure used for grid G when the engine is EFN_THETA_AUTO: three
 // passes wherever whole planes fit (Θ on a 32-vector block, 10,000-point
-// network, default chunk: 64³ 128,026 vs 105,982 vector-applies/s, 96³ 29,107
-// vs 28,800, 128³ 14,413 vs 13,342), five at 192³, where the half-plane three
-// passes move 137 instead of 229 B per voxel but issue-bound measured 3,758
-// against 3,860 vector-applies/s.
+// network, default chunk: 32³ 818,490 vs 580,530 vector-applies/s, 48³
+// 292,861 vs 231,907, 64³ 128,026 vs 105,982, 96³ 29,107 vs 28,800, 128³
+// 14,413 vs 13,342), five at 192³, where the half-plane three passes move 137
+// instead of 229 B per voxel but issue-bound measured 3,758 against 3,860
+// vector-applies/s.
 template <int G>
 constexpr int theta_passes_auto() { return has_three_pass<G>() && !efn::split_pass_grid(G) ? 3 : 5; }
 

@@ -267,6 +267,22 @@ struct MidCfg<96> {
   static constexpr int WPE = 1;
 };
 template <>
+struct MidCfg<48> {
+  using PZ = Plan<48, false>;   // 6 × 8
+  using PZF = Plan<48, true>;   // 8 × 6
+  using PY = Plan<48, false>;   // Y1 = 6, Y2 = 8
+  static constexpr int NT = 96, BG = 2;  // (G·Y1 = 288 rows of tasks: 96-thread blocks)
+  static constexpr int WPE = 1;
+};
+template <>
+struct MidCfg<32> {
+  using PZ = Plan<32, false>;   // 4 × 8
+  using PZF = Plan<32, true>;   // 8 × 4
+  using PY = Plan<32, false>;   // Y1 = 4, Y2 = 8
+  static constexpr int NT = 128, BG = 4;
+  static constexpr int WPE = 1;
+};
+template <>
 struct MidCfg<64> {
   using PZ = Plan<64, false>;   // 8 × 8
   using PZF = Plan<64, true>;

@@ -135,7 +135,7 @@ int efn_kpm_count_step(efn_ctx* ctx, const void* T1_c64, void* T0_c64, const voi
  * Chebyshev / KPM epilogues); returns the previous engine (engine < 0: query).
  *   EFN_THETA_AUTO  the per-grid choice compiled into the library (default)
  *   3               xin | yzmid | xout: 132 B of compulsory traffic per voxel
- *                   and vector, on grids that have the plane pass (64, 96, 128)
+ *                   and vector, on grids that have the plane pass (32, 48, 64, 96, 128)
  *   5               zin | y | x·ε·x | y | zout: 228 B per voxel and vector
  * Either form computes Θ of operator.py:82-95 (the two differ only by FP32
  * rounding).  No reference counterpart (XLA schedules the reference's FFTs). */

@@ -1,6 +1,6 @@
 """The three-pass Θ against the FP64 oracle, with the pass structure forced to
-3 on the grids that have it: whole planes (xin | yzmid | xout) at 64³, 96³ and
-128³, half-planes with the z-FFT's radix-2 stages in the x passes
+3 on the grids that have it: whole planes (xin | yzmid | xout) at 32³, 48³,
+64³, 96³ and 128³, half-planes with the z-FFT's radix-2 stages in the x passes
 (xin_split | yzlds | xout_split) at 192³.  Θ, the preconditioner and the fused
 Chebyshev / KPM / KPM-count steps at rel L2 < 1e-5 per vector (moments and
 estimates at 1e-5 of their dot scale), the three-pass and five-pass forms
@@ -35,7 +35,9 @@ def _eps(G, seed):
     return np.where(rng.random((G, G, G)) < 0.3, 8.41, 1.0).astype(np.float32)
 
 
-@pytest.mark.parametrize("G,k", [(64, (0, 0, 0)), (64, (0.5, 0.5, 0.5)), (96, (0.1, 0.0, -0.3)),
+@pytest.mark.parametrize("G,k", [(32, (0, 0, 0)), (32, (0.3, -0.2, 0.1)), (48, (0, 0, 0)),
+                                 (48, (0.5, 0.0, 0.25)),
+                                 (64, (0, 0, 0)), (64, (0.5, 0.5, 0.5)), (96, (0.1, 0.0, -0.3)),
                                  (96, (0, 0, 0)), (128, (0, 0, 0)), (128, (0.25, -0.1, 0.4)),
                                  (192, (0, 0, 0)), (192, (0.25, -0.1, 0.4))])
 def test_three_pass_theta_and_precondition_match_oracle(three, G, k):
@@ -69,7 +71,7 @@ def test_three_and_five_pass_agree_and_chunking_is_bitwise(three, G):
     assert _rel(a.cpu().numpy(), c.cpu().numpy()).max() < TOL
 
 
-@pytest.mark.parametrize("G", [128, 192])
+@pytest.mark.parametrize("G", [48, 128, 192])
 def test_three_pass_multi_slot_equals_single(three, G):
     import torch
     from eigenfns_amd.operator import MaxwellOperator, theta_multi
@@ -86,7 +88,7 @@ def test_three_pass_multi_slot_equals_single(three, G):
         assert torch.equal(Y[i * nrow:(i + 1) * nrow], single)
 
 
-@pytest.mark.parametrize("G", [64, 128, 192])
+@pytest.mark.parametrize("G", [32, 48, 64, 128, 192])
 def test_three_pass_fused_steps_on_the_c4_network(three, G):
     import torch
     from eigenfns_amd.operator import MaxwellOperator
@@ -128,7 +130,7 @@ def test_three_pass_fused_steps_on_the_c4_network(three, G):
     assert (np.abs(est.cpu().numpy() - want) / zs).max() < TOL
 
 
-@pytest.mark.parametrize("G", [64, 96, 128])
+@pytest.mark.parametrize("G", [32, 48, 64, 96, 128])
 def test_five_pass_still_matches_oracle_where_three_is_default(G):
     """With the engine forced to five passes on the three-pass grids, Θ still
     meets the bar (the five-pass form stays the fallback there)."""
