@@ -132,13 +132,14 @@ def path_rooflines(op, X, Y, G, stream, reps=5):
                              "frac_hbm": round(byts / t / 1e9 / HBM_PEAK_GBS, 4)}
     del S, C
     # small dense FP64 eigensolves of Rayleigh–Ritz size (72 × 72 Hermitian,
-    # lowest 24): one matrix and a batch of 16, the library's QL kernel
-    # against rocSOLVER's divide and conquer (latency-bound; no roofline)
+    # lowest 24): one matrix and a batch of 16, the library's bisection +
+    # inverse-iteration and QL kernels against rocSOLVER's divide and conquer
+    # (latency-bound; no roofline)
     from eigenfns_amd.linalg import eigh_batched_device
     gen = torch.Generator(device=X.device).manual_seed(7)
     R = torch.randn((16, 72, 72), dtype=torch.complex128, device=X.device, generator=gen)
     R = R + R.conj().transpose(1, 2)
-    for method in ("ql", "dc"):
+    for method in ("heevx", "ql", "dc"):
         for nb in (1, 16):
             t = _launch_seconds(lambda: eigh_batched_device(R[:nb], lowest=24, method=method), reps, stream)
             out[f"heev_72_lowest24_x{nb}_{method}"] = {"launch_ms": round(t * 1e3, 3)}

@@ -68,6 +68,7 @@ _SIGNATURES = {
     "efn_heevd_work_bytes": (_sz, [_i, _i]),
     "efn_heevd_batched": (_i, [_i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
     "efn_heev_ql_batched": (_i, [_i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "efn_heevx_batched": (_i, [_i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "efn_cholqr_batched": (_i, [_i, _i, _vp, _vp, _d, _vp, _vp]),
     "efn_rowreduce_work_bytes": (_sz, [_i, _i64]),
     "efn_rowreduce": (_i, [_i, _vp, _vp, _i, _i64, _vp, _vp, _sz, _vp]),

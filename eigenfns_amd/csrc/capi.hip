@@ -1362,8 +1362,12 @@ int efn_heev_batched(int nprob, int n, const void* A, double* w, void* V, int ma
   return EFN_OK;
 }
 
-int efn_heev_ql_batched(int nprob, int n, const int* n_each, const void* A, int lowest, double* w, void* V,
-                        int* info, void* stream) {
+namespace {
+// Shared checks and launch of the one-workgroup-per-matrix eigensolvers
+// (efn_heev_ql_batched, efn_heevx_batched): same arguments, same layouts.
+using HeevKernel = void (*)(const double2*, int, efn::ql::Dims, int, double*, double2*, int*);
+int heev_small_launch(HeevKernel kernel, const char* name, size_t lds, int nprob, int n, const int* n_each,
+                      const void* A, int lowest, double* w, void* V, int* info, void* stream) {
   namespace ql = efn::ql;
   if (nprob < 0 || n < 0) return fail(EFN_EINVAL, "negative size");
   if (n > ql::kMaxN) return fail(EFN_EINVAL, "n must be <= " + std::to_string(ql::kMaxN));
@@ -1384,16 +1388,29 @@ int efn_heev_ql_batched(int nprob, int n, const int* n_each, const void* A, int 
       dims.n[p] = (short)n_each[p];
     }
   }
-  const size_t lds = (size_t)ql::Layout(n).bytes();
   if (lds > 65536) {
-    const hipError_t e = hipFuncSetAttribute((const void*)ql::heev_ql_kernel,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return fail(EFN_EHIP, std::string("heev_ql LDS size: ") + hipGetErrorString(e));
+    const hipError_t e =
+        hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return fail(EFN_EHIP, std::string(name) + " LDS size: " + hipGetErrorString(e));
   }
-  hipLaunchKernelGGL(ql::heev_ql_kernel, dim3((unsigned)nprob), dim3(ql::kThreads), lds, (hipStream_t)stream,
+  hipLaunchKernelGGL(kernel, dim3((unsigned)nprob), dim3(ql::kThreads), lds, (hipStream_t)stream,
                      (const double2*)A, n, dims, lowest, w, (double2*)V, info);
   EFN_LAUNCH_CHECK();
   return EFN_OK;
+}
+}  // namespace
+
+int efn_heev_ql_batched(int nprob, int n, const int* n_each, const void* A, int lowest, double* w, void* V,
+                        int* info, void* stream) {
+  const size_t lds = n > 0 && n <= efn::ql::kMaxN ? (size_t)efn::ql::Layout(n).bytes() : 0;
+  return heev_small_launch(efn::ql::heev_ql_kernel, "heev_ql", lds, nprob, n, n_each, A, lowest, w, V, info,
+                           stream);
+}
+
+int efn_heevx_batched(int nprob, int n, const int* n_each, const void* A, int lowest, double* w, void* V,
+                      int* info, void* stream) {
+  const size_t lds = n > 0 && n <= efn::ql::kMaxN ? (size_t)efn::ql::heevx_lds_bytes(n) : 0;
+  return heev_small_launch(efn::ql::heevx_kernel, "heevx", lds, nprob, n, n_each, A, lowest, w, V, info, stream);
 }
 
 int efn_cholqr_batched(int nprob, int n, const void* G, void* C, double drop, int* rank, void* stream) {

@@ -73,8 +73,13 @@ struct Layout {
   EFN_QL_HD explicit Layout(int n_) : n(n_), ldz(n_ + 1), npk(n_ * (n_ + 1) / 2) {}
   // byte offsets, each a multiple of 16
   EFN_QL_HD long a_off() const { return 0; }                                   // cd[npk]
-  EFN_QL_HD long z_off() const { return a_off() + 16L * npk; }                 // double[n·ldz]
-  EFN_QL_HD long d_off() const { return z_off() + 8L * ((n * ldz + 1) & ~1); }  // double[n]
+  // Z: n·ldz doubles, at least 6n (one thread's inverse-iteration scratch)
+  EFN_QL_HD long zsz() const {
+    const long a = (long)n * ldz, b = 6L * n;
+    return ((a > b ? a : b) + 1) & ~1L;
+  }
+  EFN_QL_HD long z_off() const { return a_off() + 16L * npk; }                 // double[zsz]
+  EFN_QL_HD long d_off() const { return z_off() + 8L * zsz(); }                 // double[n]
   EFN_QL_HD long e_off() const { return d_off() + 8L * ((n + 1) & ~1); }        // double[n]
   EFN_QL_HD long tau_off() const { return e_off() + 8L * ((n + 1) & ~1); }      // cd[n]
   EFN_QL_HD long u_off() const { return tau_off() + 16L * n; }                  // cd[n] (v)
@@ -86,7 +91,7 @@ struct Layout {
 };
 
 // control words
-enum { kCtlL = 0, kCtlIter = 1, kCtlFail = 2, kCtlCnt = 4, kCtlLo = 6, kCtlDone = 8, kCtlPerm = 16 };
+enum { kCtlL = 0, kCtlIter = 1, kCtlFail = 2, kCtlClus = 3, kCtlCnt = 4, kCtlLo = 6, kCtlDone = 8, kCtlPerm = 16 };
 
 // Block reduction of a complex partial (every thread passes one); the sum
 // is returned to every thread.  Two barriers.  On the device each wave sums
@@ -284,29 +289,59 @@ EFN_QL_HD inline void ql_apply_row(double* __restrict__ zr, const double* __rest
   zr[lo] = carry;
 }
 
-// y_i = τ·Σ_j A₂₂(i, j)·v_j for rows i of the trailing block at k+1 (size m),
-// A₂₂ Hermitian from the packed lower triangle
-EFN_QL_HD inline void ql_hemv_row(const cd* __restrict__ a, const cd* __restrict__ u, cd* __restrict__ y, int k,
-                                  int m, int i, cd tk) {
+// Σ_j A₂₂(i, j)·v_j over the columns j ≡ q (mod G) of row i of the trailing
+// block at k+1 (size m), A₂₂ Hermitian from the packed lower triangle
+EFN_QL_HD inline void hemv_part(const cd* __restrict__ a, const cd* __restrict__ u, int k, int m, int i, int q,
+                                int G, double& sx, double& sy) {
   const int gi = k + 1 + i;
-  double sx = 0.0, sy = 0.0;
   const cd* __restrict__ row = a + pk(gi, k + 1);
-  for (int j = 0; j < i; ++j) {
+  int j = q;
+  for (; j < i; j += G) {
     const cd aij = row[j], v = u[j];
     sx += aij.x * v.x - aij.y * v.y;
     sy += aij.x * v.y + aij.y * v.x;
   }
-  {
+  if (j == i) {
     const double aii = a[pk(gi, gi)].x;
     sx += aii * u[i].x;
     sy += aii * u[i].y;
+    j += G;
   }
-  for (int j = i + 1; j < m; ++j) {
+  for (; j < m; j += G) {
     const cd aji = a[pk(k + 1 + j, gi)], v = u[j];  // A(i, j) = conj(A(j, i))
     sx += aji.x * v.x + aji.y * v.y;
     sy += aji.x * v.y - aji.y * v.x;
   }
-  y[i] = cmul(tk, cmk(sx, sy));
+}
+
+// y = τ·A₂₂·v over the trailing block at k+1 (size m).  G = 4 threads a row
+// (2 when 4m exceeds the workgroup), adjacent lanes of one wave, each summing
+// the columns j ≡ q (mod G); the partials meet by lane shuffles, (S₀+S₁) +
+// (S₂+S₃).  The host sums the same partials in the same order.
+EFN_QL_HD inline void ql_hemv(const Team& t, const cd* __restrict__ a, const cd* __restrict__ u, cd* __restrict__ y,
+                              int k, int m, cd tk) {
+  const int G = 4 * m <= kThreads ? 4 : 2;  // 2m ≤ kThreads for m < kMaxN
+#ifdef __HIP_DEVICE_COMPILE__
+  const int i = t.tid / G, q = t.tid % G;
+  double sx = 0.0, sy = 0.0;
+  if (i < m) hemv_part(a, u, k, m, i, q, G, sx, sy);
+  for (int off = 1; off < G; off <<= 1) {
+    sx += __shfl_xor(sx, off, 64);
+    sy += __shfl_xor(sy, off, 64);
+  }
+  if (i < m && q == 0) y[i] = cmul(tk, cmk(sx, sy));
+#else
+  for (int i = t.tid; i < m; i += t.nthr) {
+    double px[4] = {0.0, 0.0, 0.0, 0.0}, py[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < G; ++q) hemv_part(a, u, k, m, i, q, G, px[q], py[q]);
+    for (int off = 1; off < G; off <<= 1)
+      for (int q = 0; q < G; q += 2 * off) {
+        px[q] += px[q + off];
+        py[q] += py[q + off];
+      }
+    y[i] = cmul(tk, cmk(px[0], py[0]));
+  }
+#endif
 }
 
 // (row, column) of entry q of a packed lower triangle
@@ -336,42 +371,59 @@ EFN_QL_HD inline void ql_her2(const Team& t, cd* __restrict__ a, const cd* __res
   }
 }
 
-// The solver for one matrix.  A: n_p × n_p of a row-major array with leading
-// dimension lda (only the lower triangle is read); w: `lowest` values
-// (ascending; +inf past min(lowest, n_p)); V: lda × lowest row-major
-// (V[i·lowest + j] = component i of eigenvector j; zero past n_p rows and
-// min(lowest, n_p) columns); info: 0, or 1 + l if the QL iteration for
-// eigenvalue l did not converge (then every w is NaN).  lds:
-// Layout(nl).bytes() bytes, nl ≥ n_p.
-EFN_QL_HD inline void heev_ql_body(const Team& t, const cd* A, int lda, int n, int nl, int lowest, double* w,
-                                   cd* V, int* info, char* lds) {
+// The LDS arrays of one matrix (offsets from Layout)
+struct Work {
+  cd* a;        // packed lower triangle; Householder vectors below the subdiagonal
+  double* z;    // rotation accumulator / scratch / back-transformation staging
+  double* d;    // tridiagonal: diagonal
+  double* sub;  // tridiagonal: subdiagonal
+  cd* tau;
+  cd* u;
+  cd* y;
+  double* cs;
+  cd* red;
+  int* ctl;
+  int ldz, csn;
+};
+EFN_QL_HD inline Work carve(int nl, char* lds) {
   const Layout L(nl);
-  cd* a = reinterpret_cast<cd*>(lds + L.a_off());
-  double* z = reinterpret_cast<double*>(lds + L.z_off());
-  double* d = reinterpret_cast<double*>(lds + L.d_off());
-  double* sub = reinterpret_cast<double*>(lds + L.e_off());
-  cd* tau = reinterpret_cast<cd*>(lds + L.tau_off());
-  cd* u = reinterpret_cast<cd*>(lds + L.u_off());
-  cd* y = reinterpret_cast<cd*>(lds + L.y_off());
-  double* cs = reinterpret_cast<double*>(lds + L.cs_off());
-  cd* red = reinterpret_cast<cd*>(lds + L.red_off());
-  int* ctl = reinterpret_cast<int*>(lds + L.ctl_off());
-  const int ldz = L.ldz;
-  const int csn = (nl + 1) & ~1;
-  const int low = lowest < n ? lowest : n;
-  const double inf = HUGE_VAL;
+  Work W;
+  W.a = reinterpret_cast<cd*>(lds + L.a_off());
+  W.z = reinterpret_cast<double*>(lds + L.z_off());
+  W.d = reinterpret_cast<double*>(lds + L.d_off());
+  W.sub = reinterpret_cast<double*>(lds + L.e_off());
+  W.tau = reinterpret_cast<cd*>(lds + L.tau_off());
+  W.u = reinterpret_cast<cd*>(lds + L.u_off());
+  W.y = reinterpret_cast<cd*>(lds + L.y_off());
+  W.cs = reinterpret_cast<double*>(lds + L.cs_off());
+  W.red = reinterpret_cast<cd*>(lds + L.red_off());
+  W.ctl = reinterpret_cast<int*>(lds + L.ctl_off());
+  W.ldz = L.ldz;
+  W.csn = (nl + 1) & ~1;
+  return W;
+}
 
-  // ---- load the lower triangle, Z = I ----------------------------------------
+// Load A's lower triangle (and Z = I when z_identity) and reduce it to the
+// real tridiagonal (d, sub) by Householder reflectors (LAPACK zhetd2's
+// steps); τ_k in tau, v_k below the subdiagonal of column k of a.
+EFN_QL_HD inline void reduce_tridiag(const Team& t, const cd* A, int lda, int n, const Work& W, bool z_identity) {
+  cd* a = W.a;
+  double* z = W.z;
+  double* d = W.d;
+  double* sub = W.sub;
+  cd* tau = W.tau;
+  cd* u = W.u;
+  cd* y = W.y;
+  const int ldz = W.ldz;
   for (int i = t.tid; i < n; i += t.nthr) {
     for (int j = 0; j <= i; ++j) a[pk(i, j)] = A[(long)i * lda + j];
-    for (int j = 0; j < n; ++j) z[i * ldz + j] = i == j ? 1.0 : 0.0;
+    if (z_identity)
+      for (int j = 0; j < n; ++j) z[i * ldz + j] = i == j ? 1.0 : 0.0;
   }
   if (t.tid == 0) {
-    for (int c = 0; c < 16; ++c) ctl[c] = 0;
+    for (int c = 0; c < 16; ++c) W.ctl[c] = 0;
   }
   t.sync();
-
-  // ---- Householder reduction to real tridiagonal (d, sub) ---------------------
   for (int k = 0; k + 1 < n; ++k) {
     const int m = n - k - 1;  // rows k+1 … n−1 of column k
     // ‖column below row k+1‖²
@@ -380,7 +432,7 @@ EFN_QL_HD inline void heev_ql_body(const Team& t, const cd* A, int lda, int n, i
       const cd x = a[pk(i, k)];
       part.x += x.x * x.x + x.y * x.y;
     }
-    const double xn2 = block_sum(t, red, part).x;
+    const double xn2 = block_sum(t, W.red, part).x;
     const cd alpha = a[pk(k + 1, k)];
     cd tk = cmk(0.0, 0.0), scale = cmk(0.0, 0.0);
     double beta = alpha.x;
@@ -412,7 +464,7 @@ EFN_QL_HD inline void heev_ql_body(const Team& t, const cd* A, int lda, int n, i
     t.sync();
     if (tk.x != 0.0 || tk.y != 0.0) {
       // y = τ·A₂₂·v over the trailing block (rows and columns k+1 …)
-      for (int i = t.tid; i < m; i += t.nthr) ql_hemv_row(a, u, y, k, m, i, tk);
+      ql_hemv(t, a, u, y, k, m, tk);
       t.sync();
       // w = y − ½·τ·(yᴴv)·v
       cd pd = cmk(0.0, 0.0);
@@ -421,7 +473,7 @@ EFN_QL_HD inline void heev_ql_body(const Team& t, const cd* A, int lda, int n, i
         pd.x += c.x;
         pd.y += c.y;
       }
-      const cd yv = block_sum(t, red, pd);
+      const cd yv = block_sum(t, W.red, pd);
       const cd al = cmul(cmk(-0.5 * tk.x, -0.5 * tk.y), yv);
       for (int i = t.tid; i < m; i += t.nthr) {
         const cd c = cmul(al, u[i]);
@@ -438,6 +490,136 @@ EFN_QL_HD inline void heev_ql_body(const Team& t, const cd* A, int lda, int n, i
     sub[n - 1] = 0.0;
   }
   t.sync();
+}
+
+// V ← Q·V for the first `low` columns of V (lda × lowest row-major; on entry
+// the real eigenvectors of T in rows < n, zeros elsewhere), in batches of up
+// to kBackCols columns staged over Z, each turned by H(n−2) … H(0).  A column
+// is owned by kBackPer adjacent lanes of one wave: lane q sums the rows
+// i ≡ q (mod kBackPer) of v_kᴴ·y_c and updates the same rows, the partials
+// meet by butterfly shuffles ((p₀+p₁)+(p₂+p₃))+((p₄+p₅)+(p₆+p₇)) — so no
+// workgroup barrier separates the reflectors, and a column's result does not
+// depend on the batch it is in (a subset of columns equals the full solve's).
+EFN_QL_HD inline void back_transform(const Team& t, const Work& W, int n, int nl, int lda, int lowest, int low,
+                                     cd* V) {
+  (void)lda;
+  static_assert(kBackPer == 8 && kBackCols * kBackPer == kThreads, "one column per 8 lanes");
+#ifdef __HIP_DEVICE_COMPILE__
+  __threadfence_block();
+#endif
+  t.sync();
+  const cd* a = W.a;
+  const cd* tau = W.tau;
+  cd* yb = reinterpret_cast<cd*>(W.z);
+  int bmax = (nl * W.ldz) / (2 * (n > 0 ? n : 1));
+  if (bmax > kBackCols) bmax = kBackCols;
+  for (int c0 = 0; c0 < low; c0 += bmax) {
+    const int nb = low - c0 < bmax ? low - c0 : bmax;
+    for (int idx = t.tid; idx < n * nb; idx += t.nthr) {
+      const int i = idx / nb, c = idx % nb;
+      yb[i * nb + c] = V[(long)i * lowest + c0 + c];
+    }
+    t.sync();
+#ifdef __HIP_DEVICE_COMPILE__
+    const int c = t.tid / kBackPer, q = t.tid % kBackPer;
+    const bool act = c < nb;
+    constexpr int kRows = (kMaxN + kBackPer - 1) / kBackPer;  // rows a lane owns at most
+    for (int k = n - 2; k >= 0; --k) {
+      const cd tk = tau[k];
+      if (tk.x == 0.0 && tk.y == 0.0) continue;  // uniform: tau is in LDS
+      const int m = n - k - 1;
+      cd vv[kRows], yy[kRows];
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int i = q + r * kBackPer;
+        if (act && i < m) {
+          vv[r] = i == 0 ? cmk(1.0, 0.0) : a[pk(k + 1 + i, k)];
+          yy[r] = yb[(k + 1 + i) * nb + c];
+        }
+      }
+      double px = 0.0, py = 0.0;
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        if (act && q + r * kBackPer < m) {
+          const cd x = cdotc(vv[r], yy[r]);
+          px += x.x;
+          py += x.y;
+        }
+      for (int off = 1; off < kBackPer; off <<= 1) {
+        px += __shfl_xor(px, off, 64);
+        py += __shfl_xor(py, off, 64);
+      }
+      const cd s = cmul(tk, cmk(px, py));  // τ·(vᴴy)
+#pragma unroll
+      for (int r = 0; r < kRows; ++r) {
+        const int i = q + r * kBackPer;
+        if (act && i < m) {
+          const cd f = cmul(vv[r], s);
+          yb[(k + 1 + i) * nb + c] = cmk(yy[r].x - f.x, yy[r].y - f.y);
+        }
+      }
+    }
+#else
+    for (int c = 0; c < nb; ++c)
+      for (int k = n - 2; k >= 0; --k) {
+        const cd tk = tau[k];
+        if (tk.x == 0.0 && tk.y == 0.0) continue;
+        const int m = n - k - 1;
+        double px[kBackPer], py[kBackPer];
+        for (int q = 0; q < kBackPer; ++q) {
+          px[q] = py[q] = 0.0;
+          for (int i = q; i < m; i += kBackPer) {
+            const cd v = i == 0 ? cmk(1.0, 0.0) : a[pk(k + 1 + i, k)];
+            const cd x = cdotc(v, yb[(k + 1 + i) * nb + c]);
+            px[q] += x.x;
+            py[q] += x.y;
+          }
+        }
+        for (int off = 1; off < kBackPer; off <<= 1)
+          for (int q = 0; q < kBackPer; q += 2 * off) {
+            px[q] += px[q + off];
+            py[q] += py[q + off];
+          }
+        const cd s = cmul(tk, cmk(px[0], py[0]));
+        for (int i = 0; i < m; ++i) {
+          const cd v = i == 0 ? cmk(1.0, 0.0) : a[pk(k + 1 + i, k)];
+          const cd f = cmul(v, s);
+          cd& x = yb[(k + 1 + i) * nb + c];
+          x.x -= f.x;
+          x.y -= f.y;
+        }
+      }
+#endif
+    t.sync();
+    for (int idx = t.tid; idx < n * nb; idx += t.nthr) {
+      const int i = idx / nb, cc = idx % nb;
+      V[(long)i * lowest + c0 + cc] = yb[i * nb + cc];
+    }
+    t.sync();
+  }
+}
+
+// The QL solver for one matrix.  A: n_p × n_p of a row-major array with
+// leading dimension lda (only the lower triangle is read); w: `lowest` values
+// (ascending; +inf past min(lowest, n_p)); V: lda × lowest row-major
+// (V[i·lowest + j] = component i of eigenvector j; zero past n_p rows and
+// min(lowest, n_p) columns); info: 0, or 1 + l if the QL iteration for
+// eigenvalue l did not converge (then every w is NaN).  lds:
+// Layout(nl).bytes() bytes, nl ≥ n_p.
+EFN_QL_HD inline void heev_ql_body(const Team& t, const cd* A, int lda, int n, int nl, int lowest, double* w,
+                                   cd* V, int* info, char* lds) {
+  const Work W = carve(nl, lds);
+  double* z = W.z;
+  double* d = W.d;
+  double* sub = W.sub;
+  double* cs = W.cs;
+  int* ctl = W.ctl;
+  const int ldz = W.ldz;
+  const int csn = W.csn;
+  const int low = lowest < n ? lowest : n;
+  const double inf = HUGE_VAL;
+
+  reduce_tridiag(t, A, lda, n, W, true);
 
   // ---- implicit QL on (d, sub), rotations into Z -----------------------------
   // buffer b: c at cs[(2b)·csn + i], s at cs[(2b + 1)·csn + i]
@@ -468,74 +650,588 @@ EFN_QL_HD inline void heev_ql_body(const Team& t, const cd* A, int lda, int n, i
   for (int r = t.tid; r < lowest; r += t.nthr) w[r] = failed ? NAN : (r < low ? d[perm[r]] : inf);
   if (t.tid == 0) *info = ctl[kCtlFail];
 
-  // ---- V = Q·Z[:, perm], in batches of columns staged in the Z region --------
-  // Z's selected columns go to V first (real); each batch is read back into
-  // LDS (complex, over Z), turned by H(n−2) … H(0) and stored.
+  // ---- V = Q·Z[:, perm]: Z's selected columns to V (real), then turned ------
   for (long idx = t.tid; idx < (long)lda * lowest; idx += t.nthr) {
     const int i = (int)(idx / lowest), r = (int)(idx % lowest);
     V[idx] = (i < n && r < low) ? cmk(z[i * ldz + perm[r]], 0.0) : cmk(0.0, 0.0);
   }
+  back_transform(t, W, n, nl, lda, lowest, low, V);
+}
+
+// ============================================================================
+// Selected eigenpairs by bisection and inverse iteration (the scheme of
+// LAPACK's zheevx, i.e. dstebz + zstein, after the same reduction):
+//   1. T splits where a coupling is negligible (e² ≤ ulp²·|d_i·d_{i+1}| +
+//      safmin) into unreduced blocks;
+//   2. every eigenvalue of every block by multisection on Sturm counts: slot
+//      s (row s of T) holds the (s − b0)-th eigenvalue of the block [b0, b1)
+//      containing row s, searched by bisect_lanes(n) adjacent threads at
+//      kSturmPts points each per step; the lowest `low` over all blocks are
+//      selected (ties by slot);
+//   3. per selected eigenvalue, inverse iteration on T_b − σI (LU with partial
+//      pivoting, random start, at most kInvIts solves, stopped kInvExtra
+//      solves after the iterate's growth shows σ is an eigenvalue to working
+//      precision), the vectors of a cluster (consecutive eigenvalues of a
+//      block closer than 10⁻³·‖T_b‖₁) made orthogonal to the cluster's
+//      earlier vectors by modified Gram–Schmidt after every solve; one thread
+//      per cluster, the clusters spread over the threads that have scratch;
+//   4. the same back-transformation.
+// Every quantity of eigenpair s depends on the matrix, s and n_p only — not on
+// `lowest`, the launch size or the thread that computes it — so a lowest-k
+// solve returns the leading columns of the full one bit for bit.
+
+constexpr double kUlp = 2.220446049250313e-16;      // 2⁻⁵²
+constexpr double kSafMin = 2.2250738585072014e-308;  // smallest normal
+constexpr int kMaxBisect = 128;                      // multisection steps (≥ 3 bits each)
+constexpr int kInvIts = 5;                           // inverse-iteration solves at most
+constexpr int kInvExtra = 2;                         // solves after the growth test first passes
+
+// threads that search one eigenvalue: a function of the problem size only
+EFN_QL_HD inline int bisect_lanes(int n) {
+  const int g = kThreads / (n > 0 ? n : 1);
+  return g > 16 ? 16 : g;
+}
+
+// Inverse-iteration scratch: per thread and row four doubles (1/u0, u1, l, x)
+// and one int (the interchange; U's second superdiagonal is then the
+// coupling below, read from T).  It lives in a region after the QL
+// arrays, sized for kInvSlots threads of the launch size where the LDS has
+// room, else over Z.
+constexpr int kInvRow = 36;
+constexpr int kInvSlots = 32;
+constexpr long kLdsMax = 163840 - 256;  // dynamic LDS beside __syncthreads_or's own 256 B
+EFN_QL_HD inline long heevx_extra_bytes(int nl) {
+  const Layout L(nl);
+  const long want = (long)kInvRow * nl * kInvSlots, inz = 8L * L.zsz();
+  if (want <= inz) return 0;
+  const long room = (kLdsMax - L.bytes()) & ~15L;
+  return want < room ? want : (room > inz ? room : 0);
+}
+EFN_QL_HD inline long heevx_lds_bytes(int nl) { return Layout(nl).bytes() + heevx_extra_bytes(nl); }
+
+// threads with scratch for problems of n rows, and where it starts
+EFN_QL_HD inline int invit_slots(int n, int nl) {
+  const long xb = heevx_extra_bytes(nl);
+  const long k = (xb > 0 ? xb : 8L * Layout(nl).zsz()) / ((long)kInvRow * (n > 0 ? n : 1));
+  return k > 64 ? 64 : (int)k;
+}
+EFN_QL_HD inline double* invit_scratch(int nl, char* lds, const Work& W) {
+  return heevx_extra_bytes(nl) > 0 ? reinterpret_cast<double*>(lds + Layout(nl).bytes()) : W.z;
+}
+
+constexpr int kChunk = 8;  // rows whose operands a sequential sweep loads at once
+
+// 1/t: the hardware estimate and two Newton steps on the device
+EFN_QL_HD inline double rcp_d(double t) {
+#ifdef __HIP_DEVICE_COMPILE__
+  double y = __builtin_amdgcn_rcp(t);
+  for (int k = 0; k < 2; ++k) {
+    const double e = fma(-t, y, 1.0);
+    y = fma(y, e, y);
+  }
+  return y;
+#else
+  return 1.0 / t;
+#endif
+}
+
+// Sturm counts of the block [b0, b1) at kSturmPts points xs[0..): the sign
+// changes of the leading minors p_i = det(T_{b0..i} − x), p_i = (d_i − x)·p_{i−1}
+// − e²_i·p_{i−2} (p_{b0−1} = 1, p_{b0−2} = 0; e²_i couples rows i−1 and i) —
+// the number of eigenvalues below x, the negative pivots of T_b − x = LDLᵀ —
+// on a copy of T scaled by an exact power of two (ds, ee, xs = x·2^−σ, so
+// |entries| ≤ 1), the pair renormalised by p's exponent every kChunk rows.  A
+// minor that is exactly zero counts with its sign bit (+0 as positive, the
+// usual rule for zeros of a Sturm sequence).  Two dependent operations a row,
+// no division, no branch; the points' chains interleave.
+constexpr int kSturmPts = 4;
+EFN_QL_HD inline void sturm_step(double dv, double ev, const double* xs, double* p1, double* p2, int* c) {
+#pragma unroll
+  for (int u = 0; u < kSturmPts; ++u) {
+    const double pn = fma(-ev, p2[u], (dv - xs[u]) * p1[u]);
+    c[u] += (pn < 0.0) != (p1[u] < 0.0);
+    p2[u] = p1[u];
+    p1[u] = pn;
+  }
+}
+EFN_QL_HD inline void sturm_counts(const double* __restrict__ ds, const double* __restrict__ ee, int b0, int b1,
+                                   const double* xs, int* cnt) {
+  double p1[kSturmPts], p2[kSturmPts];
+  int c[kSturmPts];
+#pragma unroll
+  for (int u = 0; u < kSturmPts; ++u) {
+    p1[u] = 1.0;
+    p2[u] = 0.0;
+    c[u] = 0;
+  }
+  int i = b0;
+  for (; i + kChunk <= b1; i += kChunk) {
+    double dv[kChunk], ev[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      dv[j] = ds[i + j];
+      ev[j] = ee[i + j];
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) sturm_step(dv[j], ev[j], xs, p1, p2, c);
+#pragma unroll
+    for (int u = 0; u < kSturmPts; ++u) {
+      int e;
+      (void)frexp(p1[u], &e);
+      p1[u] = ldexp(p1[u], -e);
+      p2[u] = ldexp(p2[u], -e);
+    }
+  }
+  for (; i < b1; ++i) sturm_step(ds[i], ee[i], xs, p1, p2, c);
+#pragma unroll
+  for (int u = 0; u < kSturmPts; ++u) cnt[u] = c[u];
+}
+
+// Gershgorin interval of the block, widened as LAPACK dstebz does, and the
+// absolute tolerance ulp·‖T_b‖
+EFN_QL_HD inline void bisect_init(const double* d, const double* sub, int b0, int b1, double pivmin, double& lo,
+                                  double& hi, double& atol) {
+  double gl = d[b0], gu = d[b0];
+  for (int r = b0; r < b1; ++r) {
+    double rad = 0.0;
+    if (r > b0) rad += fabs(sub[r - 1]);
+    if (r + 1 < b1) rad += fabs(sub[r]);
+    gl = fmin(gl, d[r] - rad);
+    gu = fmax(gu, d[r] + rad);
+  }
+  const double tnorm = fmax(fabs(gl), fabs(gu));
+  const double fud = 2.1 * kUlp * tnorm * (b1 - b0) + 4.2 * pivmin;
+  lo = gl - fud;
+  hi = gu + fud;
+  atol = kUlp * tnorm;
+}
+
+// multisection point k of K inside (lo, hi)
+EFN_QL_HD inline double bisect_point(double lo, double hi, int k, int K) {
+  return lo + (hi - lo) * ((double)k / (double)(K + 1));
+}
+
+EFN_QL_HD inline bool bisect_done(double lo, double hi, double atol, double pivmin) {
+  const double tol = fmax(fmax(atol, pivmin), 2.0 * kUlp * fmax(fabs(lo), fabs(hi)));
+  return hi - lo <= tol;
+}
+
+// the largest of the points k0 … k0 + kSturmPts − 1 whose count is ≤ j (0:
+// none); sc = 2^−σ, the scale of ds and ee
+EFN_QL_HD inline int bisect_pts(const double* ds, const double* ee, int b0, int b1, double lo, double hi, int k0,
+                                int K, int j, double sc) {
+  double xs[kSturmPts];
+  int cnt[kSturmPts];
+#pragma unroll
+  for (int u = 0; u < kSturmPts; ++u) xs[u] = bisect_point(lo, hi, k0 + u, K) * sc;
+  sturm_counts(ds, ee, b0, b1, xs, cnt);
+  int best = 0;
+#pragma unroll
+  for (int u = 0; u < kSturmPts; ++u)
+    if (cnt[u] <= j) best = k0 + u;
+  return best;
+}
+
+// pseudo-random start entry in [−1, 1) for row i of the j-th vector of a block
+EFN_QL_HD inline double start_entry(int j, int i) {
+  unsigned h = (unsigned)(j + 1) * 0x9E3779B1u + (unsigned)(i + 1) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  return (double)h * 0x1.0p-31 - 1.0;
+}
+
+// Inverse iteration for the selected eigenvalues of one cluster (slots lead …
+// last of one block) by one thread with scratch column c of Kc: vectors into
+// V's real parts (rows of the block, column rank[s]).  Returns the number of
+// vectors whose iteration did not pass the stopping test (they are kept).
+EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* __restrict__ sub,
+                                   const double* __restrict__ ev, const double* __restrict__ onen,
+                                   const int* __restrict__ bs, const int* __restrict__ be,
+                                   const int* __restrict__ rank, int lead, int last, double* __restrict__ sc,
+                                   int nrow, int c, int Kc, int lowest, cd* __restrict__ V) {
+  // scratch: array a ∈ {1/u0, u1, l, x} at sc[(a·nrow + i)·Kc + c], piv after
+  // (with an interchange at row k, U's second superdiagonal there is T's
+  // coupling sub[k+1], otherwise zero); the sweeps load kChunk rows'
+  // operands at a time ahead of their use
+  double* __restrict__ ui = sc;
+  double* __restrict__ u1 = sc + (long)nrow * Kc;
+  double* __restrict__ lm = sc + 2L * nrow * Kc;
+  double* __restrict__ X = sc + 3L * nrow * Kc;
+  int* __restrict__ pv = reinterpret_cast<int*>(sc + 4L * nrow * Kc);
+#define EFN_S(i) ((long)(i) * Kc + c)
+  int fails = 0;
+  double sigm = 0.0;
+  for (int s = lead; s <= last; ++s) {
+    const int b0 = bs[s], nb = be[s] - b0, r = rank[s];
+    if (nb == 1) {
+      V[(long)b0 * lowest + r] = cmk(1.0, 0.0);
+      continue;
+    }
+    // shift: the eigenvalue, kept 10·ulp·|σ| above the previous one of the cluster
+    double sig = ev[s];
+    if (s > lead) {
+      const double pert = 10.0 * fabs(kUlp * sig);
+      if (sig - sigm < pert) sig = sigm + pert;
+    }
+    sigm = sig;
+    // LU of T_b − σI with partial pivoting: row k of U is (u0, u1, u2) at
+    // columns k, k+1, k+2; (l_k, interchange) eliminates row k+1
+    double umax = 0.0;
+    {
+      double a = d[b0] - sig, bb = sub[b0];
+      for (int k0 = 0; k0 + 1 < nb; k0 += kChunk) {
+        double dd[kChunk], ss[kChunk + 1];
+#pragma unroll
+        for (int j = 0; j <= kChunk; ++j) {
+          const int k = k0 + j;
+          if (j < kChunk && k + 1 < nb) dd[j] = d[b0 + k + 1];
+          if (k + 1 < nb) ss[j] = sub[b0 + k];
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int k = k0 + j;
+          if (k + 1 < nb) {
+            const double cc = ss[j], a1 = dd[j] - sig;
+            const double b1n = k + 2 < nb ? ss[j + 1] : 0.0;
+            // interchange when the row below has the larger leading entry
+            const bool p = fabs(cc) > fabs(a);
+            const double num = p ? a : cc, den = p ? cc : a;
+            const double l = den != 0.0 ? num * rcp_d(den) : 0.0;
+            const double u0 = den, v1 = p ? a1 : bb, v2 = p ? b1n : 0.0;
+            const double na = p ? bb - l * a1 : a1 - l * bb;
+            const double nbb = p ? -l * b1n : b1n;
+            ui[EFN_S(k)] = u0;
+            u1[EFN_S(k)] = v1;
+            lm[EFN_S(k)] = l;
+            pv[EFN_S(k)] = p;
+            umax = fmax(umax, fmax(fabs(u0), fmax(fabs(v1), fabs(v2))));
+            a = na;
+            bb = nbb;
+          }
+        }
+      }
+      ui[EFN_S(nb - 1)] = a;
+      u1[EFN_S(nb - 1)] = 0.0;
+      pv[EFN_S(nb - 1)] = 0;
+      umax = fmax(umax, fabs(a));
+    }
+    const double ulast = ui[EFN_S(nb - 1)];
+    // pivots below ulp·max|U| raised to that size (LAPACK dlagts' perturbation)
+    const double tol = umax > 0.0 ? kUlp * umax : kUlp;
+#pragma unroll 4
+    for (int k = 0; k < nb; ++k) {
+      double u0 = ui[EFN_S(k)];
+      if (fabs(u0) < tol) u0 = u0 < 0.0 ? -tol : tol;
+      ui[EFN_S(k)] = rcp_d(u0);
+    }
+#pragma unroll 4
+    for (int i = 0; i < nb; ++i) X[EFN_S(i)] = start_entry(s - b0, i);
+    const double onenrm = onen[s], dtpcrt = sqrt(0.1 / nb);
+    int chk = 0;
+    bool conv = false;
+    for (int its = 0; its < kInvIts && !conv; ++its) {
+      // scale so that |x|∞ = nb·‖T_b‖₁·max(ulp, |u_last|)
+      double xm = 0.0;
+#pragma unroll 8
+      for (int i = 0; i < nb; ++i) xm = fmax(xm, fabs(X[EFN_S(i)]));
+      const double scl = nb * onenrm * fmax(kUlp, fabs(ulast)) / xm;
+      // y ← L⁻¹·P·(scl·x)
+      double cur = X[EFN_S(0)] * scl;
+      for (int k0 = 0; k0 + 1 < nb; k0 += kChunk) {
+        double nx[kChunk], ll[kChunk];
+        int pp[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int k = k0 + j;
+          if (k + 1 < nb) {
+            nx[j] = X[EFN_S(k + 1)] * scl;
+            ll[j] = lm[EFN_S(k)];
+            pp[j] = pv[EFN_S(k)];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int k = k0 + j;
+          if (k + 1 < nb) {
+            if (pp[j]) {
+              X[EFN_S(k)] = nx[j];
+              cur = cur - ll[j] * nx[j];
+            } else {
+              X[EFN_S(k)] = cur;
+              cur = nx[j] - ll[j] * cur;
+            }
+          }
+        }
+      }
+      X[EFN_S(nb - 1)] = cur;
+      // x ← U⁻¹·y, rescaled by 2⁻⁵⁰⁰ whenever an entry passes 2⁵⁰⁰
+      double x1 = 0.0, x2 = 0.0;
+      for (int k1 = nb - 1; k1 >= 0; k1 -= kChunk) {
+        double yy[kChunk], v1[kChunk], v2[kChunk], vi[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int k = k1 - j;
+          if (k >= 0) {
+            yy[j] = X[EFN_S(k)];
+            v1[j] = u1[EFN_S(k)];
+            v2[j] = pv[EFN_S(k)] && k + 2 < nb ? sub[b0 + k + 1] : 0.0;
+            vi[j] = ui[EFN_S(k)];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int k = k1 - j;
+          if (k >= 0) {
+            double xk = (yy[j] - v1[j] * x1 - v2[j] * x2) * vi[j];
+            if (fabs(xk) > 0x1.0p500) {
+              for (int i = 0; i < nb; ++i) X[EFN_S(i)] *= 0x1.0p-500;
+#pragma unroll
+              for (int jj = 0; jj < kChunk; ++jj)
+                if (jj > j) yy[jj] *= 0x1.0p-500;
+              xk *= 0x1.0p-500;
+              x1 *= 0x1.0p-500;
+              x2 *= 0x1.0p-500;
+            }
+            X[EFN_S(k)] = xk;
+            x2 = x1;
+            x1 = xk;
+          }
+        }
+      }
+      // modified Gram–Schmidt against the cluster's earlier vectors
+      for (int q = lead; q < s; ++q) {
+        const cd* vq = V + (long)b0 * lowest + rank[q];
+        double dot = 0.0;
+        for (int i = 0; i < nb; ++i) dot += X[EFN_S(i)] * vq[(long)i * lowest].x;
+        for (int i = 0; i < nb; ++i) X[EFN_S(i)] -= dot * vq[(long)i * lowest].x;
+      }
+      double nrm = 0.0;
+#pragma unroll 8
+      for (int i = 0; i < nb; ++i) nrm = fmax(nrm, fabs(X[EFN_S(i)]));
+      if (nrm < dtpcrt) continue;
+      if (++chk < kInvExtra + 1) continue;
+      conv = true;
+    }
+    if (!conv) ++fails;
+    // unit 2-norm, the largest entry (first of equals) positive
+    double ss = 0.0, big = -1.0, bigv = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < nb; ++i) {
+      const double v = X[EFN_S(i)];
+      ss += v * v;
+      if (fabs(v) > big) {
+        big = fabs(v);
+        bigv = v;
+      }
+    }
+    double scl = 1.0 / sqrt(ss);
+    if (bigv < 0.0) scl = -scl;
+#pragma unroll 8
+    for (int i = 0; i < nb; ++i) V[(long)(b0 + i) * lowest + r] = cmk(X[EFN_S(i)] * scl, 0.0);
+  }
+#undef EFN_S
+  return fails;
+}
+
+// The bisection + inverse-iteration solver for one matrix: arguments and
+// outputs as heev_ql_body's (lds: heevx_lds_bytes(nl) bytes), except info: the number of eigenvectors whose
+// inverse iteration did not pass its stopping test (their vectors are still
+// returned; the eigenvalues are always the bisection's).
+EFN_QL_HD inline void heevx_body(const Team& t, const cd* A, int lda, int n, int nl, int lowest, double* w, cd* V,
+                                 int* info, char* lds) {
+  const Work W = carve(nl, lds);
+  const double* d = W.d;
+  const double* sub = W.sub;
+  // the scaled copy of T: ee[i] = (2^−σ·sub[i−1])², zero at i = 0 and at splits; ds = 2^−σ·d
+  double* ee = reinterpret_cast<double*>(W.u);
+  double* ds = ee + nl;
+  int* bs = reinterpret_cast<int*>(W.y);
+  int* be = bs + n;
+  int* rank = bs + 2 * n;
+  double* ev = W.cs;
+  double* onen = W.cs + W.csn;
+  int* ctl = W.ctl;
+  int* perm = ctl + kCtlPerm;
+  int* gr = reinterpret_cast<int*>(W.red);
+  const int low = lowest < n ? lowest : n;
+  const double inf = HUGE_VAL;
+
+  reduce_tridiag(t, A, lda, n, W, false);
+
+  // ---- split into unreduced blocks; pivmin; ‖T_b‖₁ per row; the copy of T
+  // scaled by 2^−σ ≥ 1/max_i(|d_i| + |sub_{i−1}| + |sub_i|) for the counts --
+  if (t.tid == 0) {
+    double emax = 0.0, rmax = 0.0;
+    for (int i = 0; i < n; ++i) {
+      rmax = fmax(rmax, fabs(d[i]) + (i > 0 ? fabs(sub[i - 1]) : 0.0) + (i + 1 < n ? fabs(sub[i]) : 0.0));
+      if (i + 1 < n) emax = fmax(emax, sub[i] * sub[i]);
+    }
+    int ex = 0;
+    if (rmax > 0.0) (void)frexp(rmax, &ex);
+    const double sc = ldexp(1.0, -ex);
+    W.cs[2 * W.csn] = kSafMin * fmax(1.0, emax);  // pivmin of the unscaled matrix
+    W.cs[2 * W.csn + 1] = sc;
+    int b0 = 0;
+    if (n > 0) ee[0] = 0.0;
+    for (int i = 0; i < n; ++i) {
+      ds[i] = d[i] * sc;
+      bool split = i == n - 1;
+      if (!split) {
+        const double e = sub[i] * sub[i];
+        split = e <= kUlp * kUlp * fabs(d[i] * d[i + 1]) + kSafMin;
+        const double es = sub[i] * sc;
+        ee[i + 1] = split ? 0.0 : es * es;
+      }
+      if (split) {
+        double on = 0.0;
+        for (int r = b0; r <= i; ++r) {
+          double s = fabs(d[r]);
+          if (r > b0) s += fabs(sub[r - 1]);
+          if (r < i) s += fabs(sub[r]);
+          on = fmax(on, s);
+        }
+        for (int r = b0; r <= i; ++r) {
+          bs[r] = b0;
+          be[r] = i + 1;
+          onen[r] = on;
+        }
+        b0 = i + 1;
+      }
+    }
+  }
+  t.sync();
+  const double pivmin = W.cs[2 * W.csn], scl = W.cs[2 * W.csn + 1];
+
+  // ---- every eigenvalue of every block by multisection ----------------------
+  {
+    const int G = bisect_lanes(n), K = G * kSturmPts;
+#ifdef __HIP_DEVICE_COMPILE__
+    const int s = t.tid / G, g = t.tid % G;
+    const bool mine = s < n;
+    int b0 = 0, b1 = 0, j = 0;
+    double lo = 0.0, hi = 0.0, atol = 0.0;
+    bool done = true;
+    if (mine) {
+      b0 = bs[s];
+      b1 = be[s];
+      j = s - b0;
+      if (b1 - b0 == 1) {
+        lo = hi = d[s];
+      } else {
+        bisect_init(d, sub, b0, b1, pivmin, lo, hi, atol);
+        done = bisect_done(lo, hi, atol, pivmin);
+      }
+    }
+    for (int it = 0; it < kMaxBisect; ++it) {
+      if (!__syncthreads_or(!done)) break;
+      if (!done) gr[t.tid] = bisect_pts(ds, ee, b0, b1, lo, hi, g * kSturmPts + 1, K, j, scl);
+      t.sync();
+      if (!done) {
+        int a = 0;
+        for (int q = 0; q < G; ++q) a = max(a, gr[s * G + q]);
+        const double nlo = a == 0 ? lo : bisect_point(lo, hi, a, K);
+        const double nhi = a == K ? hi : bisect_point(lo, hi, a + 1, K);
+        lo = nlo;
+        hi = nhi;
+        done = bisect_done(lo, hi, atol, pivmin);
+      }
+    }
+    if (mine && g == 0) ev[s] = 0.5 * (lo + hi);
+#else
+    for (int s = 0; s < n; ++s) {
+      const int b0 = bs[s], b1 = be[s], j = s - b0;
+      if (b1 - b0 == 1) {
+        ev[s] = 0.5 * (d[s] + d[s]);
+        continue;
+      }
+      double lo, hi, atol;
+      bisect_init(d, sub, b0, b1, pivmin, lo, hi, atol);
+      for (int it = 0; it < kMaxBisect && !bisect_done(lo, hi, atol, pivmin); ++it) {
+        int a = 0;
+        for (int g = 0; g < G; ++g) {
+          const int v = bisect_pts(ds, ee, b0, b1, lo, hi, g * kSturmPts + 1, K, j, scl);
+          a = v > a ? v : a;
+        }
+        const double nlo = a == 0 ? lo : bisect_point(lo, hi, a, K);
+        const double nhi = a == K ? hi : bisect_point(lo, hi, a + 1, K);
+        lo = nlo;
+        hi = nhi;
+      }
+      ev[s] = 0.5 * (lo + hi);
+    }
+#endif
+  }
+  t.sync();
+  // ascending within each block (a Sturm count need not be monotone in x)
+  if (t.tid == 0)
+    for (int s = 1; s < n; ++s)
+      if (bs[s] == bs[s - 1] && ev[s] < ev[s - 1]) ev[s] = ev[s - 1];
+  t.sync();
+
+  // ---- the lowest `low`: rank of each slot (ties by slot) -------------------
+  for (int s = t.tid; s < n; s += t.nthr) {
+    int r = 0;
+    const double v = ev[s];
+    for (int i = 0; i < n; ++i) r += (ev[i] < v) || (ev[i] == v && i < s);
+    rank[s] = r;
+    if (r < low) perm[r] = s;
+  }
+  t.sync();
+  for (int r = t.tid; r < lowest; r += t.nthr) w[r] = r < low ? ev[perm[r]] : inf;
+  // clusters with a selected member: (first slot, last selected slot); within
+  // a block the selected slots are a prefix (ranks grow with the slot)
+  if (t.tid == 0) {
+    int nc = 0, lead = -1, last = -1;
+    for (int s = 0; s < n; ++s) {
+      const bool leader = s == bs[s] || ev[s] - ev[s - 1] > 1e-3 * onen[s];
+      if (leader) {
+        if (last >= 0) {
+          gr[2 * nc] = lead;
+          gr[2 * nc + 1] = last;
+          ++nc;
+        }
+        lead = s;
+        last = -1;
+      }
+      if (rank[s] < low) last = s;
+    }
+    if (last >= 0) {
+      gr[2 * nc] = lead;
+      gr[2 * nc + 1] = last;
+      ++nc;
+    }
+    ctl[kCtlClus] = nc;
+  }
+  for (long idx = t.tid; idx < (long)lda * lowest; idx += t.nthr) V[idx] = cmk(0.0, 0.0);
 #ifdef __HIP_DEVICE_COMPILE__
   __threadfence_block();
 #endif
   t.sync();
-  cd* yb = reinterpret_cast<cd*>(z);
-  // complex columns of n rows that fit over Z, at most kBackCols (the
-  // partial sums of a batch fill red)
-  int bmax = (nl * ldz) / (2 * (n > 0 ? n : 1));
-  if (bmax > kBackCols) bmax = kBackCols;
-  for (int c0 = 0; c0 < low; c0 += bmax) {
-    const int nb = low - c0 < bmax ? low - c0 : bmax;
-    for (int idx = t.tid; idx < n * nb; idx += t.nthr) {
-      const int i = idx / nb, c = idx % nb;
-      yb[i * nb + c] = V[(long)i * lowest + c0 + c];
-    }
-    t.sync();
-    for (int k = n - 2; k >= 0; --k) {
-      const cd tk = tau[k];
-      if (tk.x == 0.0 && tk.y == 0.0) continue;  // uniform: tau is in LDS
-      const int m = n - k - 1;
-      // s_c = v_kᴴ·y_c over rows k+1 …: kBackPer partial sums a column, in
-      // an order that depends on n only (a column's result is the same in
-      // any batch, so a subset of columns equals the full solve's)
-      constexpr int per = kBackPer;
-      for (int slot = t.tid; slot < nb * per; slot += t.nthr) {
-        const int c = slot / per, q = slot % per;
-        cd part = cmk(0.0, 0.0);
-        for (int i = q; i < m; i += per) {
-          const cd v = i == 0 ? cmk(1.0, 0.0) : a[pk(k + 1 + i, k)];
-          const cd x = cdotc(v, yb[(k + 1 + i) * nb + c]);
-          part.x += x.x;
-          part.y += x.y;
-        }
-        red[slot] = part;
+
+  // ---- eigenvectors of T by inverse iteration, one cluster a thread ---------
+  {
+    const int Kc = invit_slots(n, nl), nc = ctl[kCtlClus];
+    double* sc = invit_scratch(nl, lds, W);
+    const int c = t.tid;
+    if (c < Kc) {
+      int fails = 0;
+      for (int q = c; q < nc; q += (t.nthr > 1 ? Kc : 1))
+        fails += invit_cluster(d, sub, ev, onen, bs, be, rank, gr[2 * q], gr[2 * q + 1], sc, n,
+                               t.nthr > 1 ? c : 0, Kc, lowest, V);
+      if (fails) {
+#ifdef __HIP_DEVICE_COMPILE__
+        atomicAdd(&ctl[kCtlFail], fails);
+#else
+        ctl[kCtlFail] += fails;
+#endif
       }
-      t.sync();
-      for (int cc = t.tid; cc < nb; cc += t.nthr) {
-        cd s = cmk(0.0, 0.0);
-        for (int j = 0; j < per; ++j) {
-          s.x += red[cc * per + j].x;
-          s.y += red[cc * per + j].y;
-        }
-        u[cc] = cmul(tk, s);  // τ·(vᴴy)
-      }
-      t.sync();
-      for (int idx = t.tid; idx < m * nb; idx += t.nthr) {
-        const int i = idx / nb, cc = idx % nb;
-        const cd v = i == 0 ? cmk(1.0, 0.0) : a[pk(k + 1 + i, k)];
-        const cd f = cmul(v, u[cc]);
-        cd& x = yb[(k + 1 + i) * nb + cc];
-        x.x -= f.x;
-        x.y -= f.y;
-      }
-      t.sync();
     }
-    for (int idx = t.tid; idx < n * nb; idx += t.nthr) {
-      const int i = idx / nb, cc = idx % nb;
-      V[(long)i * lowest + c0 + cc] = yb[i * nb + cc];
-    }
-    t.sync();
   }
+  back_transform(t, W, n, nl, lda, lowest, low, V);
+  if (t.tid == 0) *info = ctl[kCtlFail];
 }
 
 // per-problem sizes of a ragged launch, passed by value (no device array)
@@ -559,6 +1255,20 @@ __global__ __launch_bounds__(kThreads) void heev_ql_kernel(const double2* __rest
   const Team t{(int)threadIdx.x, (int)blockDim.x};
   heev_ql_body(t, reinterpret_cast<const cd*>(A) + (long)p * lda * lda, lda, n, lda, lowest,
                w + (long)p * lowest, reinterpret_cast<cd*>(V) + (long)p * lda * lowest, info + p, ql_lds);
+}
+
+// grid: (nprob); block: kThreads; dynamic LDS: heevx_lds_bytes(lda); arguments
+// as heev_ql_kernel's (info: eigenvectors whose inverse iteration did not pass
+// its stopping test)
+__global__ __launch_bounds__(kThreads) void heevx_kernel(const double2* __restrict__ A, int lda, Dims dims,
+                                                         int lowest, double* __restrict__ w,
+                                                         double2* __restrict__ V, int* __restrict__ info) {
+  extern __shared__ __align__(16) char ql_lds[];
+  const int p = blockIdx.x;
+  const int n = dims.ragged ? dims.n[p] : lda;
+  const Team t{(int)threadIdx.x, (int)blockDim.x};
+  heevx_body(t, reinterpret_cast<const cd*>(A) + (long)p * lda * lda, lda, n, lda, lowest, w + (long)p * lowest,
+             reinterpret_cast<cd*>(V) + (long)p * lda * lowest, info + p, ql_lds);
 }
 #endif
 

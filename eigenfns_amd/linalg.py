@@ -643,14 +643,23 @@ def eigh_list_host(mats, lowest=None):
     return [(w[:k], V[:, :k]) for (w, V), k in zip(res, lows)]
 
 
-def eigh_ragged_device(A: torch.Tensor, sizes, lowest: int | None = None):
+SMALL_METHODS = ("heevx", "ql")
+SMALL_METHOD = "heevx"  # the device eigensolver for k ≤ 96 when none is named
+
+
+def eigh_ragged_device(A: torch.Tensor, sizes, lowest: int | None = None, method: str | None = None):
     """The lowest eigenpairs of each problem's leading sizes[p] × sizes[p]
-    block of a (nprob ≤ 64, K, K) complex128 device stack, in one launch of
-    the library's tridiagonal QL kernel (efn_heev_ql_batched, K ≤ 96): w
-    (nprob, lowest) ascending, +inf past min(lowest, sizes[p]); V (nprob, K,
-    lowest), zero past sizes[p] rows and min(lowest, sizes[p]) columns.  A
-    problem whose iteration failed gets NaN eigenvalues."""
+    block of a (nprob ≤ 64, K, K) complex128 device stack (K ≤ 96), in one
+    launch of one of the library's tridiagonal eigensolvers: method "heevx"
+    (efn_heevx_batched: bisection and inverse iteration, the default) or "ql"
+    (efn_heev_ql_batched: implicit QL).  w (nprob, lowest) ascending, +inf
+    past min(lowest, sizes[p]); V (nprob, K, lowest), zero past sizes[p] rows
+    and min(lowest, sizes[p]) columns.  Under "ql" a problem whose iteration
+    failed gets NaN eigenvalues."""
     import ctypes as Ct
+    method = SMALL_METHOD if method is None else method
+    if method not in SMALL_METHODS:
+        raise ValueError(f"eigh_ragged_device: method must be one of {SMALL_METHODS}, got {method!r}")
     if A.dim() != 3 or A.shape[1] != A.shape[2]:
         raise ValueError(f"eigh_ragged_device: expected a (nprob, K, K) stack, got {tuple(A.shape)}")
     A = A.to(torch.complex128).contiguous()
@@ -663,31 +672,34 @@ def eigh_ragged_device(A: torch.Tensor, sizes, lowest: int | None = None):
     V = torch.empty((nprob, K, low), dtype=torch.complex128, device=A.device)
     info = torch.empty(nprob, dtype=torch.int32, device=A.device)
     sz = None if sizes is None else (Ct.c_int * nprob)(*[int(x) for x in sizes])
-    _lib.check(_lib.load().efn_heev_ql_batched(nprob, K, sz, A.data_ptr(), low, w.data_ptr(), V.data_ptr(),
-                                               info.data_ptr(), stream_ptr()), "heev_ql_batched")
+    fn = _lib.load().efn_heevx_batched if method == "heevx" else _lib.load().efn_heev_ql_batched
+    _lib.check(fn(nprob, K, sz, A.data_ptr(), low, w.data_ptr(), V.data_ptr(), info.data_ptr(), stream_ptr()),
+               f"{method} eigensolve")
     if want > low:
         w = torch.cat([w, torch.full((nprob, want - low), float("inf"), dtype=w.dtype, device=w.device)], 1)
         V = torch.cat([V, torch.zeros((nprob, K, want - low), dtype=V.dtype, device=V.device)], 2)
     return w, V
 
 
-QL_MAX_N = 96  # largest matrix of the library's tridiagonal QL kernel (its LDS)
+QL_MAX_N = 96  # largest matrix of the library's tridiagonal eigensolvers (their LDS)
 
 
 def eigh_batched_device(A: torch.Tensor, lowest: int | None = None, max_sweeps: int = 0,
                         method: str | None = None):
     """(w, V) of a (nprob, k, k) complex128 device stack on the device, FP64;
     ascending, V[:, :, j] the vector of w[:, j]; with `lowest`, the lowest
-    eigenpairs.  No host round trip.  method "ql": the library's tridiagonal
-    QL kernel (efn_heev_ql_batched, k ≤ 96, no workspace, so a graph can
-    capture it; 64 × 24² in 0.36 ms, 72² in 2.3 ms); "dc": rocSOLVER's batched
-    divide and conquer (efn_heevd_batched, any k; 2.5 ms at 72², 4.1 ms for
-    64 × 24²); "jacobi": the library's cyclic Jacobi kernel (efn_heev_batched,
-    k ≤ 96).  None: "ql" up to 96, "dc" above."""
+    eigenpairs.  No host round trip.  method "heevx": the library's
+    bisection + inverse-iteration kernel (efn_heevx_batched, k ≤ 96, no
+    workspace, so a graph can capture it); "ql": its tridiagonal QL kernel
+    (efn_heev_ql_batched, k ≤ 96, also capturable; 64 × 24² in 0.36 ms, 72²
+    in 2.3 ms); "dc": rocSOLVER's batched divide and conquer
+    (efn_heevd_batched, any k; 2.5 ms at 72², 4.1 ms for 64 × 24²); "jacobi":
+    the library's cyclic Jacobi kernel (efn_heev_batched, k ≤ 96).  None:
+    SMALL_METHOD up to 96, "dc" above."""
     if method is None:
-        method = "ql" if int(A.shape[1]) <= QL_MAX_N else "dc"
-    if method == "ql":
-        return eigh_ragged_device(A, None, lowest)
+        method = SMALL_METHOD if int(A.shape[1]) <= QL_MAX_N else "dc"
+    if method in SMALL_METHODS:
+        return eigh_ragged_device(A, None, lowest, method)
     A = A.to(torch.complex128).contiguous()
     nprob, k = int(A.shape[0]), int(A.shape[1])
     w = torch.empty((nprob, k), dtype=torch.float64, device=A.device)

@@ -280,6 +280,19 @@ int efn_heevd_batched(int nprob, int n, const void* A_c128, double* w, void* V_c
 int efn_heev_ql_batched(int nprob, int n, const int* n_each, const void* A_c128, int lowest, double* w,
                         void* V_c128, int* info_dev, void* stream);
 
+/* The same lowest eigenpairs by bisection and inverse iteration (LAPACK
+ * zheevx's scheme: dstebz + zstein) after the same Householder reduction:
+ * the eigenvalues of the tridiagonal matrix by multisection on Sturm counts,
+ * many at once; the selected eigenvectors by inverse iteration, one cluster
+ * of close eigenvalues per thread, orthogonalised within the cluster.  Same
+ * arguments, layouts and guarantees as efn_heev_ql_batched (a lowest-k solve
+ * equals the leading columns of the full one bit for bit), except info: the
+ * number of eigenvectors of the problem whose inverse iteration did not pass
+ * its stopping test (their vectors are still returned).  Allocates nothing.
+ * The host and rocSOLVER eigensolves of solver.py:119-121, :263-265, :301. */
+int efn_heevx_batched(int nprob, int n, const int* n_each, const void* A_c128, int lowest, double* w,
+                      void* V_c128, int* info_dev, void* stream);
+
 /* Process-wide combine engine: 1 = matrix cores (v_mfma_f32_16x16x4f32,
  * default, for m_out > 8), 0 = vector ALU; FP32 accumulation either way.
  * Returns the previous engine (engine < 0: query only).  No reference

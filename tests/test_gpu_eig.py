@@ -1,7 +1,8 @@
 """GPU tests of the batched Hermitian eigensolvers — efn_heev_batched (the
-library's FP64 Jacobi kernel), efn_heev_ql_batched (the library's tridiagonal
-QL kernel) and efn_heevd_batched (rocSOLVER's batched divide and conquer) —
-against LAPACK (numpy.linalg.eigh) on the same FP64 matrices.
+library's FP64 Jacobi kernel), efn_heev_ql_batched (its tridiagonal QL
+kernel), efn_heevx_batched (its bisection + inverse-iteration kernel) and
+efn_heevd_batched (rocSOLVER's batched divide and conquer) — against LAPACK
+(numpy.linalg.eigh) on the same FP64 matrices.
 
 Bars: eigenvalues within 1e-13·max|λ| of LAPACK's; residual ‖AV − VΛ‖ and
 ‖VᴴV − I‖ below 1e-12 (relative to max|λ|); the smallest eigenvalues of a
@@ -41,7 +42,7 @@ def _check(A, w, V):
     return wr
 
 
-METHODS = ["dc", "jacobi", "ql"]
+METHODS = ["dc", "jacobi", "ql", "heevx"]
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -91,7 +92,7 @@ def test_heev_batch_invariant_and_limits(method):
     for j in range(6):
         w1, V1 = eigh_batched_device(mats[j:j + 1], method=method)
         assert torch.equal(w1[0], wb[j]) and torch.equal(V1[0], Vb[j])
-    if method in ("jacobi", "ql"):
+    if method in ("jacobi", "ql", "heevx"):
         with pytest.raises(ValueError):
             eigh_batched_device(torch.zeros((1, 97, 97), dtype=torch.complex128, device="cuda"),
                                 method=method)
@@ -134,7 +135,8 @@ def test_heev_decoupled_dead_block(method):
         assert np.abs(w[j][:24] - wr[:24]).max() < 1e-10 * np.abs(wr[:24]).min()
 
 
-def test_heev_ql_ragged_sizes_in_one_launch():
+@pytest.mark.parametrize("method", ["ql", "heevx"])
+def test_heev_small_ragged_sizes_in_one_launch(method):
     """Problems of sizes 1-96 padded into one (nprob, 96, 96) stack, the lowest
     24 of each in one launch: each problem to the LAPACK bars, +inf / zero
     padding past its own size, and bitwise the same as the problem alone."""
@@ -148,7 +150,7 @@ def test_heev_ql_ragged_sizes_in_one_launch():
         if n:
             A[j, :n, :n] = _rr_like(rng, n, dead=(n - 1,) if n > 3 else ())
     At = torch.from_numpy(A).cuda()
-    w, V = eigh_ragged_device(At, sizes, low)
+    w, V = eigh_ragged_device(At, sizes, low, method)
     wn, Vn = w.cpu().numpy(), V.cpu().numpy()
     for j, n in enumerate(sizes):
         k = min(low, n)
@@ -160,11 +162,12 @@ def test_heev_ql_ragged_sizes_in_one_launch():
             assert np.abs(A[j, :n, :n] @ Vj - Vj * wn[j, :k]).max() < 1e-12 * scale
             assert np.abs(Vj.conj().T @ Vj - np.eye(k)).max() < 1e-12
         assert np.isinf(wn[j, k:]).all() and not Vn[j, n:].any() and not Vn[j, :, k:].any()
-        w1, V1 = eigh_ragged_device(At[j:j + 1], [n], low)
+        w1, V1 = eigh_ragged_device(At[j:j + 1], [n], low, method)
         assert torch.equal(w1[0], w[j]) and torch.equal(V1[0], V[j])
 
 
-def test_heev_ql_more_columns_than_the_stack():
+@pytest.mark.parametrize("method", ["ql", "heevx"])
+def test_heev_small_more_columns_than_the_stack(method):
     """Asking for more eigenpairs than the stack's size (an engine whose
     requests are all narrower than a third of its width) pads w with +inf and
     V with zero columns; the leading columns are the full solve's."""
@@ -172,8 +175,8 @@ def test_heev_ql_more_columns_than_the_stack():
     from eigenfns_amd.linalg import eigh_ragged_device
     rng = np.random.default_rng(31)
     A = torch.from_numpy(np.stack([_herm(rng, 24), _herm(rng, 24)])).cuda()
-    w, V = eigh_ragged_device(A, [24, 10], 32)
-    wf, Vf = eigh_ragged_device(A, [24, 10], 24)
+    w, V = eigh_ragged_device(A, [24, 10], 32, method)
+    wf, Vf = eigh_ragged_device(A, [24, 10], 24, method)
     assert w.shape == (2, 32) and V.shape == (2, 24, 32)
     assert torch.equal(w[:, :24], wf) and torch.equal(V[:, :, :24], Vf)
     assert torch.isinf(w[:, 24:]).all() and not V[:, :, 24:].any()
