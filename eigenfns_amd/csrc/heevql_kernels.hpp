@@ -21,6 +21,7 @@
 #pragma once
 
 #include <math.h>
+#include <string.h>
 
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
@@ -91,36 +92,76 @@ struct Layout {
 };
 
 // control words
-enum { kCtlL = 0, kCtlIter = 1, kCtlFail = 2, kCtlClus = 3, kCtlCnt = 4, kCtlLo = 6, kCtlDone = 8, kCtlPerm = 16 };
+// (QL: 0-9; bisection + inverse iteration: 2, 3, and 4-11 as two pairs of
+// 64-bit words, the maxima and the split mask; both: the permutation at 16)
+enum { kCtlL = 0, kCtlIter = 1, kCtlFail = 2, kCtlClus = 3, kCtlCnt = 4, kCtlLo = 6, kCtlDone = 8, kCtlMax = 4,
+       kCtlMask = 8, kCtlPerm = 16 };
 
 // Block reduction of a complex partial (every thread passes one); the sum
-// is returned to every thread.  Two barriers.  On the device each wave sums
-// by lane shuffles and the wave sums meet in red; on the host (one thread)
-// the value is returned.
-EFN_QL_HD inline cd block_sum(const Team& t, cd* red, cd v) {
+// is returned to every thread.  One barrier: on the device each wave sums by
+// lane shuffles and the wave sums meet in red[4·slot …]; a slot is written
+// again only after a later barrier (consecutive reductions alternate slots).
+// On the host (one thread) the value is returned.
+EFN_QL_HD inline cd block_sum(const Team& t, cd* red, cd v, int slot) {
 #ifdef __HIP_DEVICE_COMPILE__
   for (int off = 32; off > 0; off >>= 1) {
     v.x += __shfl_xor(v.x, off, 64);
     v.y += __shfl_xor(v.y, off, 64);
   }
-  if ((t.tid & 63) == 0) red[t.tid >> 6] = v;
+  cd* r = red + 4 * slot;
+  if ((t.tid & 63) == 0) r[t.tid >> 6] = v;
   t.sync();
-  cd s = red[0];
+  cd s = r[0];
   for (int i = 1; i < (t.nthr >> 6); ++i) {
-    s.x += red[i].x;
-    s.y += red[i].y;
+    s.x += r[i].x;
+    s.y += r[i].y;
   }
-  t.sync();
   return s;
 #else
-  cd s = cmk(0.0, 0.0);
-  red[0] = v;
-  for (int i = 0; i < t.nthr; ++i) {
-    s.x += red[i].x;
-    s.y += red[i].y;
-  }
-  return s;
+  (void)t;
+  (void)red;
+  (void)slot;
+  return v;
 #endif
+}
+
+// bit patterns of doubles, on both sides
+EFN_QL_HD inline long long __double_as_longlong_hd(double v) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __double_as_longlong(v);
+#else
+  long long r;
+  memcpy(&r, &v, sizeof r);
+  return r;
+#endif
+}
+EFN_QL_HD inline double __longlong_as_double_hd(long long v) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __longlong_as_double(v);
+#else
+  double r;
+  memcpy(&r, &v, sizeof r);
+  return r;
+#endif
+}
+
+// In a 128-bit mask (two words): the highest set bit below i (−1: none) and
+// the lowest set bit at or above i (the mask has one there)
+EFN_QL_HD inline int last_bit_below(const unsigned long long* m, int i) {
+  for (int w = (i - 1) >> 6; w >= 0; --w) {
+    const int hi = i - 64 * w;  // bits [0, hi) of word w lie below i
+    const unsigned long long x = hi >= 64 ? m[w] : (m[w] & ((1ull << hi) - 1ull));
+    if (x) return 64 * w + 63 - __builtin_clzll(x);
+  }
+  return -1;
+}
+EFN_QL_HD inline int first_bit_from(const unsigned long long* m, int i) {
+  for (int w = i >> 6; w < 2; ++w) {
+    const int lo = i - 64 * w;  // bits [lo, 64) of word w lie at or above i
+    const unsigned long long x = lo <= 0 ? m[w] : (m[w] & ~((1ull << lo) - 1ull));
+    if (x) return 64 * w + __builtin_ctzll(x);
+  }
+  return 127;
 }
 
 // 1/√t for t > 0: the hardware estimate and two Newton steps on the device,
@@ -290,37 +331,49 @@ EFN_QL_HD inline void ql_apply_row(double* __restrict__ zr, const double* __rest
 }
 
 // Σ_j A₂₂(i, j)·v_j over the columns j ≡ q (mod G) of row i of the trailing
-// block at k+1 (size m), A₂₂ Hermitian from the packed lower triangle
+// block at k+1 (size m), A₂₂ Hermitian from the packed lower triangle (A(i, j)
+// = conj(A(j, i)) above the diagonal, the diagonal real); four entries loaded
+// ahead of their products, alternate entries into two sums added at the end
 EFN_QL_HD inline void hemv_part(const cd* __restrict__ a, const cd* __restrict__ u, int k, int m, int i, int q,
                                 int G, double& sx, double& sy) {
   const int gi = k + 1 + i;
-  const cd* __restrict__ row = a + pk(gi, k + 1);
-  int j = q;
-  for (; j < i; j += G) {
-    const cd aij = row[j], v = u[j];
-    sx += aij.x * v.x - aij.y * v.y;
-    sy += aij.x * v.y + aij.y * v.x;
+  const int row = pk(gi, k + 1);
+  double ax[2] = {0.0, 0.0}, ay[2] = {0.0, 0.0};
+  for (int j0 = q; j0 < m; j0 += 4 * G) {
+    cd av[4], uv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = j0 + r * G;
+      if (j < m) {
+        av[r] = a[j <= i ? row + j : pk(k + 1 + j, gi)];
+        uv[r] = u[j];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int j = j0 + r * G;
+      if (j < m) {
+        const double re = av[r].x, im = j < i ? av[r].y : (j == i ? 0.0 : -av[r].y);
+        ax[r & 1] += re * uv[r].x - im * uv[r].y;
+        ay[r & 1] += re * uv[r].y + im * uv[r].x;
+      }
+    }
   }
-  if (j == i) {
-    const double aii = a[pk(gi, gi)].x;
-    sx += aii * u[i].x;
-    sy += aii * u[i].y;
-    j += G;
-  }
-  for (; j < m; j += G) {
-    const cd aji = a[pk(k + 1 + j, gi)], v = u[j];  // A(i, j) = conj(A(j, i))
-    sx += aji.x * v.x + aji.y * v.y;
-    sy += aji.x * v.y - aji.y * v.x;
-  }
+  sx += ax[0] + ax[1];
+  sy += ay[0] + ay[1];
 }
 
-// y = τ·A₂₂·v over the trailing block at k+1 (size m).  G = 4 threads a row
-// (2 when 4m exceeds the workgroup), adjacent lanes of one wave, each summing
-// the columns j ≡ q (mod G); the partials meet by lane shuffles, (S₀+S₁) +
-// (S₂+S₃).  The host sums the same partials in the same order.
-EFN_QL_HD inline void ql_hemv(const Team& t, const cd* __restrict__ a, const cd* __restrict__ u, cd* __restrict__ y,
-                              int k, int m, cd tk) {
-  const int G = 4 * m <= kThreads ? 4 : 2;  // 2m ≤ kThreads for m < kMaxN
+// y = τ·A₂₂·v over the trailing block at k+1 (size m).  G threads a row —
+// the largest power of two up to 16 with G·m ≤ kThreads (2m ≤ kThreads for
+// m < kMaxN) — adjacent lanes of one wave, each summing the columns j ≡ q
+// (mod G); the partials meet by butterfly lane shuffles ((S₀+S₁)+(S₂+S₃))+….
+// The host sums the same partials in the same order.
+EFN_QL_HD inline int hemv_lanes(int m) { return 16 * m <= kThreads ? 16 : 8 * m <= kThreads ? 8 : 4 * m <= kThreads ? 4 : 2; }
+// Returns this thread's part of yᴴv (Σ conj(y_i)·v_i over its rows; on the
+// device the q = 0 lane of each row).
+EFN_QL_HD inline cd ql_hemv(const Team& t, const cd* __restrict__ a, const cd* __restrict__ u, cd* __restrict__ y,
+                            int k, int m, cd tk) {
+  const int G = hemv_lanes(m);
 #ifdef __HIP_DEVICE_COMPILE__
   const int i = t.tid / G, q = t.tid % G;
   double sx = 0.0, sy = 0.0;
@@ -329,10 +382,17 @@ EFN_QL_HD inline void ql_hemv(const Team& t, const cd* __restrict__ a, const cd*
     sx += __shfl_xor(sx, off, 64);
     sy += __shfl_xor(sy, off, 64);
   }
-  if (i < m && q == 0) y[i] = cmul(tk, cmk(sx, sy));
+  cd part = cmk(0.0, 0.0);
+  if (i < m && q == 0) {
+    const cd yi = cmul(tk, cmk(sx, sy));
+    y[i] = yi;
+    part = cdotc(yi, u[i]);
+  }
+  return part;
 #else
   for (int i = t.tid; i < m; i += t.nthr) {
-    double px[4] = {0.0, 0.0, 0.0, 0.0}, py[4] = {0.0, 0.0, 0.0, 0.0};
+    double px[16], py[16];
+    for (int q = 0; q < G; ++q) px[q] = py[q] = 0.0;
     for (int q = 0; q < G; ++q) hemv_part(a, u, k, m, i, q, G, px[q], py[q]);
     for (int off = 1; off < G; off <<= 1)
       for (int q = 0; q < G; q += 2 * off) {
@@ -341,34 +401,49 @@ EFN_QL_HD inline void ql_hemv(const Team& t, const cd* __restrict__ a, const cd*
       }
     y[i] = cmul(tk, cmk(px[0], py[0]));
   }
+  cd part = cmk(0.0, 0.0);
+  for (int i = t.tid; i < m; i += t.nthr) {
+    const cd c = cdotc(y[i], u[i]);
+    part.x += c.x;
+    part.y += c.y;
+  }
+  return part;
 #endif
 }
 
-// (row, column) of entry q of a packed lower triangle
-EFN_QL_HD inline void tri_ij(int q, int& i, int& j) {
-  int r = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
-  while (r * (r + 1) / 2 > q) --r;
-  while ((r + 1) * (r + 2) / 2 <= q) ++r;
-  i = r;
-  j = q - r * (r + 1) / 2;
-}
-
-// A₂₂ ← A₂₂ − v·wᴴ − w·vᴴ over the trailing lower triangle (size m), its
-// entries dealt evenly over the team; the diagonal kept real
-EFN_QL_HD inline void ql_her2(const Team& t, cd* __restrict__ a, const cd* __restrict__ u, const cd* __restrict__ w,
-                              int k, int m) {
-  const int tot = m * (m + 1) / 2;
+// A₂₂ ← A₂₂ − v·wᴴ − w·vᴴ over the trailing lower triangle (size m), the
+// diagonal kept real.  Rows p and m−1−p are dealt as one run of m+1 entries
+// (row p's p+1, then row m−1−p's m−p), the runs evenly over the team with
+// incremental indices (for odd m the middle row's run stops at its length).
+// Returns this thread's part of the next step's column norm: Σ|A₂₂(i, 0)|²
+// over its updated entries with i ≥ 2 (global rows ≥ k+3 of column k+1).
+EFN_QL_HD inline cd ql_her2(const Team& t, cd* __restrict__ a, const cd* __restrict__ u, const cd* __restrict__ w,
+                            int k, int m) {
+  cd part = cmk(0.0, 0.0);
+  const int L = m + 1, tot = ((m + 1) / 2) * L;
+  int p = t.tid / L, r = t.tid % L;
+  const int dp = t.nthr / L, dr = t.nthr % L;
   for (int q = t.tid; q < tot; q += t.nthr) {
-    int i, j;
-    tri_ij(q, i, j);
-    const cd vi = u[i], wi = w[i], vj = u[j], wj = w[j];
-    cd x = a[pk(k + 1 + i, k + 1 + j)];
-    // v_i·conj(w_j) + w_i·conj(v_j)
-    x.x -= (vi.x * wj.x + vi.y * wj.y) + (wi.x * vj.x + wi.y * vj.y);
-    x.y -= (vi.y * wj.x - vi.x * wj.y) + (wi.y * vj.x - wi.x * vj.y);
-    if (i == j) x.y = 0.0;
-    a[pk(k + 1 + i, k + 1 + j)] = x;
+    const bool first = r <= p;
+    const int i = first ? p : m - 1 - p, j = first ? r : r - p - 1;
+    if (first || i != p) {
+      const cd vi = u[i], wi = w[i], vj = u[j], wj = w[j];
+      cd x = a[pk(k + 1 + i, k + 1 + j)];
+      // v_i·conj(w_j) + w_i·conj(v_j)
+      x.x -= (vi.x * wj.x + vi.y * wj.y) + (wi.x * vj.x + wi.y * vj.y);
+      x.y -= (vi.y * wj.x - vi.x * wj.y) + (wi.y * vj.x - wi.x * vj.y);
+      if (i == j) x.y = 0.0;
+      a[pk(k + 1 + i, k + 1 + j)] = x;
+      if (j == 0 && i >= 2) part.x += x.x * x.x + x.y * x.y;
+    }
+    p += dp;
+    r += dr;
+    if (r >= L) {
+      r -= L;
+      ++p;
+    }
   }
+  return part;
 }
 
 // The LDS arrays of one matrix (offsets from Layout)
@@ -403,10 +478,12 @@ EFN_QL_HD inline Work carve(int nl, char* lds) {
   return W;
 }
 
-// Load A's lower triangle (and Z = I when z_identity) and reduce it to the
-// real tridiagonal (d, sub) by Householder reflectors (LAPACK zhetd2's
-// steps); τ_k in tau, v_k below the subdiagonal of column k of a.
-EFN_QL_HD inline void reduce_tridiag(const Team& t, const cd* A, int lda, int n, const Work& W, bool z_identity) {
+// Load A's lower triangle (rows and columns in the order rowmap gives, if
+// any; Z = I when z_identity) and reduce it to the real tridiagonal (d, sub)
+// by Householder reflectors (LAPACK zhetd2's steps); τ_k in tau, v_k below
+// the subdiagonal of column k of a.
+EFN_QL_HD inline void reduce_tridiag(const Team& t, const cd* A, int lda, int n, const Work& W, bool z_identity,
+                                     const int* rowmap = nullptr) {
   cd* a = W.a;
   double* z = W.z;
   double* d = W.d;
@@ -416,7 +493,15 @@ EFN_QL_HD inline void reduce_tridiag(const Team& t, const cd* A, int lda, int n,
   cd* y = W.y;
   const int ldz = W.ldz;
   for (int i = t.tid; i < n; i += t.nthr) {
-    for (int j = 0; j <= i; ++j) a[pk(i, j)] = A[(long)i * lda + j];
+    if (rowmap) {  // row i of the permuted matrix is row rowmap[i] of A
+      const int ri = rowmap[i];
+      for (int j = 0; j <= i; ++j) {
+        const int rj = rowmap[j];
+        a[pk(i, j)] = ri >= rj ? A[(long)ri * lda + rj] : cconj(A[(long)rj * lda + ri]);
+      }
+    } else {
+      for (int j = 0; j <= i; ++j) a[pk(i, j)] = A[(long)i * lda + j];
+    }
     if (z_identity)
       for (int j = 0; j < n; ++j) z[i * ldz + j] = i == j ? 1.0 : 0.0;
   }
@@ -424,15 +509,23 @@ EFN_QL_HD inline void reduce_tridiag(const Team& t, const cd* A, int lda, int n,
     for (int c = 0; c < 16; ++c) W.ctl[c] = 0;
   }
   t.sync();
+  // four barriers a step: the column norm's reduction (whose partials the
+  // previous step's rank-2 update leaves, or a loop after a skipped update),
+  // v written, yᴴv's reduction (fused with A₂₂·v), w written
+  cd npart = cmk(0.0, 0.0);
+  bool have = false;
   for (int k = 0; k + 1 < n; ++k) {
     const int m = n - k - 1;  // rows k+1 … n−1 of column k
     // ‖column below row k+1‖²
-    cd part = cmk(0.0, 0.0);
-    for (int i = k + 2 + t.tid; i < n; i += t.nthr) {
-      const cd x = a[pk(i, k)];
-      part.x += x.x * x.x + x.y * x.y;
+    if (!have) {
+      npart = cmk(0.0, 0.0);
+      for (int i = k + 2 + t.tid; i < n; i += t.nthr) {
+        const cd x = a[pk(i, k)];
+        npart.x += x.x * x.x + x.y * x.y;
+      }
     }
-    const double xn2 = block_sum(t, W.red, part).x;
+    const double xn2 = block_sum(t, W.red, npart, 0).x;
+    have = false;
     const cd alpha = a[pk(k + 1, k)];
     cd tk = cmk(0.0, 0.0), scale = cmk(0.0, 0.0);
     double beta = alpha.x;
@@ -463,28 +556,22 @@ EFN_QL_HD inline void reduce_tridiag(const Team& t, const cd* A, int lda, int n,
     }
     t.sync();
     if (tk.x != 0.0 || tk.y != 0.0) {
-      // y = τ·A₂₂·v over the trailing block (rows and columns k+1 …)
-      ql_hemv(t, a, u, y, k, m, tk);
-      t.sync();
+      // y = τ·A₂₂·v over the trailing block (rows and columns k+1 …), and yᴴv
+      const cd yv = block_sum(t, W.red, ql_hemv(t, a, u, y, k, m, tk), 1);
       // w = y − ½·τ·(yᴴv)·v
-      cd pd = cmk(0.0, 0.0);
-      for (int i = t.tid; i < m; i += t.nthr) {
-        const cd c = cdotc(y[i], u[i]);
-        pd.x += c.x;
-        pd.y += c.y;
-      }
-      const cd yv = block_sum(t, W.red, pd);
       const cd al = cmul(cmk(-0.5 * tk.x, -0.5 * tk.y), yv);
       for (int i = t.tid; i < m; i += t.nthr) {
         const cd c = cmul(al, u[i]);
         y[i] = cmk(y[i].x + c.x, y[i].y + c.y);
       }
       t.sync();
-      // A₂₂ ← A₂₂ − v·wᴴ − w·vᴴ (lower triangle, real diagonal)
-      ql_her2(t, a, u, y, k, m);
-      t.sync();
+      // A₂₂ ← A₂₂ − v·wᴴ − w·vᴴ (lower triangle, real diagonal); no barrier
+      // after it: the next step's first reduction orders its writes
+      npart = ql_her2(t, a, u, y, k, m);
+      have = true;
     }
   }
+  t.sync();
   if (t.tid == 0 && n > 0) {
     d[n - 1] = a[pk(n - 1, n - 1)].x;
     sub[n - 1] = 0.0;
@@ -492,27 +579,48 @@ EFN_QL_HD inline void reduce_tridiag(const Team& t, const cd* A, int lda, int n,
   t.sync();
 }
 
+// Reflector k's vector at global row g: zero above k+1, one at k+1, below
+// it the entries kept in column k of a
+EFN_QL_HD inline cd refl(const cd* a, int k, int g) {
+  return g <= k ? cmk(0.0, 0.0) : (g == k + 1 ? cmk(1.0, 0.0) : a[pk(g, k)]);
+}
+
 // V ← Q·V for the first `low` columns of V (lda × lowest row-major; on entry
-// the real eigenvectors of T in rows < n, zeros elsewhere), in batches of up
-// to kBackCols columns staged over Z, each turned by H(n−2) … H(0).  A column
-// is owned by kBackPer adjacent lanes of one wave: lane q sums the rows
-// i ≡ q (mod kBackPer) of v_kᴴ·y_c and updates the same rows, the partials
-// meet by butterfly shuffles ((p₀+p₁)+(p₂+p₃))+((p₄+p₅)+(p₆+p₇)) — so no
-// workgroup barrier separates the reflectors, and a column's result does not
-// depend on the batch it is in (a subset of columns equals the full solve's).
+// the real eigenvectors of T in rows < n, zeros elsewhere; row i stored to
+// row rowmap[i] when a map is given), in batches of up to kBackCols columns
+// staged over Z.  The reflectors apply in pairs (k, k−1),
+// k = n−2, n−4, …: with a = v_k, b = v_{k−1} and c = bᴴa (one per pair,
+// computed first), α = τ_k·aᴴy, β = τ_{k−1}·(bᴴy − α·c) and y ← y − a·α − b·β
+// is H(k−1)·H(k)·y from one pass over y.  A column is owned by kBackPer
+// adjacent lanes of one wave: lane q sums the rows ≡ q (mod kBackPer) of the
+// pair's range and updates the same rows, the partials meeting by butterfly
+// shuffles ((p₀+p₁)+(p₂+p₃))+((p₄+p₅)+(p₆+p₇)) — so no workgroup barrier
+// separates the pairs, and a column's result does not depend on the batch
+// it is in (a subset of columns equals the full solve's).
 EFN_QL_HD inline void back_transform(const Team& t, const Work& W, int n, int nl, int lda, int lowest, int low,
-                                     cd* V) {
+                                     cd* V, const int* rowmap = nullptr) {
   (void)lda;
   static_assert(kBackPer == 8 && kBackCols * kBackPer == kThreads, "one column per 8 lanes");
+  const cd* a = W.a;
+  const cd* tau = W.tau;
+  cd* cpair = W.u;  // c of the pair (k, k−1) at cpair[k]
+  for (int k = n - 2 - 2 * t.tid; k >= 1; k -= 2 * t.nthr) {
+    cd c = cmk(0.0, 0.0);
+    for (int g = k + 1; g < n; ++g) {
+      const cd x = cdotc(refl(a, k - 1, g), refl(a, k, g));
+      c.x += x.x;
+      c.y += x.y;
+    }
+    cpair[k] = c;
+  }
 #ifdef __HIP_DEVICE_COMPILE__
   __threadfence_block();
 #endif
   t.sync();
-  const cd* a = W.a;
-  const cd* tau = W.tau;
   cd* yb = reinterpret_cast<cd*>(W.z);
   int bmax = (nl * W.ldz) / (2 * (n > 0 ? n : 1));
   if (bmax > kBackCols) bmax = kBackCols;
+  constexpr int kRows = (kMaxN + kBackPer - 1) / kBackPer;  // rows a lane owns at most
   for (int c0 = 0; c0 < low; c0 += bmax) {
     const int nb = low - c0 < bmax ? low - c0 : bmax;
     for (int idx = t.tid; idx < n * nb; idx += t.nthr) {
@@ -523,77 +631,91 @@ EFN_QL_HD inline void back_transform(const Team& t, const Work& W, int n, int nl
 #ifdef __HIP_DEVICE_COMPILE__
     const int c = t.tid / kBackPer, q = t.tid % kBackPer;
     const bool act = c < nb;
-    constexpr int kRows = (kMaxN + kBackPer - 1) / kBackPer;  // rows a lane owns at most
-    for (int k = n - 2; k >= 0; --k) {
-      const cd tk = tau[k];
-      if (tk.x == 0.0 && tk.y == 0.0) continue;  // uniform: tau is in LDS
-      const int m = n - k - 1;
-      cd vv[kRows], yy[kRows];
+    for (int k = n - 2; k >= 0; k -= 2) {
+      const cd ta = tau[k], tb = k >= 1 ? tau[k - 1] : cmk(0.0, 0.0);
+      if (ta.x == 0.0 && ta.y == 0.0 && tb.x == 0.0 && tb.y == 0.0) continue;  // uniform: tau is in LDS
+      const int r0 = k >= 1 ? k : k + 1;  // first row either vector reaches
+      cd va[kRows], vb[kRows], yy[kRows];
 #pragma unroll
       for (int r = 0; r < kRows; ++r) {
-        const int i = q + r * kBackPer;
-        if (act && i < m) {
-          vv[r] = i == 0 ? cmk(1.0, 0.0) : a[pk(k + 1 + i, k)];
-          yy[r] = yb[(k + 1 + i) * nb + c];
+        const int g = r0 + q + r * kBackPer;
+        if (act && g < n) {
+          va[r] = refl(a, k, g);
+          vb[r] = k >= 1 ? refl(a, k - 1, g) : cmk(0.0, 0.0);
+          yy[r] = yb[g * nb + c];
         }
       }
-      double px = 0.0, py = 0.0;
+      double sax = 0.0, say = 0.0, sbx = 0.0, sby = 0.0;
 #pragma unroll
       for (int r = 0; r < kRows; ++r)
-        if (act && q + r * kBackPer < m) {
-          const cd x = cdotc(vv[r], yy[r]);
-          px += x.x;
-          py += x.y;
+        if (act && r0 + q + r * kBackPer < n) {
+          const cd xa = cdotc(va[r], yy[r]), xb = cdotc(vb[r], yy[r]);
+          sax += xa.x;
+          say += xa.y;
+          sbx += xb.x;
+          sby += xb.y;
         }
       for (int off = 1; off < kBackPer; off <<= 1) {
-        px += __shfl_xor(px, off, 64);
-        py += __shfl_xor(py, off, 64);
+        sax += __shfl_xor(sax, off, 64);
+        say += __shfl_xor(say, off, 64);
+        sbx += __shfl_xor(sbx, off, 64);
+        sby += __shfl_xor(sby, off, 64);
       }
-      const cd s = cmul(tk, cmk(px, py));  // τ·(vᴴy)
+      const cd al = cmul(ta, cmk(sax, say));
+      const cd cc = k >= 1 ? cpair[k] : cmk(0.0, 0.0);
+      const cd ac = cmul(al, cc);
+      const cd be = cmul(tb, cmk(sbx - ac.x, sby - ac.y));
 #pragma unroll
       for (int r = 0; r < kRows; ++r) {
-        const int i = q + r * kBackPer;
-        if (act && i < m) {
-          const cd f = cmul(vv[r], s);
-          yb[(k + 1 + i) * nb + c] = cmk(yy[r].x - f.x, yy[r].y - f.y);
+        const int g = r0 + q + r * kBackPer;
+        if (act && g < n) {
+          const cd fa = cmul(va[r], al), fb = cmul(vb[r], be);
+          yb[g * nb + c] = cmk(yy[r].x - fa.x - fb.x, yy[r].y - fa.y - fb.y);
         }
       }
     }
 #else
     for (int c = 0; c < nb; ++c)
-      for (int k = n - 2; k >= 0; --k) {
-        const cd tk = tau[k];
-        if (tk.x == 0.0 && tk.y == 0.0) continue;
-        const int m = n - k - 1;
-        double px[kBackPer], py[kBackPer];
+      for (int k = n - 2; k >= 0; k -= 2) {
+        const cd ta = tau[k], tb = k >= 1 ? tau[k - 1] : cmk(0.0, 0.0);
+        if (ta.x == 0.0 && ta.y == 0.0 && tb.x == 0.0 && tb.y == 0.0) continue;
+        const int r0 = k >= 1 ? k : k + 1;
+        double sx[2][kBackPer], sy[2][kBackPer];
         for (int q = 0; q < kBackPer; ++q) {
-          px[q] = py[q] = 0.0;
-          for (int i = q; i < m; i += kBackPer) {
-            const cd v = i == 0 ? cmk(1.0, 0.0) : a[pk(k + 1 + i, k)];
-            const cd x = cdotc(v, yb[(k + 1 + i) * nb + c]);
-            px[q] += x.x;
-            py[q] += x.y;
+          sx[0][q] = sy[0][q] = sx[1][q] = sy[1][q] = 0.0;
+          for (int g = r0 + q; g < n; g += kBackPer) {
+            const cd y = yb[g * nb + c];
+            const cd xa = cdotc(refl(a, k, g), y);
+            const cd xb = cdotc(k >= 1 ? refl(a, k - 1, g) : cmk(0.0, 0.0), y);
+            sx[0][q] += xa.x;
+            sy[0][q] += xa.y;
+            sx[1][q] += xb.x;
+            sy[1][q] += xb.y;
           }
         }
-        for (int off = 1; off < kBackPer; off <<= 1)
-          for (int q = 0; q < kBackPer; q += 2 * off) {
-            px[q] += px[q + off];
-            py[q] += py[q + off];
-          }
-        const cd s = cmul(tk, cmk(px[0], py[0]));
-        for (int i = 0; i < m; ++i) {
-          const cd v = i == 0 ? cmk(1.0, 0.0) : a[pk(k + 1 + i, k)];
-          const cd f = cmul(v, s);
-          cd& x = yb[(k + 1 + i) * nb + c];
-          x.x -= f.x;
-          x.y -= f.y;
+        for (int v = 0; v < 2; ++v)
+          for (int off = 1; off < kBackPer; off <<= 1)
+            for (int q = 0; q < kBackPer; q += 2 * off) {
+              sx[v][q] += sx[v][q + off];
+              sy[v][q] += sy[v][q + off];
+            }
+        const cd al = cmul(ta, cmk(sx[0][0], sy[0][0]));
+        const cd cc = k >= 1 ? cpair[k] : cmk(0.0, 0.0);
+        const cd ac = cmul(al, cc);
+        const cd be = cmul(tb, cmk(sx[1][0] - ac.x, sy[1][0] - ac.y));
+        for (int g = r0; g < n; ++g) {
+          const cd fa = cmul(refl(a, k, g), al);
+          const cd fb = cmul(k >= 1 ? refl(a, k - 1, g) : cmk(0.0, 0.0), be);
+          cd& x = yb[g * nb + c];
+          x.x -= fa.x + fb.x;
+          x.y -= fa.y + fb.y;
         }
       }
 #endif
     t.sync();
     for (int idx = t.tid; idx < n * nb; idx += t.nthr) {
       const int i = idx / nb, cc = idx % nb;
-      V[(long)i * lowest + c0 + cc] = yb[i * nb + cc];
+      V[(long)(rowmap ? rowmap[i] : i) * lowest + c0 + cc] = yb[i * nb + cc];
     }
     t.sync();
   }
@@ -684,7 +806,7 @@ constexpr double kUlp = 2.220446049250313e-16;      // 2⁻⁵²
 constexpr double kSafMin = 2.2250738585072014e-308;  // smallest normal
 constexpr int kMaxBisect = 128;                      // multisection steps (≥ 3 bits each)
 constexpr int kInvIts = 5;                           // inverse-iteration solves at most
-constexpr int kInvExtra = 2;                         // solves after the growth test first passes
+constexpr int kInvExtra = 1;                         // solves after the growth test first passes
 
 // threads that search one eigenvalue: a function of the problem size only
 EFN_QL_HD inline int bisect_lanes(int n) {
@@ -788,7 +910,7 @@ EFN_QL_HD inline void sturm_counts(const double* __restrict__ ds, const double* 
 }
 
 // Gershgorin interval of the block, widened as LAPACK dstebz does, and the
-// absolute tolerance ulp·‖T_b‖
+// absolute tolerance 8·ulp·‖T_b‖ (midpoints within 4 ulp·‖T_b‖)
 EFN_QL_HD inline void bisect_init(const double* d, const double* sub, int b0, int b1, double pivmin, double& lo,
                                   double& hi, double& atol) {
   double gl = d[b0], gu = d[b0];
@@ -803,7 +925,7 @@ EFN_QL_HD inline void bisect_init(const double* d, const double* sub, int b0, in
   const double fud = 2.1 * kUlp * tnorm * (b1 - b0) + 4.2 * pivmin;
   lo = gl - fud;
   hi = gu + fud;
-  atol = kUlp * tnorm;
+  atol = 8.0 * kUlp * tnorm;
 }
 
 // multisection point k of K inside (lo, hi)
@@ -844,14 +966,16 @@ EFN_QL_HD inline double start_entry(int j, int i) {
 }
 
 // Inverse iteration for the selected eigenvalues of one cluster (slots lead …
-// last of one block) by one thread with scratch column c of Kc: vectors into
-// V's real parts (rows of the block, column rank[s]).  Returns the number of
-// vectors whose iteration did not pass the stopping test (they are kept).
+// last of one block) by one thread with scratch column c of Kc: the unit
+// vector of slot s into vs[rank[s]·vr + g·vi] for the block's rows g (the
+// rest stays as it was, zero), each made orthogonal to the cluster's earlier
+// vectors there after every solve.  Returns the number of vectors whose
+// iteration did not pass the stopping test (they are kept).
 EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* __restrict__ sub,
                                    const double* __restrict__ ev, const double* __restrict__ onen,
                                    const int* __restrict__ bs, const int* __restrict__ be,
                                    const int* __restrict__ rank, int lead, int last, double* __restrict__ sc,
-                                   int nrow, int c, int Kc, int lowest, cd* __restrict__ V) {
+                                   int nrow, int c, int Kc, double* vs, long vr, long vi) {
   // scratch: array a ∈ {1/u0, u1, l, x} at sc[(a·nrow + i)·Kc + c], piv after
   // (with an interchange at row k, U's second superdiagonal there is T's
   // coupling sub[k+1], otherwise zero); the sweeps load kChunk rows'
@@ -867,7 +991,7 @@ EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* _
   for (int s = lead; s <= last; ++s) {
     const int b0 = bs[s], nb = be[s] - b0, r = rank[s];
     if (nb == 1) {
-      V[(long)b0 * lowest + r] = cmk(1.0, 0.0);
+      vs[r * vr + b0 * vi] = 1.0;
       continue;
     }
     // shift: the eigenvalue, kept 10·ulp·|σ| above the previous one of the cluster
@@ -927,16 +1051,18 @@ EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* _
       if (fabs(u0) < tol) u0 = u0 < 0.0 ? -tol : tol;
       ui[EFN_S(k)] = rcp_d(u0);
     }
+    double xm = 0.0;  // |x|∞ of the current iterate
 #pragma unroll 4
-    for (int i = 0; i < nb; ++i) X[EFN_S(i)] = start_entry(s - b0, i);
-    const double onenrm = onen[s], dtpcrt = sqrt(0.1 / nb);
+    for (int i = 0; i < nb; ++i) {
+      const double v = start_entry(s - b0, i);
+      X[EFN_S(i)] = v;
+      xm = fmax(xm, fabs(v));
+    }
+    const double onenrm = onen[b0], dtpcrt = sqrt(0.1 / nb);
     int chk = 0;
     bool conv = false;
     for (int its = 0; its < kInvIts && !conv; ++its) {
       // scale so that |x|∞ = nb·‖T_b‖₁·max(ulp, |u_last|)
-      double xm = 0.0;
-#pragma unroll 8
-      for (int i = 0; i < nb; ++i) xm = fmax(xm, fabs(X[EFN_S(i)]));
       const double scl = nb * onenrm * fmax(kUlp, fabs(ulast)) / xm;
       // y ← L⁻¹·P·(scl·x)
       double cur = X[EFN_S(0)] * scl;
@@ -968,7 +1094,7 @@ EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* _
       }
       X[EFN_S(nb - 1)] = cur;
       // x ← U⁻¹·y, rescaled by 2⁻⁵⁰⁰ whenever an entry passes 2⁵⁰⁰
-      double x1 = 0.0, x2 = 0.0;
+      double x1 = 0.0, x2 = 0.0, big = 0.0;
       for (int k1 = nb - 1; k1 >= 0; k1 -= kChunk) {
         double yy[kChunk], v1[kChunk], v2[kChunk], vi[kChunk];
 #pragma unroll
@@ -994,46 +1120,97 @@ EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* _
               xk *= 0x1.0p-500;
               x1 *= 0x1.0p-500;
               x2 *= 0x1.0p-500;
+              big *= 0x1.0p-500;
             }
+            big = fmax(big, fabs(xk));
             X[EFN_S(k)] = xk;
             x2 = x1;
             x1 = xk;
           }
         }
       }
-      // modified Gram–Schmidt against the cluster's earlier vectors
-      for (int q = lead; q < s; ++q) {
-        const cd* vq = V + (long)b0 * lowest + rank[q];
-        double dot = 0.0;
-        for (int i = 0; i < nb; ++i) dot += X[EFN_S(i)] * vq[(long)i * lowest].x;
-        for (int i = 0; i < nb; ++i) X[EFN_S(i)] -= dot * vq[(long)i * lowest].x;
+      // modified Gram–Schmidt against the cluster's earlier vectors (read
+      // kChunk rows at a time)
+      if (s > lead) {
+        for (int q = lead; q < s; ++q) {
+          const double* vq = vs + rank[q] * vr + b0 * vi;
+          double dot = 0.0;
+          for (int i0 = 0; i0 < nb; i0 += kChunk) {
+            double vv[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j)
+              if (i0 + j < nb) vv[j] = vq[(i0 + j) * vi];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j)
+              if (i0 + j < nb) dot += X[EFN_S(i0 + j)] * vv[j];
+          }
+          for (int i0 = 0; i0 < nb; i0 += kChunk) {
+            double vv[kChunk];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j)
+              if (i0 + j < nb) vv[j] = vq[(i0 + j) * vi];
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j)
+              if (i0 + j < nb) X[EFN_S(i0 + j)] -= dot * vv[j];
+          }
+        }
+        big = 0.0;
+        for (int i = 0; i < nb; ++i) big = fmax(big, fabs(X[EFN_S(i)]));
       }
-      double nrm = 0.0;
-#pragma unroll 8
-      for (int i = 0; i < nb; ++i) nrm = fmax(nrm, fabs(X[EFN_S(i)]));
-      if (nrm < dtpcrt) continue;
+      xm = big;
+      if (big < dtpcrt) continue;
       if (++chk < kInvExtra + 1) continue;
       conv = true;
     }
     if (!conv) ++fails;
     // unit 2-norm, the largest entry (first of equals) positive
-    double ss = 0.0, big = -1.0, bigv = 0.0;
+    double ss = 0.0, top = -1.0, bigv = 0.0;
 #pragma unroll 8
     for (int i = 0; i < nb; ++i) {
       const double v = X[EFN_S(i)];
       ss += v * v;
-      if (fabs(v) > big) {
-        big = fabs(v);
+      if (fabs(v) > top) {
+        top = fabs(v);
         bigv = v;
       }
     }
     double scl = 1.0 / sqrt(ss);
     if (bigv < 0.0) scl = -scl;
 #pragma unroll 8
-    for (int i = 0; i < nb; ++i) V[(long)(b0 + i) * lowest + r] = cmk(X[EFN_S(i)] * scl, 0.0);
+    for (int i = 0; i < nb; ++i) vs[r * vr + (b0 + i) * vi] = X[EFN_S(i)] * scl;
   }
 #undef EFN_S
   return fails;
+}
+
+// Modified Gram–Schmidt over one cluster's unit vectors in LDS (vector of
+// slot s at zf[rank[s]·n + g], block rows g), in slot order, each result
+// normalised with its largest entry (first of equals) positive.
+EFN_QL_HD inline void orth_cluster(double* __restrict__ zf, int n, const int* __restrict__ bs,
+                                   const int* __restrict__ be, const int* __restrict__ rank, int lead, int last) {
+  const int b0 = bs[lead], nb = be[lead] - b0;
+  for (int s = lead + 1; s <= last; ++s) {
+    double* xj = zf + (long)rank[s] * n + b0;
+    for (int q = lead; q < s; ++q) {
+      const double* xq = zf + (long)rank[q] * n + b0;
+      double dot = 0.0;
+#pragma unroll 8
+      for (int i = 0; i < nb; ++i) dot += xq[i] * xj[i];
+#pragma unroll 8
+      for (int i = 0; i < nb; ++i) xj[i] -= dot * xq[i];
+    }
+    double ss = 0.0, top = -1.0, bigv = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      ss += xj[i] * xj[i];
+      if (fabs(xj[i]) > top) {
+        top = fabs(xj[i]);
+        bigv = xj[i];
+      }
+    }
+    double scl = 1.0 / sqrt(ss);
+    if (bigv < 0.0) scl = -scl;
+    for (int i = 0; i < nb; ++i) xj[i] *= scl;
+  }
 }
 
 // The bisection + inverse-iteration solver for one matrix: arguments and
@@ -1059,51 +1236,104 @@ EFN_QL_HD inline void heevx_body(const Team& t, const cd* A, int lda, int n, int
   const int low = lowest < n ? lowest : n;
   const double inf = HUGE_VAL;
 
-  reduce_tridiag(t, A, lda, n, W, false);
+  // ---- rows with no off-diagonal entry are eigenvectors already: ordered
+  // last (live rows first, each group in its order), the reduction leaves
+  // them untouched and T splits there, so the live block's norm — not a
+  // penalty on such rows — sets the cluster scale below.  V's rows are
+  // mapped back at the last store.
+  int* rowmap = reinterpret_cast<int*>(W.cs + 2 * W.csn);
+  int* iso = reinterpret_cast<int*>(W.cs + 3 * W.csn);
+  for (int i = t.tid; i < n; i += t.nthr) {
+    bool z = true;
+    for (int j = 0; j < i && z; ++j) z = A[(long)i * lda + j].x == 0.0 && A[(long)i * lda + j].y == 0.0;
+    for (int j = i + 1; j < n && z; ++j) z = A[(long)j * lda + i].x == 0.0 && A[(long)j * lda + i].y == 0.0;
+    iso[i] = z;
+  }
+  t.sync();
+  if (t.tid == 0) {
+    int nlive = 0;
+    for (int i = 0; i < n; ++i) nlive += !iso[i];
+    for (int i = 0, p = 0, q = nlive; i < n; ++i) rowmap[iso[i] ? q++ : p++] = i;
+  }
+  t.sync();
+
+  reduce_tridiag(t, A, lda, n, W, false, rowmap);
 
   // ---- split into unreduced blocks; pivmin; ‖T_b‖₁ per row; the copy of T
-  // scaled by 2^−σ ≥ 1/max_i(|d_i| + |sub_{i−1}| + |sub_i|) for the counts --
-  if (t.tid == 0) {
-    double emax = 0.0, rmax = 0.0;
-    for (int i = 0; i < n; ++i) {
-      rmax = fmax(rmax, fabs(d[i]) + (i > 0 ? fabs(sub[i - 1]) : 0.0) + (i + 1 < n ? fabs(sub[i]) : 0.0));
-      if (i + 1 < n) emax = fmax(emax, sub[i] * sub[i]);
-    }
-    int ex = 0;
-    if (rmax > 0.0) (void)frexp(rmax, &ex);
-    const double sc = ldexp(1.0, -ex);
-    W.cs[2 * W.csn] = kSafMin * fmax(1.0, emax);  // pivmin of the unscaled matrix
-    W.cs[2 * W.csn + 1] = sc;
-    int b0 = 0;
-    if (n > 0) ee[0] = 0.0;
-    for (int i = 0; i < n; ++i) {
-      ds[i] = d[i] * sc;
-      bool split = i == n - 1;
-      if (!split) {
+  // scaled by 2^−σ ≥ 1/max_i(|d_i| + |sub_{i−1}| + |sub_i|) for the counts.
+  // Row sums and split flags per row; the two maxima by 64-bit atomic max on
+  // the bit patterns (non-negative doubles order as their bits); each row
+  // then finds its block and the block's norm by scanning its own block.
+  int* split = bs + 3 * n;
+  double* rows = W.cs + 3 * W.csn;
+  unsigned long long* mx = reinterpret_cast<unsigned long long*>(ctl + kCtlMax);  // rmax, emax
+  if (t.tid == 0) mx[0] = mx[1] = 0ull;
+  t.sync();
+  {
+    unsigned long long rb = 0ull, eb = 0ull;
+    for (int i = t.tid; i < n; i += t.nthr) {
+      const double s = fabs(d[i]) + (i > 0 ? fabs(sub[i - 1]) : 0.0) + (i + 1 < n ? fabs(sub[i]) : 0.0);
+      rows[i] = s;
+      bool sp = i == n - 1;
+      if (!sp) {
         const double e = sub[i] * sub[i];
-        split = e <= kUlp * kUlp * fabs(d[i] * d[i + 1]) + kSafMin;
-        const double es = sub[i] * sc;
-        ee[i + 1] = split ? 0.0 : es * es;
+        sp = e <= kUlp * kUlp * fabs(d[i] * d[i + 1]) + kSafMin;
+        const unsigned long long b = (unsigned long long)__double_as_longlong_hd(e);
+        eb = b > eb ? b : eb;
       }
-      if (split) {
-        double on = 0.0;
-        for (int r = b0; r <= i; ++r) {
-          double s = fabs(d[r]);
-          if (r > b0) s += fabs(sub[r - 1]);
-          if (r < i) s += fabs(sub[r]);
-          on = fmax(on, s);
-        }
-        for (int r = b0; r <= i; ++r) {
-          bs[r] = b0;
-          be[r] = i + 1;
-          onen[r] = on;
-        }
-        b0 = i + 1;
-      }
+      split[i] = sp;
+      const unsigned long long b = (unsigned long long)__double_as_longlong_hd(s);
+      rb = b > rb ? b : rb;
+    }
+#ifdef __HIP_DEVICE_COMPILE__
+    if (rb) atomicMax(&mx[0], rb);
+    if (eb) atomicMax(&mx[1], eb);
+#else
+    mx[0] = rb > mx[0] ? rb : mx[0];
+    mx[1] = eb > mx[1] ? eb : mx[1];
+#endif
+  }
+  t.sync();
+  const double rmax = __longlong_as_double_hd((long long)mx[0]), emax = __longlong_as_double_hd((long long)mx[1]);
+  int ex = 0;
+  if (rmax > 0.0) (void)frexp(rmax, &ex);
+  const double scl = ldexp(1.0, -ex);
+  const double pivmin = kSafMin * fmax(1.0, emax);  // of the unscaled matrix
+  // block bounds from the split flags as a 128-bit mask (n ≤ 96): a block
+  // ends at each split row
+  unsigned long long* msk = reinterpret_cast<unsigned long long*>(ctl + kCtlMask);
+#ifdef __HIP_DEVICE_COMPILE__
+  if (t.tid < 64) {
+    const unsigned long long m0 = __ballot(t.tid < n && split[t.tid]);
+    const unsigned long long m1 = __ballot(64 + t.tid < n && split[64 + t.tid]);
+    if (t.tid == 0) {
+      msk[0] = m0;
+      msk[1] = m1;
+    }
+  }
+#else
+  if (t.tid == 0) {
+    msk[0] = msk[1] = 0ull;
+    for (int i = 0; i < n; ++i)
+      if (split[i]) msk[i >> 6] |= 1ull << (i & 63);
+  }
+#endif
+  t.sync();
+  for (int i = t.tid; i < n; i += t.nthr) {
+    ds[i] = d[i] * scl;
+    const double es = i > 0 ? sub[i - 1] * scl : 0.0;
+    ee[i] = i > 0 && !split[i - 1] ? es * es : 0.0;
+    const int b0 = i == 0 ? 0 : last_bit_below(msk, i) + 1, b1 = first_bit_from(msk, i);
+    bs[i] = b0;
+    be[i] = b1 + 1;
+    if (i == b0) {  // the block's ‖T_b‖₁, kept at its first row
+      double on = 0.0;
+      for (int r = b0; r <= b1; ++r) on = fmax(on, fabs(d[r]) + (r > b0 ? fabs(sub[r - 1]) : 0.0) +
+                                                       (r < b1 ? fabs(sub[r]) : 0.0));
+      onen[b0] = on;
     }
   }
   t.sync();
-  const double pivmin = W.cs[2 * W.csn], scl = W.cs[2 * W.csn + 1];
 
   // ---- every eigenvalue of every block by multisection ----------------------
   {
@@ -1166,9 +1396,15 @@ EFN_QL_HD inline void heevx_body(const Team& t, const cd* A, int lda, int n, int
   }
   t.sync();
   // ascending within each block (a Sturm count need not be monotone in x)
-  if (t.tid == 0)
-    for (int s = 1; s < n; ++s)
-      if (bs[s] == bs[s - 1] && ev[s] < ev[s - 1]) ev[s] = ev[s - 1];
+  if (t.tid == 0) {
+    double prev = 0.0;
+    for (int s = 0; s < n; ++s) {
+      double v = ev[s];
+      if (s > bs[s] && v < prev) v = prev;
+      ev[s] = v;
+      prev = v;
+    }
+  }
   t.sync();
 
   // ---- the lowest `low`: rank of each slot (ties by slot) -------------------
@@ -1181,56 +1417,79 @@ EFN_QL_HD inline void heevx_body(const Team& t, const cd* A, int lda, int n, int
   }
   t.sync();
   for (int r = t.tid; r < lowest; r += t.nthr) w[r] = r < low ? ev[perm[r]] : inf;
-  // clusters with a selected member: (first slot, last selected slot); within
-  // a block the selected slots are a prefix (ranks grow with the slot)
-  if (t.tid == 0) {
-    int nc = 0, lead = -1, last = -1;
-    for (int s = 0; s < n; ++s) {
-      const bool leader = s == bs[s] || ev[s] - ev[s - 1] > 1e-3 * onen[s];
-      if (leader) {
-        if (last >= 0) {
-          gr[2 * nc] = lead;
-          gr[2 * nc + 1] = last;
-          ++nc;
-        }
-        lead = s;
-        last = -1;
-      }
-      if (rank[s] < low) last = s;
-    }
-    if (last >= 0) {
-      gr[2 * nc] = lead;
-      gr[2 * nc + 1] = last;
-      ++nc;
-    }
-    ctl[kCtlClus] = nc;
-  }
-  for (long idx = t.tid; idx < (long)lda * lowest; idx += t.nthr) V[idx] = cmk(0.0, 0.0);
+  // clusters with a selected member, as (first slot, last selected slot) in
+  // slots taken by an atomic counter (a cluster's vectors do not depend on
+  // its place in the list); within a block the selected slots are a prefix
+  // (ranks grow with the slot).  A cluster leads with a block's first slot or
+  // a gap above 10⁻³·‖T_b‖₁ to the slot before.
+  for (int s = t.tid; s < n; s += t.nthr) {
+    const bool leader = s == bs[s] || ev[s] - ev[s - 1] > 1e-3 * onen[bs[s]];
+    if (leader && rank[s] < low) {
+      int last = s;
+      while (last + 1 < be[s] && rank[last + 1] < low && !(ev[last + 1] - ev[last] > 1e-3 * onen[bs[s]])) ++last;
 #ifdef __HIP_DEVICE_COMPILE__
-  __threadfence_block();
+      const int slot = atomicAdd(&ctl[kCtlClus], 1);
+#else
+      const int slot = ctl[kCtlClus]++;
 #endif
-  t.sync();
-
-  // ---- eigenvectors of T by inverse iteration, one cluster a thread ---------
+      gr[2 * slot] = s;
+      gr[2 * slot + 1] = last;
+    }
+  }
+  // ---- eigenvectors of T by inverse iteration.  With scratch beside Z (the
+  // launch leaves LDS room), every selected vector is iterated by its own
+  // thread into Z (column-major, zf[rank·n + row]) and each cluster is then
+  // made orthonormal by one thread; without, one thread iterates a whole
+  // cluster, orthogonalising after every solve, straight into V.  An
+  // independent vector's components along its cluster's other vectors are
+  // of size ulp·‖T_b‖/gap, so removing them moves its residual by ulp·‖T_b‖.
   {
     const int Kc = invit_slots(n, nl), nc = ctl[kCtlClus];
     double* sc = invit_scratch(nl, lds, W);
-    const int c = t.tid;
-    if (c < Kc) {
-      int fails = 0;
-      for (int q = c; q < nc; q += (t.nthr > 1 ? Kc : 1))
-        fails += invit_cluster(d, sub, ev, onen, bs, be, rank, gr[2 * q], gr[2 * q + 1], sc, n,
-                               t.nthr > 1 ? c : 0, Kc, lowest, V);
-      if (fails) {
+    const bool par = heevx_extra_bytes(nl) > 0;
+    double* zf = W.z;
+    int fails = 0;
+    if (par) {
+      for (long idx = t.tid; idx < (long)low * n; idx += t.nthr) zf[idx] = 0.0;
+      t.sync();
+      if (t.tid < Kc)
+        for (int r = t.tid; r < low; r += (t.nthr > 1 ? Kc : 1))
+          fails += invit_cluster(d, sub, ev, onen, bs, be, rank, perm[r], perm[r], sc, n, t.nthr > 1 ? t.tid : 0,
+                                 Kc, zf, n, 1);
 #ifdef __HIP_DEVICE_COMPILE__
-        atomicAdd(&ctl[kCtlFail], fails);
-#else
-        ctl[kCtlFail] += fails;
+      __threadfence_block();
 #endif
+      t.sync();
+      for (int q = t.tid; q < nc; q += t.nthr)
+        if (gr[2 * q + 1] > gr[2 * q]) orth_cluster(zf, n, bs, be, rank, gr[2 * q], gr[2 * q + 1]);
+#ifdef __HIP_DEVICE_COMPILE__
+      __threadfence_block();
+#endif
+      t.sync();
+      for (long idx = t.tid; idx < (long)lda * lowest; idx += t.nthr) {
+        const int i = (int)(idx / lowest), r = (int)(idx % lowest);
+        V[idx] = cmk(i < n && r < low ? zf[(long)r * n + i] : 0.0, 0.0);
       }
+    } else {
+      for (long idx = t.tid; idx < (long)lda * lowest; idx += t.nthr) V[idx] = cmk(0.0, 0.0);
+#ifdef __HIP_DEVICE_COMPILE__
+      __threadfence_block();
+#endif
+      t.sync();
+      if (t.tid < Kc)
+        for (int q = t.tid; q < nc; q += (t.nthr > 1 ? Kc : 1))
+          fails += invit_cluster(d, sub, ev, onen, bs, be, rank, gr[2 * q], gr[2 * q + 1], sc, n,
+                                 t.nthr > 1 ? t.tid : 0, Kc, reinterpret_cast<double*>(V), 2, 2L * lowest);
+    }
+    if (fails) {
+#ifdef __HIP_DEVICE_COMPILE__
+      atomicAdd(&ctl[kCtlFail], fails);
+#else
+      ctl[kCtlFail] += fails;
+#endif
     }
   }
-  back_transform(t, W, n, nl, lda, lowest, low, V);
+  back_transform(t, W, n, nl, lda, lowest, low, V, rowmap);
   if (t.tid == 0) *info = ctl[kCtlFail];
 }
 

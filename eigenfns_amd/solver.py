@@ -236,8 +236,10 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
             return torch.stack([lam, st[:, 2], rr2, st[:, 3]])
 
         def small_eigh(A):
+            # the update reads the lowest m eigenvectors only; asking the
+            # device solver for no more also skips the dead-row penalty cluster
             if graph:
-                w, V = eigh_batched_device(((A + A.conj().T) / 2)[None])
+                w, V = eigh_batched_device(((A + A.conj().T) / 2)[None], lowest=m)
                 return w[0], V[0]
             return eigh(A)
 
@@ -349,7 +351,7 @@ def lobpcg_blocks(op, nev: int, m: int = 96, guard: int = 48, tol: float = 1e-4,
                     g = (torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph())
                     with torch.cuda.graph(g[0]):
                         g_A, g_nb = tail_rr_matrix(True)
-                    g_V = torch.empty_like(g_A)
+                    g_V = torch.empty((g_A.shape[0], m), dtype=g_A.dtype, device=g_A.device)
                     with torch.cuda.graph(g[1]):
                         g_stats = tail_update(g_V, g_nb)
                 g[0].replay()

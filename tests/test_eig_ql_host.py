@@ -7,8 +7,8 @@ code with 256 threads and barriers between their phases), against LAPACK
 
 Bars: eigenvalues within 1e-13·max|A| of LAPACK's, residual ‖AV − VΛ‖ within
 1e-12·max|A|, ‖VᴴV − I‖ below 1e-12, zero padding past the problem's size;
-random, repeated-eigenvalue, close-cluster, tight-cluster, diagonal, zero and penalised
-(Rayleigh–Ritz with dead rows) matrices of sizes 1-96, every lowest subset
+random, repeated-eigenvalue, close-cluster, tight-cluster, diagonal, zero, penalised
+(Rayleigh–Ritz with dead rows) and isolated (dead rows decoupled) matrices of sizes 1-96, every lowest subset
 the leading columns of the full solve bit for bit."""
 import ctypes as C
 import shutil
@@ -81,6 +81,12 @@ def _matrix(rng, n, kind):
         A = np.diag(rng.standard_normal(n)).astype(np.complex128)
     elif kind == "zero":
         A = np.zeros((n, n), np.complex128)
+    elif kind == "isolated":  # dead rows decoupled exactly, penalties 1e4·(1 + 1e-6·j) (the solver's form)
+        A = X @ X.conj().T / n
+        dead = np.flatnonzero(rng.random(n) < 0.3)
+        A[dead, :] = 0
+        A[:, dead] = 0
+        A[dead, dead] = 1e4 * (1.0 + 1e-6 * np.arange(len(dead)))
     else:  # PSD with dead rows penalised at 1e4 (a Rayleigh–Ritz matrix)
         A = X @ X.conj().T / n
         dead = rng.random(n) < 0.3
@@ -89,7 +95,7 @@ def _matrix(rng, n, kind):
 
 
 @pytest.mark.parametrize("method", ["ql", "heevx"])
-@pytest.mark.parametrize("kind", ["random", "repeated", "close", "tight", "diagonal", "zero", "penalised"])
+@pytest.mark.parametrize("kind", ["random", "repeated", "close", "tight", "diagonal", "zero", "penalised", "isolated"])
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 33, 72, 96])
 def test_body_matches_lapack(ql, n, kind, method):
     rng = np.random.default_rng(1000 * n + len(kind))
