@@ -793,10 +793,10 @@ EFN_QL_HD inline void heev_ql_body(const Team& t, const cd* A, int lda, int n, i
 //   3. per selected eigenvalue, inverse iteration on T_b − σI (LU with partial
 //      pivoting, random start, at most kInvIts solves, stopped kInvExtra
 //      solves after the iterate's growth shows σ is an eigenvalue to working
-//      precision), the vectors of a cluster (consecutive eigenvalues of a
-//      block closer than 10⁻³·‖T_b‖₁) made orthogonal to the cluster's
-//      earlier vectors by modified Gram–Schmidt after every solve; one thread
-//      per cluster, the clusters spread over the threads that have scratch;
+//      precision), one thread per vector; then the vectors of each cluster
+//      (consecutive eigenvalues of a block closer than 10⁻³·‖T_b‖₁) made
+//      orthonormal by modified Gram–Schmidt, one thread per cluster, one
+//      more solve from each, and Gram–Schmidt again;
 //   4. the same back-transformation.
 // Every quantity of eigenpair s depends on the matrix, s and n_p only — not on
 // `lowest`, the launch size or the thread that computes it — so a lowest-k
@@ -969,13 +969,14 @@ EFN_QL_HD inline double start_entry(int j, int i) {
 // last of one block) by one thread with scratch column c of Kc: the unit
 // vector of slot s into vs[rank[s]·vr + g·vi] for the block's rows g (the
 // rest stays as it was, zero), each made orthogonal to the cluster's earlier
-// vectors there after every solve.  Returns the number of vectors whose
-// iteration did not pass the stopping test (they are kept).
+// vectors there after every solve.  With refine, one solve from the vector
+// already there instead (no stopping test).  Returns the number of vectors
+// whose iteration did not pass the stopping test (they are kept).
 EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* __restrict__ sub,
                                    const double* __restrict__ ev, const double* __restrict__ onen,
                                    const int* __restrict__ bs, const int* __restrict__ be,
                                    const int* __restrict__ rank, int lead, int last, double* __restrict__ sc,
-                                   int nrow, int c, int Kc, double* vs, long vr, long vi) {
+                                   int nrow, int c, int Kc, double* vs, long vr, long vi, bool refine = false) {
   // scratch: array a ∈ {1/u0, u1, l, x} at sc[(a·nrow + i)·Kc + c], piv after
   // (with an interchange at row k, U's second superdiagonal there is T's
   // coupling sub[k+1], otherwise zero); the sweeps load kChunk rows'
@@ -1054,14 +1055,14 @@ EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* _
     double xm = 0.0;  // |x|∞ of the current iterate
 #pragma unroll 4
     for (int i = 0; i < nb; ++i) {
-      const double v = start_entry(s - b0, i);
+      const double v = refine ? vs[r * vr + (b0 + i) * vi] : start_entry(s - b0, i);
       X[EFN_S(i)] = v;
       xm = fmax(xm, fabs(v));
     }
     const double onenrm = onen[b0], dtpcrt = sqrt(0.1 / nb);
     int chk = 0;
     bool conv = false;
-    for (int its = 0; its < kInvIts && !conv; ++its) {
+    for (int its = 0; its < (refine ? 1 : kInvIts) && !conv; ++its) {
       // scale so that |x|∞ = nb·‖T_b‖₁·max(ulp, |u_last|)
       const double scl = nb * onenrm * fmax(kUlp, fabs(ulast)) / xm;
       // y ← L⁻¹·P·(scl·x)
@@ -1158,11 +1159,12 @@ EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* _
         for (int i = 0; i < nb; ++i) big = fmax(big, fabs(X[EFN_S(i)]));
       }
       xm = big;
+      if (refine) break;
       if (big < dtpcrt) continue;
       if (++chk < kInvExtra + 1) continue;
       conv = true;
     }
-    if (!conv) ++fails;
+    if (!conv && !refine) ++fails;
     // unit 2-norm, the largest entry (first of equals) positive
     double ss = 0.0, top = -1.0, bigv = 0.0;
 #pragma unroll 8
@@ -1184,21 +1186,26 @@ EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* _
 }
 
 // Modified Gram–Schmidt over one cluster's unit vectors in LDS (vector of
-// slot s at zf[rank[s]·n + g], block rows g), in slot order, each result
-// normalised with its largest entry (first of equals) positive.
+// slot s at zf[rank[s]·n + g], block rows g), in slot order, each vector
+// projected against the earlier ones twice (a second pass restores the
+// orthogonality the first loses to cancellation when vectors of a tight
+// cluster are nearly dependent), then normalised with its largest entry
+// (first of equals) positive.
 EFN_QL_HD inline void orth_cluster(double* __restrict__ zf, int n, const int* __restrict__ bs,
                                    const int* __restrict__ be, const int* __restrict__ rank, int lead, int last) {
   const int b0 = bs[lead], nb = be[lead] - b0;
   for (int s = lead + 1; s <= last; ++s) {
     double* xj = zf + (long)rank[s] * n + b0;
-    for (int q = lead; q < s; ++q) {
-      const double* xq = zf + (long)rank[q] * n + b0;
-      double dot = 0.0;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int q = lead; q < s; ++q) {
+        const double* xq = zf + (long)rank[q] * n + b0;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};  // rows i ≡ 0…3 (mod 4), summed in that order
 #pragma unroll 8
-      for (int i = 0; i < nb; ++i) dot += xq[i] * xj[i];
+        for (int i = 0; i < nb; ++i) acc[i & 3] += xq[i] * xj[i];
+        const double dot = (acc[0] + acc[1]) + (acc[2] + acc[3]);
 #pragma unroll 8
-      for (int i = 0; i < nb; ++i) xj[i] -= dot * xq[i];
-    }
+        for (int i = 0; i < nb; ++i) xj[i] -= dot * xq[i];
+      }
     double ss = 0.0, top = -1.0, bigv = 0.0;
     for (int i = 0; i < nb; ++i) {
       ss += xj[i] * xj[i];
@@ -1244,9 +1251,17 @@ EFN_QL_HD inline void heevx_body(const Team& t, const cd* A, int lda, int n, int
   int* rowmap = reinterpret_cast<int*>(W.cs + 2 * W.csn);
   int* iso = reinterpret_cast<int*>(W.cs + 3 * W.csn);
   for (int i = t.tid; i < n; i += t.nthr) {
-    bool z = true;
-    for (int j = 0; j < i && z; ++j) z = A[(long)i * lda + j].x == 0.0 && A[(long)i * lda + j].y == 0.0;
-    for (int j = i + 1; j < n && z; ++j) z = A[(long)j * lda + i].x == 0.0 && A[(long)j * lda + i].y == 0.0;
+    bool z = true;  // every off-diagonal entry of row i (lower triangle: row i and column i), loads issued 8 at a time
+    for (int j0 = 0; j0 < n; j0 += kChunk) {
+      cd v[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const int j = j0 + u;
+        v[u] = j < n && j != i ? (j < i ? A[(long)i * lda + j] : A[(long)j * lda + i]) : cmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) z = z && v[u].x == 0.0 && v[u].y == 0.0;
+    }
     iso[i] = z;
   }
   t.sync();
@@ -1436,50 +1451,104 @@ EFN_QL_HD inline void heevx_body(const Team& t, const cd* A, int lda, int n, int
       gr[2 * slot + 1] = last;
     }
   }
-  // ---- eigenvectors of T by inverse iteration.  With scratch beside Z (the
-  // launch leaves LDS room), every selected vector is iterated by its own
-  // thread into Z (column-major, zf[rank·n + row]) and each cluster is then
-  // made orthonormal by one thread; without, one thread iterates a whole
-  // cluster, orthogonalising after every solve, straight into V.  An
-  // independent vector's components along its cluster's other vectors are
-  // of size ulp·‖T_b‖/gap, so removing them moves its residual by ulp·‖T_b‖.
+  // ---- eigenvectors of T: every selected vector by inverse iteration on its
+  // own thread, then each cluster made orthonormal once by one thread, in Z
+  // (column-major, zf[rank·n + row]).  The iterations' scratch sits beside Z
+  // where the launch leaves LDS room, the vectors then go straight into Z;
+  // otherwise the scratch is Z itself, the vectors go to V and are staged
+  // into Z afterwards (the same arithmetic either way).  An independent
+  // vector's components along its cluster's other vectors are of size
+  // ulp·‖T_b‖/gap, so removing them moves its residual by ulp·‖T_b‖.
   {
     const int Kc = invit_slots(n, nl), nc = ctl[kCtlClus];
     double* sc = invit_scratch(nl, lds, W);
-    const bool par = heevx_extra_bytes(nl) > 0;
+    const bool inz = heevx_extra_bytes(nl) > 0;
     double* zf = W.z;
+    double* vd = reinterpret_cast<double*>(V);
     int fails = 0;
-    if (par) {
+    if (inz)
       for (long idx = t.tid; idx < (long)low * n; idx += t.nthr) zf[idx] = 0.0;
+    else
+      for (long idx = t.tid; idx < (long)lda * lowest; idx += t.nthr) V[idx] = cmk(0.0, 0.0);
+#ifdef __HIP_DEVICE_COMPILE__
+    __threadfence_block();
+#endif
+    t.sync();
+    if (t.tid < Kc)
+      for (int r = t.tid; r < low; r += (t.nthr > 1 ? Kc : 1))
+        fails += invit_cluster(d, sub, ev, onen, bs, be, rank, perm[r], perm[r], sc, n, t.nthr > 1 ? t.tid : 0, Kc,
+                               inz ? zf : vd, inz ? n : 2, inz ? 1 : 2L * lowest);
+#ifdef __HIP_DEVICE_COMPILE__
+    __threadfence_block();
+#endif
+    t.sync();
+    if (!inz) {
+      for (long idx = t.tid; idx < (long)low * n; idx += t.nthr) {
+        const int r = (int)(idx / n), i = (int)(idx % n);
+        zf[idx] = V[(long)i * lowest + r].x;
+      }
       t.sync();
-      if (t.tid < Kc)
-        for (int r = t.tid; r < low; r += (t.nthr > 1 ? Kc : 1))
-          fails += invit_cluster(d, sub, ev, onen, bs, be, rank, perm[r], perm[r], sc, n, t.nthr > 1 ? t.tid : 0,
-                                 Kc, zf, n, 1);
+    }
+    for (int q = t.tid; q < nc; q += t.nthr)
+      if (gr[2 * q + 1] > gr[2 * q]) orth_cluster(zf, n, bs, be, rank, gr[2 * q], gr[2 * q + 1]);
+#ifdef __HIP_DEVICE_COMPILE__
+    __threadfence_block();
+#endif
+    t.sync();
+    // members of clusters: one more solve from the orthonormal vector (it
+    // damps what the Gram–Schmidt amplified from the other vectors' errors
+    // outside the cluster) and Gram–Schmidt again
+    // (a slot is a member when a neighbour of its block lies within the
+    // cluster gap, selected or not, so the set does not depend on `lowest`)
+    int* inclus = bs + 3 * n;  // the split flags' place, free again
+    bool any = false;
+    for (int s = t.tid; s < n; s += t.nthr) {
+      const double gap = 1e-3 * onen[bs[s]];
+      const bool m = (s > bs[s] && !(ev[s] - ev[s - 1] > gap)) || (s + 1 < be[s] && !(ev[s + 1] - ev[s] > gap));
+      inclus[s] = m;
+      any = any || (m && rank[s] < low);
+    }
+#ifdef __HIP_DEVICE_COMPILE__
+    any = __syncthreads_or(any);
+#else
+    t.sync();
+#endif
+    if (any) {
+      if (!inz)
+        for (long idx = t.tid; idx < (long)low * n; idx += t.nthr) {
+          const int r = (int)(idx / n), i = (int)(idx % n);
+          V[(long)i * lowest + r] = cmk(zf[idx], 0.0);
+        }
 #ifdef __HIP_DEVICE_COMPILE__
       __threadfence_block();
 #endif
       t.sync();
+      if (t.tid < Kc)
+        for (int r = t.tid; r < low; r += (t.nthr > 1 ? Kc : 1))
+          if (inclus[perm[r]])
+            invit_cluster(d, sub, ev, onen, bs, be, rank, perm[r], perm[r], sc, n, t.nthr > 1 ? t.tid : 0, Kc,
+                          inz ? zf : vd, inz ? n : 2, inz ? 1 : 2L * lowest, true);
+#ifdef __HIP_DEVICE_COMPILE__
+      __threadfence_block();
+#endif
+      t.sync();
+      if (!inz) {
+        for (long idx = t.tid; idx < (long)low * n; idx += t.nthr) {
+          const int r = (int)(idx / n), i = (int)(idx % n);
+          zf[idx] = V[(long)i * lowest + r].x;
+        }
+        t.sync();
+      }
       for (int q = t.tid; q < nc; q += t.nthr)
         if (gr[2 * q + 1] > gr[2 * q]) orth_cluster(zf, n, bs, be, rank, gr[2 * q], gr[2 * q + 1]);
 #ifdef __HIP_DEVICE_COMPILE__
       __threadfence_block();
 #endif
       t.sync();
-      for (long idx = t.tid; idx < (long)lda * lowest; idx += t.nthr) {
-        const int i = (int)(idx / lowest), r = (int)(idx % lowest);
-        V[idx] = cmk(i < n && r < low ? zf[(long)r * n + i] : 0.0, 0.0);
-      }
-    } else {
-      for (long idx = t.tid; idx < (long)lda * lowest; idx += t.nthr) V[idx] = cmk(0.0, 0.0);
-#ifdef __HIP_DEVICE_COMPILE__
-      __threadfence_block();
-#endif
-      t.sync();
-      if (t.tid < Kc)
-        for (int q = t.tid; q < nc; q += (t.nthr > 1 ? Kc : 1))
-          fails += invit_cluster(d, sub, ev, onen, bs, be, rank, gr[2 * q], gr[2 * q + 1], sc, n,
-                                 t.nthr > 1 ? t.tid : 0, Kc, reinterpret_cast<double*>(V), 2, 2L * lowest);
+    }
+    for (long idx = t.tid; idx < (long)lda * lowest; idx += t.nthr) {
+      const int i = (int)(idx / lowest), r = (int)(idx % lowest);
+      V[idx] = cmk(i < n && r < low ? zf[(long)r * n + i] : 0.0, 0.0);
     }
     if (fails) {
 #ifdef __HIP_DEVICE_COMPILE__

@@ -644,15 +644,21 @@ def eigh_list_host(mats, lowest=None):
 
 
 SMALL_METHODS = ("heevx", "ql")
-SMALL_METHOD = "heevx"  # the device eigensolver for k ≤ 96 when none is named
+# The device eigensolver for k ≤ 96 when none is named.  On general Hermitian
+# matrices "heevx" is the faster (72², lowest 24: 0.58 ms against 1.95 ms),
+# but LOBPCG's Rayleigh–Ritz matrices put the wanted Ritz values close
+# together relative to the W and P directions' large Rayleigh quotients, so
+# its clusters (and their orthogonalisation) grow; there QL measured faster
+# (C1 as one graph: 0.30 s against 0.33 s; C2: 7.2 s against 8.0 s).
+SMALL_METHOD = "ql"
 
 
 def eigh_ragged_device(A: torch.Tensor, sizes, lowest: int | None = None, method: str | None = None):
     """The lowest eigenpairs of each problem's leading sizes[p] × sizes[p]
     block of a (nprob ≤ 64, K, K) complex128 device stack (K ≤ 96), in one
     launch of one of the library's tridiagonal eigensolvers: method "heevx"
-    (efn_heevx_batched: bisection and inverse iteration, the default) or "ql"
-    (efn_heev_ql_batched: implicit QL).  w (nprob, lowest) ascending, +inf
+    (efn_heevx_batched: bisection and inverse iteration) or "ql"
+    (efn_heev_ql_batched: implicit QL); None: SMALL_METHOD.  w (nprob, lowest) ascending, +inf
     past min(lowest, sizes[p]); V (nprob, K, lowest), zero past sizes[p] rows
     and min(lowest, sizes[p]) columns.  Under "ql" a problem whose iteration
     failed gets NaN eigenvalues."""
