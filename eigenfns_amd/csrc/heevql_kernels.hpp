@@ -1185,39 +1185,98 @@ EFN_QL_HD inline int invit_cluster(const double* __restrict__ d, const double* _
   return fails;
 }
 
-// Modified Gram–Schmidt over one cluster's unit vectors in LDS (vector of
-// slot s at zf[rank[s]·n + g], block rows g), in slot order, each vector
-// projected against the earlier ones twice (a second pass restores the
-// orthogonality the first loses to cancellation when vectors of a tight
-// cluster are nearly dependent), then normalised with its largest entry
-// (first of equals) positive.
+// Classical Gram–Schmidt, twice, over one cluster's unit vectors in LDS
+// (vector of slot s at zf[rank[s]·n + g], block rows g): in slot order each
+// vector loses its components along all the earlier ones, taken from one
+// copy of it, and again (the second pass restores the orthogonality the
+// first loses to cancellation when vectors of a tight cluster are nearly
+// dependent), then is normalised with its largest entry (first of equals)
+// positive.  On the device one wave runs a cluster without workgroup
+// barriers: lane l forms the dot product with the cluster's l-th earlier
+// vector (rows in order), the dots meet in a per-wave LDS row, each lane
+// updates its own rows (earlier vectors in order), and the norm and the
+// largest entry come from lane shuffles; dsc: ≥ 96 doubles of the wave's own.
+EFN_QL_HD inline void wave_order() {
+#ifdef __HIP_DEVICE_COMPILE__
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#endif
+}
 EFN_QL_HD inline void orth_cluster(double* __restrict__ zf, int n, const int* __restrict__ bs,
-                                   const int* __restrict__ be, const int* __restrict__ rank, int lead, int last) {
+                                   const int* __restrict__ be, const int* __restrict__ rank, int lead, int last,
+                                   double* __restrict__ dsc, int lane) {
   const int b0 = bs[lead], nb = be[lead] - b0;
+#ifdef __HIP_DEVICE_COMPILE__
+  constexpr int W = 64;
+#else
+  constexpr int W = 1;
+  (void)lane;
+  lane = 0;
+#endif
   for (int s = lead + 1; s <= last; ++s) {
     double* xj = zf + (long)rank[s] * n + b0;
-    for (int pass = 0; pass < 2; ++pass)
-      for (int q = lead; q < s; ++q) {
-        const double* xq = zf + (long)rank[q] * n + b0;
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};  // rows i ≡ 0…3 (mod 4), summed in that order
-#pragma unroll 8
-        for (int i = 0; i < nb; ++i) acc[i & 3] += xq[i] * xj[i];
-        const double dot = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-#pragma unroll 8
-        for (int i = 0; i < nb; ++i) xj[i] -= dot * xq[i];
+    const int c = s - lead;  // earlier vectors
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int l = lane; l < c; l += W) {
+        const double* xq = zf + (long)rank[lead + l] * n + b0;
+        double dot = 0.0;
+        for (int i = 0; i < nb; ++i) dot += xq[i] * xj[i];
+        dsc[l] = dot;
       }
+      wave_order();
+      for (int i = lane; i < nb; i += W) {
+        double acc = xj[i];
+        for (int l = 0; l < c; ++l) acc -= dsc[l] * zf[(long)rank[lead + l] * n + b0 + i];
+        xj[i] = acc;
+      }
+      wave_order();
+    }
+    // ‖x‖² and the largest entry (first of equals)
     double ss = 0.0, top = -1.0, bigv = 0.0;
-    for (int i = 0; i < nb; ++i) {
-      ss += xj[i] * xj[i];
-      if (fabs(xj[i]) > top) {
-        top = fabs(xj[i]);
-        bigv = xj[i];
+    int at = 1 << 30;
+    for (int i = lane; i < nb; i += W) {
+      const double v = xj[i];
+      ss += v * v;
+      if (fabs(v) > top) {
+        top = fabs(v);
+        bigv = v;
+        at = i;
       }
     }
+#ifdef __HIP_DEVICE_COMPILE__
+    for (int off = 1; off < W; off <<= 1) {
+      ss += __shfl_xor(ss, off, 64);
+      const double ot = __shfl_xor(top, off, 64), ov = __shfl_xor(bigv, off, 64);
+      const int oa = __shfl_xor(at, off, 64);
+      if (ot > top || (ot == top && oa < at)) {
+        top = ot;
+        bigv = ov;
+        at = oa;
+      }
+    }
+#endif
     double scl = 1.0 / sqrt(ss);
     if (bigv < 0.0) scl = -scl;
-    for (int i = 0; i < nb; ++i) xj[i] *= scl;
+    for (int i = lane; i < nb; i += W) xj[i] *= scl;
+    wave_order();
   }
+}
+
+// orth_cluster over every cluster with two members or more: one wave a
+// cluster on the device (waves take clusters in turn), then a barrier
+EFN_QL_HD inline void orth_all(const Team& t, double* zf, int n, const int* bs, const int* be, const int* rank,
+                               const int* gr, int nc, double* dsc) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const int wave = t.tid >> 6, nw = t.nthr >> 6, lane = t.tid & 63;
+  for (int q = wave; q < nc; q += nw)
+    if (gr[2 * q + 1] > gr[2 * q]) orth_cluster(zf, n, bs, be, rank, gr[2 * q], gr[2 * q + 1], dsc + 96 * wave, lane);
+  __threadfence_block();
+#else
+  for (int q = 0; q < nc; ++q)
+    if (gr[2 * q + 1] > gr[2 * q]) orth_cluster(zf, n, bs, be, rank, gr[2 * q], gr[2 * q + 1], dsc, 0);
+#endif
+  t.sync();
 }
 
 // The bisection + inverse-iteration solver for one matrix: arguments and
@@ -1464,6 +1523,9 @@ EFN_QL_HD inline void heevx_body(const Team& t, const cd* A, int lda, int n, int
     double* sc = invit_scratch(nl, lds, W);
     const bool inz = heevx_extra_bytes(nl) > 0;
     double* zf = W.z;
+    // per-wave rows of 96 doubles for the Gram–Schmidt dots, in red past the
+    // cluster list (2·96 ints)
+    double* dsc = reinterpret_cast<double*>(W.red) + 96;
     double* vd = reinterpret_cast<double*>(V);
     int fails = 0;
     if (inz)
@@ -1489,12 +1551,7 @@ EFN_QL_HD inline void heevx_body(const Team& t, const cd* A, int lda, int n, int
       }
       t.sync();
     }
-    for (int q = t.tid; q < nc; q += t.nthr)
-      if (gr[2 * q + 1] > gr[2 * q]) orth_cluster(zf, n, bs, be, rank, gr[2 * q], gr[2 * q + 1]);
-#ifdef __HIP_DEVICE_COMPILE__
-    __threadfence_block();
-#endif
-    t.sync();
+    orth_all(t, zf, n, bs, be, rank, gr, nc, dsc);
     // members of clusters: one more solve from the orthonormal vector (it
     // damps what the Gram–Schmidt amplified from the other vectors' errors
     // outside the cluster) and Gram–Schmidt again
@@ -1539,12 +1596,7 @@ EFN_QL_HD inline void heevx_body(const Team& t, const cd* A, int lda, int n, int
         }
         t.sync();
       }
-      for (int q = t.tid; q < nc; q += t.nthr)
-        if (gr[2 * q + 1] > gr[2 * q]) orth_cluster(zf, n, bs, be, rank, gr[2 * q], gr[2 * q + 1]);
-#ifdef __HIP_DEVICE_COMPILE__
-      __threadfence_block();
-#endif
-      t.sync();
+      orth_all(t, zf, n, bs, be, rank, gr, nc, dsc);
     }
     for (long idx = t.tid; idx < (long)lda * lowest; idx += t.nthr) {
       const int i = (int)(idx / lowest), r = (int)(idx % lowest);

@@ -644,13 +644,12 @@ def eigh_list_host(mats, lowest=None):
 
 
 SMALL_METHODS = ("heevx", "ql")
-# The device eigensolver for k ≤ 96 when none is named.  On general Hermitian
-# matrices "heevx" is the faster (72², lowest 24: 0.58 ms against 1.95 ms),
-# but LOBPCG's Rayleigh–Ritz matrices put the wanted Ritz values close
-# together relative to the W and P directions' large Rayleigh quotients, so
-# its clusters (and their orthogonalisation) grow; there QL measured faster
-# (C1 as one graph: 0.30 s against 0.33 s; C2: 7.2 s against 8.0 s).
-SMALL_METHOD = "ql"
+# The device eigensolver for k ≤ 96 when none is named: "heevx" measured
+# faster on general Hermitian matrices (72², lowest 24: 0.58 ms against
+# 1.95 ms) and on LOBPCG's Rayleigh–Ritz matrices, whose wanted Ritz values
+# form clusters (0.81 against 1.95 ms; C1 as one graph 0.27 s against 0.30 s,
+# C2 6.8 s against 7.2 s).
+SMALL_METHOD = "heevx"
 
 
 def eigh_ragged_device(A: torch.Tensor, sizes, lowest: int | None = None, method: str | None = None):
