@@ -644,12 +644,14 @@ def eigh_list_host(mats, lowest=None):
 
 
 SMALL_METHODS = ("heevx", "ql")
-# The device eigensolver for k ≤ 96 when none is named: "heevx" measured
+# The device eigensolver for k ≤ 96 when none is named.  "heevx" measured
 # faster on general Hermitian matrices (72², lowest 24: 0.58 ms against
-# 1.95 ms) and on LOBPCG's Rayleigh–Ritz matrices, whose wanted Ritz values
-# form clusters (0.81 against 1.95 ms; C1 as one graph 0.27 s against 0.30 s,
-# C2 6.8 s against 7.2 s).
-SMALL_METHOD = "heevx"
+# 1.95 ms) and on LOBPCG's Rayleigh–Ritz matrices (0.81 against 1.95 ms; C1
+# as one graph 0.27 s against 0.30 s, C2 6.8 s against 7.2 s), but on some of
+# those matrices its lowest-m solve on the device differs from the full
+# solve's leading columns in the last bits (up to 4e-14), which breaks the
+# graph-replay-equals-eager guarantee; QL keeps it.
+SMALL_METHOD = "ql"
 
 
 def eigh_ragged_device(A: torch.Tensor, sizes, lowest: int | None = None, method: str | None = None):
